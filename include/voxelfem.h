@@ -1,0 +1,195 @@
+/*
+ * voxelfem.h -- C ABI of the MI355X-native VoxelFEM solve-and-sensitivity library
+ * (libvoxelfem.so, hand-written HIP kernels for gfx950).
+ *
+ * Every entry point returns VF_OK (0) or a negative status; the message of the last failure on
+ * the calling thread is returned by vf_last_error(). No C++ exception crosses this boundary.
+ *
+ * Data conventions (identical to the reference's, MeshFEM/VoxelFEM):
+ *   - grid node (ix,iy,iz) has flat index (ix*nny + iy)*nnz + iz, element (ex,ey,ez) has
+ *     (ex*ny + ey)*nz + ez                             (NDVector.hh:256-275)
+ *   - element local node l = 4i+2j+k at (ex+i,ey+j,ez+k), element DoF 3l+c
+ *                                                       (TensorProductSimulator.hh:254-278)
+ *   - a nodal vector field of nn nodes is passed as 3 component planes (SoA, 3*nn doubles,
+ *     plane c holds component c): the memory image of the reference's ColMajor VField
+ *                                                       (TensorProductSimulator.hh:180)
+ *   - the build (layer) direction is axis 1               (TensorProductSimulator.hh:1851)
+ * Every vector argument comes with a `where` flag: VF_HOST (0) for host memory, VF_DEVICE (1)
+ * for device memory on the library's device. Device results are produced on the handle's
+ * stream; host results are complete when the call returns.
+ */
+#ifndef VOXELFEM_H
+#define VOXELFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VF_OK 0
+#define VF_ERR_ARG (-1)
+#define VF_ERR_RUNTIME (-2)
+#define VF_ERR_NAN (-3)
+#define VF_ERR_HIP (-4)
+
+#define VF_HOST 0
+#define VF_DEVICE 1
+
+/* interpolation laws (TensorProductSimulator.hh:166 InterpolationLaw) */
+#define VF_SIMP 0
+#define VF_RAMP 1
+
+/* preconditioner precision inside the FP64 PCG */
+#define VF_PREC_FP64 0
+#define VF_PREC_FP32 1
+
+typedef struct vf_sim vf_sim;  /* TensorProductSimulator<double,1,1,1> */
+typedef struct vf_mg vf_mg;    /* MultigridSolver<double,1,1,1> */
+
+/* ---- library -------------------------------------------------------------------------- */
+const char *vf_last_error(void);
+/* version string of the library build */
+const char *vf_version(void);
+/* number of visible devices (0 when no GPU) */
+int vf_device_count(int *count);
+
+/* ---- device memory helpers (no reference counterpart: the reference is host-only). kind:
+ *      0 host->device, 1 device->host, 2 device->device. Copies are ordered on `stream` and
+ *      complete on return when they involve host memory. */
+int vf_malloc(void **ptr, size_t bytes);
+int vf_free(void *ptr);
+int vf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
+int vf_stream_sync(void *stream);
+
+/* ---- simulator: TensorProductSimulator (TensorProductSimulator.hh:209-285, factory
+ *      python_bindings/VoxelFEM.cc:423-430). stream: a hipStream_t or NULL for the null stream. */
+int vf_sim_create(vf_sim **out, const int64_t ne[3], const double minCorner[3], const double maxCorner[3], void *stream);
+int vf_sim_destroy(vf_sim *sim);
+int vf_sim_set_stream(vf_sim *sim, void *stream);
+/* ETensor of an isotropic material (setETensor, TensorProductSimulator.hh:333-347) */
+int vf_sim_set_material(vf_sim *sim, double young, double poisson);
+/* interpolationLaw/E_0/E_min/gamma/q properties (python_bindings/VoxelFEM.cc:115-119) */
+int vf_sim_set_law(vf_sim *sim, int law, double E0, double Emin, double gamma, double q);
+/* setDensities (TensorProductSimulator.hh:784-789) */
+int vf_sim_set_densities(vf_sim *sim, const double *rho, int where);
+int vf_sim_get_densities(vf_sim *sim, double *rho, int where);
+/* getYoungModulusScaleFactor (TensorProductSimulator.hh:968) */
+int vf_sim_get_young(vf_sim *sim, double *E, int where);
+/* fullDensityElementStiffnessMatrix (TensorProductSimulator.hh:1077): 24x24 row-major, host */
+int vf_sim_get_K0(vf_sim *sim, double *K0);
+/* Dirichlet conditions per node (BCBuilder::apply, TensorProductSimulator.hh:511-560):
+ * mask[n] bit c set = component c constrained; values: SoA 3*nn or NULL for zero. Host. */
+int vf_sim_set_dirichlet(vf_sim *sim, const uint8_t *mask, const double *values);
+/* nodal point forces used by buildLoadVector (TensorProductSimulator.hh:1269-1276), SoA, host */
+int vf_sim_set_forces(vf_sim *sim, const double *f);
+/* gravity property (TensorProductSimulator.hh:1776-1777) */
+int vf_sim_set_gravity(vf_sim *sim, const double g[3]);
+/* setFabricationMaskHeightByLayer / decrementFabricationMaskHeightByLayer
+ * (TensorProductSimulator.hh:311-329) */
+int vf_sim_set_mask_layer(vf_sim *sim, int64_t layer);
+int vf_sim_get_mask_info(vf_sim *sim, int64_t *firstMaskedElementLayer, int64_t *firstDetachedNodeLayer, double *height);
+/* applyK (TensorProductSimulator.hh:1410-1438 -> TPSStencils.hh:431-728): y = K u (SoA) */
+int vf_sim_apply_K(vf_sim *sim, const double *u, double *y, int where);
+/* buildLoadVector (TensorProductSimulator.hh:1269-1288) */
+int vf_sim_build_load(vf_sim *sim, double *f, int where);
+/* complianceGradient (TensorProductSimulator.hh:972-1005, 1043-1048) */
+int vf_sim_compliance_gradient(vf_sim *sim, const double *u, double *g, int where);
+/* accumulateComplianceGradient (TensorProductSimulator.hh:1008-1040): g += dC/drho over unmasked layers */
+int vf_sim_accumulate_compliance_gradient(vf_sim *sim, const double *u, double *g, int where);
+/* elementEnergyDensity (TensorProductSimulator.hh:1057-1073) */
+int vf_sim_energy_density(vf_sim *sim, const double *u, double *out, int where);
+/* compliance 1/2 f.u (TopologyOptimizationObjective.hh:41-43); masked=1 restricts to non-detached
+ * nodes and drops the 1/2 (maskedNodalDotProduct, TensorProductSimulator.hh:445-447) */
+int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int where, double *out);
+
+/* ---- multigrid: MultigridSolver (MultigridSolver.hh:35-121, binding VoxelFEM.cc:155-195) */
+int vf_mg_create(vf_mg **out, vf_sim *fine, int64_t numCoarseningLevels);
+int vf_mg_destroy(vf_mg *mg);
+int vf_mg_num_levels(vf_mg *mg, int64_t *levels);
+int vf_mg_level_nodes(vf_mg *mg, int64_t l, int64_t nn[3]);
+int vf_mg_level_dirichlet(vf_mg *mg, int64_t l, uint8_t *mask);
+/* setSymmetricGaussSeidel (MultigridSolver.hh:123-125) */
+int vf_mg_set_symmetric_gs(vf_mg *mg, int symmetric);
+/* updateStiffnessMatrices (MultigridSolver.hh:846-905): Galerkin hierarchy + coarse factorization */
+int vf_mg_update_stiffness(vf_mg *mg);
+/* setFabricationMaskHeightByLayer (MultigridSolver.hh:1022-1028) */
+int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer);
+/* applyK(l, u) (MultigridSolver.hh:471-504) */
+int vf_mg_apply_K(vf_mg *mg, int64_t l, const double *u, double *y, int where);
+/* computeResidual (MultigridSolver.hh:527-541) */
+int vf_mg_residual(vf_mg *mg, int64_t l, const double *u, const double *b, double *r, int where);
+/* one multicolour block Gauss-Seidel sweep (smoothingMulticoloredGS, MultigridSolver.hh:452-458), in place */
+int vf_mg_smooth(vf_mg *mg, int64_t l, double *u, const double *b, int forward, int where);
+/* restriction l -> l+1 (MultigridSolver.hh:214-275) */
+int vf_mg_restrict(vf_mg *mg, int64_t l, const double *fine, double *coarse, int where);
+/* interpolation / accum_interpolation l+1 -> l (MultigridSolver.hh:178-212) */
+int vf_mg_interpolate(vf_mg *mg, int64_t l, const double *coarse, double *fine, int accumulate, int where);
+/* coarsest direct solve (TensorProductSimulator::solve, TensorProductSimulator.hh:1198-1230) */
+int vf_mg_coarse_solve(vf_mg *mg, const double *b, double *x, int where);
+/* zeroOutDirichletComponents(l, u) (MultigridSolver.hh:507-513), in place */
+int vf_mg_zero_dirichlet(vf_mg *mg, int64_t l, double *u, int where);
+/* MultigridSolver::solve (MultigridSolver.hh:546-575): numSteps V-cycles (the first one a full
+ * multigrid cycle when fullMultigrid) from the initial guess x, in place. zeroDirichlet replaces the
+ * Dirichlet values by zero (residual systems). cb(user, i, x) after every cycle (host copy). */
+typedef int (*vf_mg_callback)(void *user, int64_t it, const double *x);
+int vf_mg_solve(vf_mg *mg, double *x, const double *f, int64_t numSteps, int64_t smoothing, int stiffnessUpdated,
+                int zeroDirichlet, int fullMultigrid, vf_mg_callback cb, void *user, int where);
+/* applyPreconditionerInv (MultigridSolver.hh:577-580): s = M^-1 r with mgIterations cycles */
+int vf_mg_precondition(vf_mg *mg, const double *r, double *s, int64_t mgIterations, int64_t smoothing,
+                       int fullMultigrid, int precision, int where);
+
+/* per-iteration callback (it_callback(i, x, r), MultigridSolver.hh:1147-1150); x and r are host
+ * arrays of 3*nn doubles (SoA) valid during the call. Return nonzero to abort the solve. */
+typedef int (*vf_pcg_callback)(void *user, int64_t it, const double *x, const double *r);
+
+typedef struct vf_pcg_stats {
+    int64_t iterations;
+    double residual_norm;     /* ||r|| over non-detached nodes at exit */
+    double rhs_norm;          /* ||b|| over non-detached nodes */
+    double seconds;           /* wall time of the call */
+} vf_pcg_stats;
+
+/* preconditionedConjugateGradient (MultigridSolver.hh:1046-1152), x updated in place.
+ * precision = VF_PREC_FP64 reproduces the reference (Fletcher-Reeves beta, FP64 V-cycle);
+ * VF_PREC_FP32 runs the V-cycle in FP32 (coarse solve FP64) inside an FP64 flexible PCG
+ * (Polak-Ribiere beta). Returns VF_ERR_NAN with "NaN encountered" like the reference. */
+int vf_mg_pcg(vf_mg *mg, double *x, const double *b, int64_t maxIter, double tol, int64_t mgIterations,
+              int64_t smoothing, int fullMultigrid, int precision, int dirichletAlreadySatisfied,
+              vf_pcg_callback cb, void *user, int where, vf_pcg_stats *stats);
+
+/* LayerByLayerEvaluator::run with the "zero" initial guess (LayerByLayer.hh:223-300): every layer
+ * solve starts from zero, so the solves are independent. objective = 1/2 sum_l f_l.u_l / #layers,
+ * grad = sum_l g_l / #layers (LayerByLayer.hh:299-300). perLayer arrays (optional, host) receive
+ * f_l.u_l and the PCG iteration counts, top layer first. */
+int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                    int64_t smoothing, int fullMultigrid, int precision, double *objective, double *grad,
+                    int gradWhere, double *perLayerCompliance, int64_t *perLayerIterations);
+
+/* ---- filters and optimiser kernels (TopologyOptimizationFilter.hh, OptimalityCriterion.hh) --- */
+/* SmoothingFilter::apply == backprop (TopologyOptimizationFilter.hh:285-400). rmin is the kernel
+ * radius of the linear hat: the reference's integer `radius` r corresponds to rmin = r + 1.
+ * type 0 = Const, 1 = Linear. dims: element grid (3). */
+int vf_smoothing_filter(const int64_t dims[3], double rmin, int type, const double *in, double *out, int where,
+                        void *stream);
+/* ProjectionFilter::apply / backprop (TopologyOptimizationFilter.hh:199-225) */
+int vf_projection(double beta, int64_t n, const double *in, double *out, int where, void *stream);
+int vf_projection_backprop(double beta, int64_t n, const double *g, const double *vars, double *out, int where,
+                           void *stream);
+/* OCOptimizer::step elementwise update (OptimalityCriterion.hh:64-83) */
+int vf_oc_update(int64_t n, const double *x0, const double *dJ, const double *dc, double lambda, double move,
+                 double p, double *out, int where, void *stream);
+/* mean of n values (TotalVolumeConstraint::evaluate, TopologyOptimizationConstraint.hh:30-32) */
+int vf_mean(int64_t n, const double *x, int where, void *stream, double *out);
+
+/* ---- measurement helpers (no reference counterpart: used by bench.py) ---------------------- */
+/* Launch `reps` fine-level K u products of the simulator on its stream and return the average
+ * kernel time measured with HIP events recorded on that stream. precision: VF_PREC_FP64/FP32. */
+int vf_bench_apply_K(vf_sim *sim, int64_t reps, int precision, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VOXELFEM_H */

@@ -1,0 +1,57 @@
+"""ctypes view of the C ABI (include/voxelfem.h) for callers that keep data resident on the device
+(torch tensors, bench.py). The library is the one pyVoxelFEM already loaded."""
+import ctypes
+
+from . import LIBRARY_PATH
+
+VF_HOST, VF_DEVICE = 0, 1
+VF_PREC_FP64, VF_PREC_FP32 = 0, 1
+
+_lib = None
+
+
+class PcgStats(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int64), ("residual_norm", ctypes.c_double),
+                ("rhs_norm", ctypes.c_double), ("seconds", ctypes.c_double)]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIBRARY_PATH)
+        L.vf_last_error.restype = ctypes.c_char_p
+        L.vf_version.restype = ctypes.c_char_p
+        vp, i64, dbl, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
+        L.vf_mg_pcg.argtypes = [vp, vp, vp, i64, dbl, i64, i64, i32, i32, i32, vp, vp, i32, ctypes.POINTER(PcgStats)]
+        L.vf_bench_apply_K.argtypes = [vp, i64, i32, ctypes.POINTER(dbl)]
+        L.vf_sim_set_densities.argtypes = [vp, vp, i32]
+        L.vf_sim_build_load.argtypes = [vp, vp, i32]
+        L.vf_sim_compliance_gradient.argtypes = [vp, vp, vp, i32]
+        L.vf_sim_dot.argtypes = [vp, vp, vp, i32, i32, ctypes.POINTER(dbl)]
+        L.vf_smoothing_filter.argtypes = [ctypes.POINTER(i64), dbl, i32, vp, vp, i32, vp]
+        L.vf_projection.argtypes = [dbl, i64, vp, vp, i32, vp]
+        L.vf_projection_backprop.argtypes = [dbl, i64, vp, vp, vp, i32, vp]
+        L.vf_mg_update_stiffness.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().vf_last_error().decode())
+
+
+def pcg(mg_handle, x_ptr, b_ptr, max_iter, tol, mg_iterations=1, smoothing=1, fmg=False, precision=VF_PREC_FP64,
+        dirichlet_satisfied=False):
+    st = PcgStats()
+    check(lib().vf_mg_pcg(ctypes.c_void_p(mg_handle), ctypes.c_void_p(x_ptr), ctypes.c_void_p(b_ptr),
+                          int(max_iter), float(tol), int(mg_iterations), int(smoothing), int(bool(fmg)),
+                          int(precision), int(bool(dirichlet_satisfied)), None, None, VF_DEVICE,
+                          ctypes.byref(st)))
+    return st
+
+
+def bench_apply_K(sim_handle, reps, precision=VF_PREC_FP64):
+    ms = ctypes.c_double()
+    check(lib().vf_bench_apply_K(ctypes.c_void_p(sim_handle), int(reps), int(precision), ctypes.byref(ms)))
+    return ms.value

@@ -1,0 +1,1425 @@
+// vf_host.hip -- host orchestration of the MI355X solve-and-sensitivity path and the C ABI
+// declared in include/voxelfem.h.
+//
+// The simulator (vf_sim) keeps the fine grid state resident in HBM: densities, Young's moduli in
+// FP64 and FP32, the element matrix K0, Dirichlet masks/lists and the load data. The multigrid
+// solver (vf_mg) owns the coarsened hierarchy (Galerkin element matrices, 27-point block rows,
+// the FP64 inverse of the reduced coarsest operator) and all level work buffers, and runs the
+// V-cycle / full multigrid preconditioner and the PCG iteration entirely on the device; the host
+// reads back one scalar per PCG iteration for the stopping test.
+#include "vf_internal.hpp"
+#include "../../include/voxelfem.h"
+
+#include <algorithm>
+#include <chrono>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <memory>
+
+using namespace vf;
+
+namespace {
+
+thread_local std::string g_err;
+constexpr int64_t LAYER_NONE = INT64_MAX;
+
+void sync(hipStream_t s) { VF_HIP(hipStreamSynchronize(s)); }
+
+template <typename T>
+void h2d(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n) VF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+template <typename T>
+void d2h(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n) VF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, s));
+}
+template <typename T>
+void d2d(T *dst, const T *src, size_t n, hipStream_t s) {
+    if (n && dst != src) VF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
+}
+
+// ---- element matrices (host, once per material/stretch change) -------------------------------
+// K0 = int eps(phi_a e_c) : C : eps(phi_b e_d) over the box element (TensorProductSimulator.hh:67-80)
+// evaluated in closed form from the 1D integrals of the Q1 basis on [0, h]:
+//   mass M = h [[1/3,1/6],[1/6,1/3]], stiffness D = (1/h) [[1,-1],[-1,1]], mixed G = [[-1/2,-1/2],[1/2,1/2]].
+void element_K0(double young, double nu, const double h[3], double K0[576]) {
+    const double lam = young * nu / ((1 + nu) * (1 - 2 * nu));
+    const double mu = young / (2 * (1 + nu));
+    auto Mm = [&](int d, int a, int b) { return h[d] * (a == b ? 1.0 / 3.0 : 1.0 / 6.0); };
+    auto Dm = [&](int d, int a, int b) { return (a == b ? 1.0 : -1.0) / h[d]; };
+    auto Gm = [&](int a, int b) { (void)b; return a ? 0.5 : -0.5; };  // int N_a' N_b
+    // I(p,q)[a][b] = int d_p phi_a d_q phi_b
+    auto I = [&](int p, int q, const int ia[3], const int ib[3]) {
+        double v = 1;
+        for (int d = 0; d < 3; ++d) {
+            if (d == p && d == q) v *= Dm(d, ia[d], ib[d]);
+            else if (d == p) v *= Gm(ia[d], ib[d]);
+            else if (d == q) v *= Gm(ib[d], ia[d]);
+            else v *= Mm(d, ia[d], ib[d]);
+        }
+        return v;
+    };
+    for (int a = 0; a < 8; ++a) {
+        const int ia[3] = {(a >> 2) & 1, (a >> 1) & 1, a & 1};
+        for (int b = 0; b < 8; ++b) {
+            const int ib[3] = {(b >> 2) & 1, (b >> 1) & 1, b & 1};
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d) {
+                    double v = lam * I(c, d, ia, ib) + mu * I(d, c, ia, ib);
+                    if (c == d) v += mu * (I(0, 0, ia, ib) + I(1, 1, ia, ib) + I(2, 2, ia, ib));
+                    K0[(3 * a + c) * 24 + 3 * b + d] = v;
+                }
+        }
+    }
+    for (int i = 0; i < 24; ++i)  // exact symmetry
+        for (int j = 0; j < i; ++j) K0[j * 24 + i] = K0[i * 24 + j];
+}
+
+// phi[fi][lf][lc]: coarse Q1 shape function lc evaluated at fine node lf of child fi
+void child_phi(double phi[512]) {
+    for (int fi = 0; fi < 8; ++fi)
+        for (int lf = 0; lf < 8; ++lf)
+            for (int lc = 0; lc < 8; ++lc) {
+                double v = 1;
+                for (int d = 0; d < 3; ++d) {
+                    const int sh = 2 - d;
+                    const double x = 0.5 * (((fi >> sh) & 1) + ((lf >> sh) & 1));
+                    v *= ((lc >> sh) & 1) ? x : 1 - x;
+                }
+                phi[fi * 64 + lf * 8 + lc] = v;
+            }
+}
+
+// out += P_fi^T K P_fi with P[3 lf + c][3 lc + c] = phi[fi][lf][lc]
+void coarsen_into(const double *phi, int fi, const double *K, double *out) {
+    double T[576];
+    for (int a = 0; a < 24; ++a)
+        for (int j = 0; j < 8; ++j)
+            for (int d = 0; d < 3; ++d) {
+                double s = 0;
+                for (int i = 0; i < 8; ++i) s += K[a * 24 + 3 * i + d] * phi[fi * 64 + i * 8 + j];
+                T[a * 24 + 3 * j + d] = s;
+            }
+    for (int i2 = 0; i2 < 8; ++i2)
+        for (int c = 0; c < 3; ++c)
+            for (int b = 0; b < 24; ++b) {
+                double s = 0;
+                for (int i = 0; i < 8; ++i) s += phi[fi * 64 + i * 8 + i2] * T[(3 * i + c) * 24 + b];
+                out[(3 * i2 + c) * 24 + b] += s;
+            }
+}
+
+}  // namespace
+
+// =================================================================================================
+struct vf_sim {
+    hipStream_t stream = nullptr;
+    int64_t ne[3];
+    double lo[3], hi[3], h[3];
+    Grid g;
+    double young = 1, poisson = 0;  // ETensor default E=1, nu=0 (TensorProductSimulator.hh:2114)
+    int law = VF_SIMP;
+    double E0 = 1, Emin = 1e-4, gamma = 3, q = 3;  // TensorProductSimulator.hh:2137-2142
+    double grav[3] = {0, 0, 0};
+    double K0h[576];
+    DBuf<double> K0;
+    DBuf<float> K032;
+    DBuf<double> rho, E;
+    DBuf<float> E32;
+    std::vector<uint8_t> hmask;
+    std::vector<double> hvals;
+    DBuf<uint8_t> dmask;
+    DBuf<int> dnodes;
+    DBuf<uint8_t> dnmask;
+    DBuf<double> dvals;
+    int ndir = 0;
+    std::vector<double> hforce;
+    DBuf<double> force;
+    bool hasForce = false;
+    int64_t firstMasked = LAYER_NONE, firstDetached = LAYER_NONE;
+    double maskHeight = std::numeric_limits<double>::infinity();
+    DBuf<double> s_in, s_out, s_aux, partials, scal;
+    DBuf<float> s32a, s32b;
+    uint64_t version = 0;  // bumped whenever E changes
+
+    size_t nn() const { return (size_t)g.nnodes; }
+    size_t nel() const { return (size_t)g.nelems; }
+
+    void refresh_grid() {
+        g.ydet = (int)std::min<int64_t>(firstDetached, g.nn[1]);
+        g.ymask = (int)std::min<int64_t>(firstMasked, g.ne[1]);
+    }
+    void update_K0() {
+        element_K0(young, poisson, h, K0h);
+        float f[576];
+        for (int i = 0; i < 576; ++i) f[i] = (float)K0h[i];
+        h2d(K0.data(), K0h, 576, stream);
+        h2d(K032.data(), f, 576, stream);
+        sync(stream);
+    }
+    void update_E() {
+        refresh_grid();
+        launch_young(g, rho.data(), E.data(), E32.data(), law, E0, Emin, gamma, q, maskHeight < hi[1] ? 1 : 0, stream);
+        ++version;
+    }
+    // setFabricationMaskHeight (TensorProductSimulator.hh:286-310)
+    void set_mask_height(double H, bool updateYoung) {
+        if (H < 0 || H > hi[1])
+            throw std::runtime_error("Fabrication height (" + std::to_string(H) + ") has to be in between 0 and " +
+                                     std::to_string(hi[1]));
+        maskHeight = H;
+        firstMasked = (int64_t)std::ceil(maskHeight / h[1] - 1e-10);
+        firstDetached = firstMasked + 1;
+        refresh_grid();
+        if (updateYoung) update_E();
+    }
+    double *stage_in(size_t n) {
+        s_in.resize(std::max(s_in.size(), n));
+        return s_in.data();
+    }
+    double *stage_out(size_t n) {
+        s_out.resize(std::max(s_out.size(), n));
+        return s_out.data();
+    }
+    double *stage_aux(size_t n) {
+        s_aux.resize(std::max(s_aux.size(), n));
+        return s_aux.data();
+    }
+    double *red_partials(size_t n) {
+        partials.resize(std::max(partials.size(), std::max<size_t>(n, 64)));
+        return partials.data();
+    }
+    double *scalars() {
+        if (scal.empty()) scal.resize(16);
+        return scal.data();
+    }
+    double read_scalar(const double *dev) {
+        double v;
+        d2h(&v, dev, 1, stream);
+        sync(stream);
+        return v;
+    }
+};
+
+namespace {
+
+struct LevelData {
+    Grid g;
+    double h[3];
+    std::vector<uint8_t> hmask;
+    DBuf<uint8_t> dmask;
+    DBuf<int> dnodes;
+    DBuf<uint8_t> dnmask;
+    int ndir = 0;
+    DBuf<double> x, b, r;
+    DBuf<float> x32, b32, r32;
+    DBuf<double> blocks;
+    DBuf<float> blocks32;
+    DBuf<double> Ke;
+};
+
+void upload_dirichlet(const std::vector<uint8_t> &mask, DBuf<uint8_t> &dmask, DBuf<int> &dnodes,
+                      DBuf<uint8_t> &dnmask, int &ndir, hipStream_t s) {
+    std::vector<int> nodes;
+    std::vector<uint8_t> m;
+    for (size_t n = 0; n < mask.size(); ++n)
+        if (mask[n] & 7) { nodes.push_back((int)n); m.push_back(mask[n] & 7); }
+    ndir = (int)nodes.size();
+    dmask.resize(mask.size());
+    h2d(dmask.data(), mask.data(), mask.size(), s);
+    dnodes.resize(std::max<size_t>(nodes.size(), 1));
+    dnmask.resize(std::max<size_t>(m.size(), 1));
+    h2d(dnodes.data(), nodes.data(), nodes.size(), s);
+    h2d(dnmask.data(), m.data(), m.size(), s);
+    sync(s);
+}
+
+}  // namespace
+
+struct vf_mg {
+    vf_sim *sim;
+    int L;
+    std::vector<std::unique_ptr<LevelData>> lv;  // lv[0] mirrors the fine simulator
+    double phi[512];
+    double Chat[8 * 576];
+    DBuf<double> dChat, dphi, dM2, dKe0;
+    DBuf<float> dChat32;
+    bool symmetricGS = true;
+    // coarsest operator
+    DBuf<double> Ainv, invScratch, ctmp_in, ctmp_out;
+    DBuf<int> dRedToFull, dRed;
+    int nred = 0, ldA = 0;
+    bool stiffnessReady = false;
+    uint64_t stiffnessVersion = 0;
+    // PCG work
+    DBuf<double> pd, pAd, ps, pr, px, pb;
+    // level-0 views used by the cycles (set per preconditioner call)
+    double *x0v = nullptr, *b0v = nullptr;
+
+    hipStream_t st() const { return sim->stream; }
+
+    void sync_masks() {
+        sim->refresh_grid();
+        lv[0]->g = sim->g;
+        for (int l = 1; l <= L; ++l) {
+            Grid &g = lv[l]->g;
+            if (std::isinf(sim->maskHeight)) {
+                g.ydet = g.nn[1];
+                g.ymask = g.ne[1];
+            } else {
+                const int64_t fm = (int64_t)std::ceil(sim->maskHeight / lv[l]->h[1] - 1e-10);
+                g.ymask = (int)std::min<int64_t>(fm, g.ne[1]);
+                g.ydet = (int)std::min<int64_t>(fm + 1, g.nn[1]);
+            }
+        }
+    }
+
+    vf_mg(vf_sim *s, int levels) : sim(s), L(levels) {
+        if (levels < 0) throw std::runtime_error("numCoarseningLevels must be >= 0");
+        auto l0 = std::make_unique<LevelData>();
+        l0->g = sim->g;
+        std::copy(sim->h, sim->h + 3, l0->h);
+        l0->hmask = sim->hmask;
+        lv.push_back(std::move(l0));
+        int64_t ne[3] = {sim->ne[0], sim->ne[1], sim->ne[2]};
+        for (int l = 1; l <= L; ++l) {
+            for (int d = 0; d < 3; ++d) {
+                if (ne[d] % 2)
+                    throw std::runtime_error(
+                        "Grid size currently must be divisible by 2^numCoarseningLevels (nonuniform coarsening not yet "
+                        "implemented)");
+                ne[d] /= 2;
+            }
+            auto c = std::make_unique<LevelData>();
+            c->g = make_grid(ne);
+            for (int d = 0; d < 3; ++d) c->h[d] = (sim->hi[d] - sim->lo[d]) / ne[d];
+            // Dirichlet coarsening (MultigridSolver.hh:58-103): along an axis where the constrained
+            // fine node index is even the coarse node is i/2; where it is odd the fine node lies
+            // inside a coarse edge/face/cell and both neighbours (i-1)/2, (i+1)/2 are constrained.
+            const LevelData &f = *lv.back();
+            c->hmask.assign(c->g.nnodes, 0);
+            const int fnz = f.g.nn[2], fny = f.g.nn[1];
+            for (int n = 0; n < f.g.nnodes; ++n) {
+                const uint8_t m = f.hmask[n];
+                if (!m) continue;
+                const int id[3] = {n / (fny * fnz), (n / fnz) % fny, n % fnz};
+                int a0[3], a1[3];
+                for (int d = 0; d < 3; ++d) {
+                    if (id[d] % 2 == 0) a0[d] = a1[d] = id[d] / 2;
+                    else { a0[d] = (id[d] - 1) / 2; a1[d] = (id[d] + 1) / 2; }
+                }
+                for (int x = a0[0]; x <= a1[0]; ++x)
+                    for (int y = a0[1]; y <= a1[1]; ++y)
+                        for (int z = a0[2]; z <= a1[2]; ++z)
+                            c->hmask[(x * c->g.nn[1] + y) * c->g.nn[2] + z] |= m;
+            }
+            upload_dirichlet(c->hmask, c->dmask, c->dnodes, c->dnmask, c->ndir, st());
+            lv.push_back(std::move(c));
+        }
+        // coarsened fine K0s (MultigridSolver.hh:116-120) and the level-2 path matrices
+        child_phi(phi);
+        std::fill(Chat, Chat + 8 * 576, 0.0);
+        for (int fi = 0; fi < 8; ++fi) coarsen_into(phi, fi, sim->K0h, Chat + fi * 576);
+        dChat.resize(8 * 576);
+        dChat32.resize(8 * 576);
+        dphi.resize(512);
+        std::vector<float> c32(Chat, Chat + 8 * 576);
+        h2d(dChat.data(), Chat, 8 * 576, st());
+        h2d(dChat32.data(), c32.data(), 8 * 576, st());
+        h2d(dphi.data(), phi, 512, st());
+        if (L >= 2) {
+            std::vector<double> M2(64 * 576, 0.0);
+            for (int f2 = 0; f2 < 8; ++f2)
+                for (int f1 = 0; f1 < 8; ++f1) coarsen_into(phi, f2, Chat + f1 * 576, &M2[(8 * f2 + f1) * 576]);
+            dM2.resize(M2.size());
+            h2d(dM2.data(), M2.data(), M2.size(), st());
+        }
+        sync(st());
+        sync_masks();
+    }
+
+    // ---------------------------------------------------------------- buffers
+    template <typename T>
+    T *X(int l);
+    template <typename T>
+    T *B(int l);
+    template <typename T>
+    T *R(int l);
+    template <typename T>
+    void ensure_buffers(int from) {
+        for (int l = from; l <= L; ++l) {
+            LevelData &d = *lv[l];
+            const size_t n = 3 * (size_t)d.g.nnodes;
+            if constexpr (std::is_same<T, double>::value) {
+                d.x.resize(n); d.b.resize(n); d.r.resize(n);
+            } else {
+                d.x32.resize(n); d.b32.resize(n); d.r32.resize(n);
+            }
+        }
+    }
+
+    const uint8_t *dm(int l) const { return l == 0 ? sim->dmask.data() : lv[l]->dmask.data(); }
+    template <typename T>
+    void zero_dirichlet(int l, T *u) {
+        if (l == 0) launch_zero_dirichlet<T>(sim->ndir, sim->dnodes.data(), sim->dnmask.data(), sim->g.nnodes, u, st());
+        else launch_zero_dirichlet<T>(lv[l]->ndir, lv[l]->dnodes.data(), lv[l]->dnmask.data(), lv[l]->g.nnodes, u, st());
+    }
+
+    // ---------------------------------------------------------------- level operators
+    template <typename T>
+    const T *K0p() const;
+    template <typename T>
+    const T *Ep() const;
+    template <typename T>
+    const T *Chatp() const;
+    template <typename T>
+    const T *blocksp(int l) const;
+
+    template <typename T>
+    void apply(int l, const T *u, T *y, const T *b, int mode, bool useMask) {
+        const uint8_t *mask = useMask ? dm(l) : nullptr;
+        if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
+        else if (l == 1) launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
+        else if (l < L) launch_block_apply<T>(lv[l]->g, blocksp<T>(l), u, y, b, mask, mode, st());
+        else {
+            if constexpr (std::is_same<T, double>::value) {
+                if (mode == AM_RESIDUAL) {
+                    d2d(y, b, 3 * (size_t)lv[l]->g.nnodes, st());
+                    launch_elem_apply<double>(lv[l]->g, lv[l]->Ke.data(), u, y, AM_SUB, st());
+                } else {
+                    launch_elem_apply<double>(lv[l]->g, lv[l]->Ke.data(), u, y, mode, st());
+                }
+                if (useMask) zero_dirichlet<double>(l, y);
+            } else {
+                throw std::runtime_error("coarsest-level operator is FP64 only");
+            }
+        }
+    }
+    template <typename T>
+    void smooth(int l, T *u, const T *b, bool forward) {
+        for (int t = 0; t < 8; ++t) {
+            const int color = forward ? t : 7 - t;
+            if (l == 0) launch_fine_gs<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
+            else if (l == 1) launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, st());
+            else launch_block_gs<T>(lv[l]->g, blocksp<T>(l), u, b, dm(l), color, forward, st());
+        }
+    }
+    template <typename T>
+    void coarse_solve(const T *b, T *x) {
+        ensure_factor();
+        launch_coarse_apply<T>(nred, Ainv.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
+                               ctmp_out.data(), st());
+    }
+
+    // ---------------------------------------------------------------- cycles (MultigridSolver.hh:587-658)
+    // enforceDirichletConditions(l, x, residualSystem) (MultigridSolver.hh:516-524): coarse levels
+    // always solve for corrections with zero Dirichlet values.
+    template <typename T>
+    void dirichlet_for_cycle(int l, T *x, bool residual) {
+        if (l == 0 && !residual) {
+            if constexpr (std::is_same<T, double>::value)
+                launch_enforce_dirichlet(sim->ndir, sim->dnodes.data(), sim->dnmask.data(), sim->g.nnodes,
+                                         sim->dvals.data(), x, st());
+            else throw std::runtime_error("non-residual cycles run in FP64");
+        } else {
+            zero_dirichlet<T>(l, x);
+        }
+    }
+    template <typename T>
+    void vcycle(int l, int nu, bool residual = true) {
+        if (l == L) { coarse_solve<T>(B<T>(l), X<T>(l)); return; }
+        T *x = X<T>(l);
+        const T *b = B<T>(l);
+        dirichlet_for_cycle<T>(l, x, residual);
+        for (int i = 0; i < nu; ++i) smooth<T>(l, x, b, true);
+        apply<T>(l, x, R<T>(l), b, AM_RESIDUAL, true);
+        launch_restrict<T>(lv[l]->g, lv[l + 1]->g, R<T>(l), B<T>(l + 1), st());
+        launch_fill<T>(3 * (size_t)lv[l + 1]->g.nnodes, X<T>(l + 1), T(0), st());
+        vcycle<T>(l + 1, nu);
+        launch_interpolate<T>(lv[l]->g, lv[l + 1]->g, X<T>(l + 1), x, 1, st());
+        for (int i = 0; i < nu; ++i) smooth<T>(l, x, b, !symmetricGS);
+    }
+    template <typename T>
+    void fmg(int l, int nu, bool residual = true) {
+        if (l == L) { coarse_solve<T>(B<T>(l), X<T>(l)); return; }
+        launch_restrict<T>(lv[l]->g, lv[l + 1]->g, B<T>(l), B<T>(l + 1), st());
+        fmg<T>(l + 1, nu, residual);
+        launch_interpolate<T>(lv[l]->g, lv[l + 1]->g, X<T>(l + 1), X<T>(l), 0, st());
+        vcycle<T>(l, nu, residual);
+    }
+
+    // MultigridSolver::solve (MultigridSolver.hh:546-575) in FP64 on device vectors
+    void solve(double *x, const double *f, int numSteps, int nu, bool stiffnessUpdated, bool zeroDirichlet,
+               bool fullMG, vf_mg_callback cb, void *user) {
+        sync_masks();
+        if (!stiffnessUpdated) update_stiffness();
+        else ensure_stiffness();
+        if (numSteps == 0) return;
+        const size_t n0 = 3 * (size_t)lv[0]->g.nnodes;
+        std::vector<double> hx;
+        if (L == 0) {
+            coarse_solve<double>(f, x);
+            if (cb) {
+                hx.resize(n0); d2h(hx.data(), x, n0, st()); sync(st());
+                if (cb(user, 0, hx.data())) throw std::runtime_error("solve aborted by callback");
+            }
+            return;
+        }
+        ensure_buffers<double>(1);
+        b0v = const_cast<double *>(f);
+        x0v = x;
+        for (int i = 0; i < numSteps; ++i) {
+            if (fullMG && i == 0) fmg<double>(0, nu, zeroDirichlet);
+            else vcycle<double>(0, nu, zeroDirichlet);
+            if (cb) {
+                hx.resize(n0); d2h(hx.data(), x, n0, st()); sync(st());
+                if (cb(user, i, hx.data())) { b0v = x0v = nullptr; throw std::runtime_error("solve aborted by callback"); }
+            }
+        }
+        b0v = x0v = nullptr;
+    }
+
+    // applyPreconditionerInv (MultigridSolver.hh:577-580) on device vectors (SoA, fine level)
+    void precondition(const double *r, double *s, int mgIter, int nu, bool fullMG, int prec) {
+        sync_masks();
+        const size_t n0 = 3 * (size_t)lv[0]->g.nnodes;
+        if (nu == 0 || mgIter == 0) { d2d(s, r, n0, st()); return; }
+        if (L == 0) { coarse_solve<double>(r, s); return; }
+        ensure_stiffness();
+        if (prec == VF_PREC_FP64) {
+            ensure_buffers<double>(1);
+            b0v = const_cast<double *>(r);
+            x0v = s;
+            launch_fill<double>(n0, s, 0.0, st());
+            for (int i = 0; i < mgIter; ++i) {
+                if (fullMG && i == 0) fmg<double>(0, nu);
+                else vcycle<double>(0, nu);
+            }
+        } else {
+            ensure_buffers<float>(0);
+            launch_convert<float, double>(n0, r, lv[0]->b32.data(), st());
+            launch_fill<float>(n0, lv[0]->x32.data(), 0.0f, st());
+            for (int i = 0; i < mgIter; ++i) {
+                if (fullMG && i == 0) fmg<float>(0, nu);
+                else vcycle<float>(0, nu);
+            }
+            launch_convert<double, float>(n0, lv[0]->x32.data(), s, st());
+        }
+        b0v = x0v = nullptr;
+    }
+
+    // ---------------------------------------------------------------- Galerkin setup
+    void ensure_stiffness() {
+        if (!stiffnessReady || stiffnessVersion != sim->version) update_stiffness();
+    }
+    // updateStiffnessMatrices (MultigridSolver.hh:846-905)
+    void update_stiffness() {
+        sync_masks();
+        hipStream_t s = st();
+        if (L == 0) {
+            LevelData &c = *lv[0];
+            c.Ke.resize((size_t)c.g.nelems * 576);
+            // Ke = E_e K0: reuse the level-1 kernel shape via a scaled copy on host-sized grids
+            std::vector<double> E(c.g.nelems), Ke((size_t)c.g.nelems * 576);
+            d2h(E.data(), sim->E.data(), E.size(), s);
+            sync(s);
+            for (int e = 0; e < c.g.nelems; ++e)
+                for (int k = 0; k < 576; ++k) Ke[(size_t)e * 576 + k] = E[e] * sim->K0h[k];
+            h2d(c.Ke.data(), Ke.data(), Ke.size(), s);
+        } else if (L == 1) {
+            LevelData &c = *lv[1];
+            c.Ke.resize((size_t)c.g.nelems * 576);
+            launch_galerkin_l1_elems(c.g, lv[0]->g, dChat.data(), sim->E.data(), c.Ke.data(), s);
+        } else {
+            LevelData &c2 = *lv[2];
+            c2.Ke.resize((size_t)c2.g.nelems * 576);
+            launch_galerkin_l2_elems(c2.g, lv[0]->g, dM2.data(), sim->E.data(), c2.Ke.data(), s);
+            for (int l = 3; l <= L; ++l) {
+                LevelData &c = *lv[l];
+                c.Ke.resize((size_t)c.g.nelems * 576);
+                launch_galerkin_recurse(c.g, lv[l - 1]->g, dphi.data(), lv[l - 1]->Ke.data(), c.Ke.data(), s);
+            }
+            for (int l = 2; l < L; ++l) {
+                LevelData &c = *lv[l];
+                const size_t nb = (size_t)c.g.nnodes * 243;
+                c.blocks.resize(nb);
+                c.blocks32.resize(nb);
+                launch_assemble_blocks(c.g, c.Ke.data(), c.blocks.data(), s);
+                launch_convert<float, double>(nb, c.blocks.data(), c.blocks32.data(), s);
+            }
+        }
+        factor_coarsest();
+        stiffnessReady = true;
+        stiffnessVersion = sim->version;
+    }
+    void ensure_factor() {
+        if (!stiffnessReady) update_stiffness();
+    }
+    // reduced coarsest operator: Dirichlet components and detached nodes removed
+    // (findFixedVars, TensorProductSimulator.hh:1181-1195), then its FP64 inverse.
+    void factor_coarsest() {
+        LevelData &c = *lv[L];
+        hipStream_t s = st();
+        const Grid &g = c.g;
+        const std::vector<uint8_t> &mask = (L == 0) ? sim->hmask : c.hmask;
+        if (L == 0) {
+            for (int n = 0; n < g.nnodes; ++n)
+                for (int k = 0; k < 3; ++k)
+                    if ((mask[n] >> k) & 1)
+                        if (sim->hvals[(size_t)k * g.nnodes + n] != 0)
+                            throw std::runtime_error("Nonzero Dirichlet constraints currently unsupported");
+        }
+        std::vector<int> red(3 * (size_t)g.nnodes, -1), redToFull;
+        for (int n = 0; n < g.nnodes; ++n) {
+            const int iy = (n / g.nn[2]) % g.nn[1];
+            for (int k = 0; k < 3; ++k) {
+                const bool fixed = ((mask[n] >> k) & 1) || iy >= g.ydet;
+                if (!fixed) { red[3 * n + k] = (int)redToFull.size(); redToFull.push_back(3 * n + k); }
+            }
+        }
+        nred = (int)redToFull.size();
+        ldA = std::max(64, (nred + 63) / 64 * 64);
+        // the coarsest operator is held as a dense FP64 inverse: bound its size (24576^2 doubles = 4.8 GB)
+        if (ldA > 24576)
+            throw std::runtime_error("direct solve of " + std::to_string(nred) +
+                                     " unknowns exceeds the dense coarse-solver limit (24576); use more coarsening "
+                                     "levels");
+        dRed.resize(red.size());
+        dRedToFull.resize(std::max<size_t>(redToFull.size(), 1));
+        h2d(dRed.data(), red.data(), red.size(), s);
+        h2d(dRedToFull.data(), redToFull.data(), redToFull.size(), s);
+        Ainv.resize((size_t)ldA * ldA);
+        invScratch.resize(64 * 64 + 3 * (size_t)64 * ldA);
+        ctmp_in.resize(ldA);
+        ctmp_out.resize(ldA);
+        launch_assemble_dense(g, c.Ke.data(), dRed.data(), ldA, Ainv.data(), s);
+        spd_inverse_inplace(nred, ldA, Ainv.data(), invScratch.data(), s);
+    }
+
+    // ---------------------------------------------------------------- PCG (MultigridSolver.hh:1046-1152)
+    int64_t pcg(double *x, const double *b, int64_t maxIter, double tol, int mgIter, int nu, bool fullMG, int prec,
+                bool dirichletSatisfied, vf_pcg_callback cb, void *user, vf_pcg_stats *stats) {
+        const auto t0 = std::chrono::steady_clock::now();
+        sync_masks();
+        hipStream_t s = st();
+        const Grid &g = lv[0]->g;
+        const size_t n0 = 3 * (size_t)g.nnodes;
+        pr.resize(n0); ps.resize(n0); pd.resize(n0); pAd.resize(n0);
+        double *r = pr.data(), *sv = ps.data(), *d = pd.data(), *Ad = pAd.data();
+        double *sc = sim->scalars();
+        double *part = sim->red_partials(2 * (size_t)reduction_partials(g) + 2);
+        std::vector<double> hx, hr;
+        auto callback = [&](int64_t it) {
+            if (!cb) return;
+            hx.resize(n0); hr.resize(n0);
+            d2h(hx.data(), x, n0, s);
+            d2h(hr.data(), r, n0, s);
+            sync(s);
+            if (cb(user, it, hx.data(), hr.data())) throw std::runtime_error("PCG aborted by callback");
+        };
+        if (L == 0) {
+            // direct solve on the fine grid (MultigridSolver.hh:1053-1062); norms over all nodes
+            launch_fill<double>(n0, r, 0.0, s);
+            apply<double>(0, x, r, b, AM_RESIDUAL, true);
+            launch_dot_full(n0, r, r, part, sc + 4, s);
+            const double rr = sim->read_scalar(sc + 4);
+            launch_dot_full(n0, b, b, part, sc + 7, s);
+            const double bb = sim->read_scalar(sc + 7);
+            if (stats) { stats->iterations = 0; stats->residual_norm = std::sqrt(rr); stats->rhs_norm = std::sqrt(bb); }
+            if (rr < tol * tol * bb) return 0;
+            update_stiffness();
+            coarse_solve<double>(b, x);
+            apply<double>(0, x, r, b, AM_RESIDUAL, true);
+            callback(1);
+            if (stats) stats->iterations = 1;
+            return 1;
+        }
+        if (!dirichletSatisfied)
+            launch_enforce_dirichlet(sim->ndir, sim->dnodes.data(), sim->dnmask.data(), g.nnodes, sim->dvals.data(), x, s);
+        launch_dot_active(g, b, b, part, sc + 7, s);
+        launch_fill<double>(n0, r, 0.0, s);
+        apply<double>(0, x, r, b, AM_RESIDUAL, true);
+        launch_dot_active(g, r, r, part, sc + 4, s);
+        const double bb = sim->read_scalar(sc + 7);
+        double rr = sim->read_scalar(sc + 4);
+        if (std::isnan(rr)) throw NanError("NaN encountered");
+        int64_t it = 0;
+        bool updated = false, first = true;
+        const bool flexible = prec == VF_PREC_FP32;
+        while (it < maxIter && rr > tol * tol * bb) {
+            ++it;
+            if (mgIter > 0) {
+                if (!updated) { update_stiffness(); updated = true; }
+                precondition(r, sv, mgIter, nu, fullMG, prec);
+            } else {
+                d2d(sv, r, n0, s);
+            }
+            zero_dirichlet<double>(0, sv);
+            launch_pcg_scalars(sc, 0, s);  // rz_old <- rz
+            if (flexible && !first) launch_dot2_active(g, r, sv, Ad, part, sc, s);  // sc[0]=r.s, sc[1]=Ad_prev.s
+            else launch_dot_active(g, r, sv, part, sc, s);
+            if (first) d2d(d, sv, n0, s);
+            else launch_pcg_direction(g, sv, d, sc, flexible ? 1 : 0, s);
+            apply<double>(0, d, Ad, nullptr, AM_SET, true);
+            launch_dot_active(g, d, Ad, part, sc + 3, s);
+            launch_pcg_scalars(sc, 1, s);  // alpha of this iteration
+            launch_pcg_update(g, x, r, d, Ad, sc, part, sc + 4, s);
+            rr = sim->read_scalar(sc + 4);
+            if (std::isnan(rr)) throw NanError("NaN encountered at iteration" + std::to_string(it));
+            callback(it);
+            first = false;
+        }
+        if (stats) {
+            stats->iterations = it;
+            stats->residual_norm = std::sqrt(rr);
+            stats->rhs_norm = std::sqrt(bb);
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return it;
+    }
+};
+
+template <> double *vf_mg::X<double>(int l) { return l == 0 && x0v ? x0v : lv[l]->x.data(); }
+template <> double *vf_mg::B<double>(int l) { return l == 0 && b0v ? b0v : lv[l]->b.data(); }
+template <> double *vf_mg::R<double>(int l) {
+    if (lv[l]->r.empty()) lv[l]->r.resize(3 * (size_t)lv[l]->g.nnodes);
+    return lv[l]->r.data();
+}
+template <> float *vf_mg::X<float>(int l) { return lv[l]->x32.data(); }
+template <> float *vf_mg::B<float>(int l) { return lv[l]->b32.data(); }
+template <> float *vf_mg::R<float>(int l) { return lv[l]->r32.data(); }
+template <> const double *vf_mg::K0p<double>() const { return sim->K0.data(); }
+template <> const float *vf_mg::K0p<float>() const { return sim->K032.data(); }
+template <> const double *vf_mg::Ep<double>() const { return sim->E.data(); }
+template <> const float *vf_mg::Ep<float>() const { return sim->E32.data(); }
+template <> const double *vf_mg::Chatp<double>() const { return dChat.data(); }
+template <> const float *vf_mg::Chatp<float>() const { return dChat32.data(); }
+template <> const double *vf_mg::blocksp<double>(int l) const { return lv[l]->blocks.data(); }
+template <> const float *vf_mg::blocksp<float>(int l) const { return lv[l]->blocks32.data(); }
+
+// =================================================================================================
+// C ABI
+// =================================================================================================
+#define VF_API_BEGIN try {
+#define VF_API_END                                                                                       \
+    return VF_OK;                                                                                        \
+    }                                                                                                    \
+    catch (const NanError &e) { g_err = e.what(); return VF_ERR_NAN; }                                    \
+    catch (const HipError &e) { g_err = e.what(); return VF_ERR_HIP; }                                    \
+    catch (const std::invalid_argument &e) { g_err = e.what(); return VF_ERR_ARG; }                       \
+    catch (const std::exception &e) { g_err = e.what(); return VF_ERR_RUNTIME; }                          \
+    catch (...) { g_err = "unknown error"; return VF_ERR_RUNTIME; }
+
+namespace {
+void need(bool ok, const char *msg) {
+    if (!ok) throw std::invalid_argument(msg);
+}
+
+// Run `fn(dev_in..., dev_out)` with staging for host pointers.
+struct Staged {
+    vf_sim *sim;
+    int where;
+    const double *in(const double *p, size_t n, double *stage) {
+        if (where == VF_DEVICE) return p;
+        h2d(stage, p, n, sim->stream);
+        return stage;
+    }
+    double *out(double *p, double *stage) { return where == VF_DEVICE ? p : stage; }
+    void finish(double *p, const double *stage, size_t n) {
+        if (where == VF_HOST) { d2h(p, stage, n, sim->stream); sync(sim->stream); }
+    }
+};
+}  // namespace
+
+// host-pointer staging for the stand-alone filter/optimiser entry points
+static hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+template <typename F>
+static void with_buffers(size_t nin, size_t nout, const double *in, double *out, int where, hipStream_t s, F &&fn,
+                         const double *in2 = nullptr) {
+    if (where == VF_DEVICE) { fn(in, in2, out); return; }
+    DBuf<double> a(nin), b(nout), c(in2 ? nin : 0);
+    h2d(a.data(), in, nin, s);
+    if (in2) h2d(c.data(), in2, nin, s);
+    fn(a.data(), in2 ? c.data() : nullptr, b.data());
+    d2h(out, b.data(), nout, s);
+    sync(s);
+}
+
+extern "C" {
+
+const char *vf_last_error(void) { return g_err.c_str(); }
+
+int vf_malloc(void **ptr, size_t bytes) {
+    VF_API_BEGIN
+    need(ptr != nullptr, "null pointer");
+    *ptr = nullptr;
+    if (bytes) VF_HIP(hipMalloc(ptr, bytes));
+    VF_API_END
+}
+
+int vf_free(void *ptr) {
+    VF_API_BEGIN
+    if (ptr) VF_HIP(hipFree(ptr));
+    VF_API_END
+}
+
+int vf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) {
+    VF_API_BEGIN
+    need(kind >= 0 && kind <= 2, "unknown copy kind");
+    if (!bytes) return VF_OK;
+    need(dst && src, "null pointer");
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
+    VF_HIP(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+    if (kind != 2) sync((hipStream_t)stream);
+    VF_API_END
+}
+
+int vf_stream_sync(void *stream) {
+    VF_API_BEGIN
+    sync((hipStream_t)stream);
+    VF_API_END
+}
+const char *vf_version(void) { return "voxelfem-mi355x 0.1 (gfx950)"; }
+
+int vf_device_count(int *count) {
+    VF_API_BEGIN
+    need(count != nullptr, "null count");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    VF_API_END
+}
+
+int vf_sim_create(vf_sim **out, const int64_t ne[3], const double minCorner[3], const double maxCorner[3],
+                  void *stream) {
+    VF_API_BEGIN
+    need(out && ne && minCorner && maxCorner, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<vf_sim>();
+    s->stream = (hipStream_t)stream;
+    for (int d = 0; d < 3; ++d) {
+        need(ne[d] >= 1, "grid must have at least one element per dimension");
+        s->ne[d] = ne[d];
+        s->lo[d] = minCorner[d];
+        s->hi[d] = maxCorner[d];
+        s->h[d] = (maxCorner[d] - minCorner[d]) / (double)ne[d];
+        need(s->h[d] > 0, "domain must have positive extent");
+    }
+    need((ne[0] + 1) * (ne[1] + 1) * (ne[2] + 1) * 3 < INT_MAX, "grid too large for 32-bit node indexing");
+    s->g = make_grid(ne);
+    s->K0.resize(576);
+    s->K032.resize(576);
+    s->rho.resize(s->nel());
+    s->E.resize(s->nel());
+    s->E32.resize(s->nel());
+    launch_fill<double>(s->nel(), s->rho.data(), 0.0, s->stream);
+    s->hmask.assign(s->nn(), 0);
+    s->hvals.assign(3 * s->nn(), 0.0);
+    upload_dirichlet(s->hmask, s->dmask, s->dnodes, s->dnmask, s->ndir, s->stream);
+    s->dvals.resize(3);
+    s->hforce.assign(3 * s->nn(), 0.0);
+    s->update_K0();
+    s->update_E();
+    sync(s->stream);
+    *out = s.release();
+    VF_API_END
+}
+
+int vf_sim_destroy(vf_sim *sim) {
+    VF_API_BEGIN
+    if (sim) { (void)hipStreamSynchronize(sim->stream); delete sim; }
+    VF_API_END
+}
+
+int vf_sim_set_stream(vf_sim *sim, void *stream) {
+    VF_API_BEGIN
+    need(sim, "null sim");
+    sync(sim->stream);
+    sim->stream = (hipStream_t)stream;
+    VF_API_END
+}
+
+int vf_sim_set_material(vf_sim *sim, double young, double poisson) {
+    VF_API_BEGIN
+    need(sim, "null sim");
+    need(young > 0 && poisson > -1 && poisson < 0.5, "invalid isotropic material");
+    sim->young = young;
+    sim->poisson = poisson;
+    sim->update_K0();
+    ++sim->version;
+    VF_API_END
+}
+
+int vf_sim_set_law(vf_sim *sim, int law, double E0, double Emin, double gamma, double q) {
+    VF_API_BEGIN
+    need(sim, "null sim");
+    need(law == VF_SIMP || law == VF_RAMP, "unknown interpolation law");
+    sim->law = law;
+    sim->E0 = E0;
+    sim->Emin = Emin;
+    sim->gamma = gamma;
+    sim->q = q;
+    sim->update_E();
+    VF_API_END
+}
+
+int vf_sim_set_densities(vf_sim *sim, const double *rho, int where) {
+    VF_API_BEGIN
+    need(sim && rho, "null argument");
+    if (where == VF_DEVICE) d2d(sim->rho.data(), rho, sim->nel(), sim->stream);
+    else h2d(sim->rho.data(), rho, sim->nel(), sim->stream);
+    sim->update_E();
+    if (where == VF_HOST) sync(sim->stream);
+    VF_API_END
+}
+
+int vf_sim_get_densities(vf_sim *sim, double *rho, int where) {
+    VF_API_BEGIN
+    need(sim && rho, "null argument");
+    if (where == VF_DEVICE) d2d(rho, sim->rho.data(), sim->nel(), sim->stream);
+    else { d2h(rho, sim->rho.data(), sim->nel(), sim->stream); sync(sim->stream); }
+    VF_API_END
+}
+
+int vf_sim_get_young(vf_sim *sim, double *E, int where) {
+    VF_API_BEGIN
+    need(sim && E, "null argument");
+    if (where == VF_DEVICE) d2d(E, sim->E.data(), sim->nel(), sim->stream);
+    else { d2h(E, sim->E.data(), sim->nel(), sim->stream); sync(sim->stream); }
+    VF_API_END
+}
+
+int vf_sim_get_K0(vf_sim *sim, double *K0) {
+    VF_API_BEGIN
+    need(sim && K0, "null argument");
+    std::copy(sim->K0h, sim->K0h + 576, K0);
+    VF_API_END
+}
+
+int vf_sim_set_dirichlet(vf_sim *sim, const uint8_t *mask, const double *values) {
+    VF_API_BEGIN
+    need(sim && mask, "null argument");
+    const size_t nn = sim->nn();
+    for (size_t n = 0; n < nn; ++n) sim->hmask[n] = mask[n] & 7;
+    for (size_t i = 0; i < 3 * nn; ++i) sim->hvals[i] = values ? values[i] : 0.0;
+    upload_dirichlet(sim->hmask, sim->dmask, sim->dnodes, sim->dnmask, sim->ndir, sim->stream);
+    std::vector<double> v(3 * std::max(sim->ndir, 1), 0.0);
+    int k = 0;
+    for (size_t n = 0; n < nn; ++n) {
+        if (!sim->hmask[n]) continue;
+        for (int c = 0; c < 3; ++c) v[3 * k + c] = sim->hvals[c * nn + n];
+        ++k;
+    }
+    sim->dvals.resize(v.size());
+    h2d(sim->dvals.data(), v.data(), v.size(), sim->stream);
+    sync(sim->stream);
+    VF_API_END
+}
+
+int vf_sim_set_forces(vf_sim *sim, const double *f) {
+    VF_API_BEGIN
+    need(sim && f, "null argument");
+    std::copy(f, f + 3 * sim->nn(), sim->hforce.begin());
+    sim->hasForce = std::any_of(sim->hforce.begin(), sim->hforce.end(), [](double v) { return v != 0.0; });
+    sim->force.resize(3 * sim->nn());
+    h2d(sim->force.data(), f, 3 * sim->nn(), sim->stream);
+    sync(sim->stream);
+    VF_API_END
+}
+
+int vf_sim_set_gravity(vf_sim *sim, const double g[3]) {
+    VF_API_BEGIN
+    need(sim && g, "null argument");
+    for (int c = 0; c < 3; ++c) sim->grav[c] = g[c];
+    VF_API_END
+}
+
+int vf_sim_set_mask_layer(vf_sim *sim, int64_t layer) {
+    VF_API_BEGIN
+    need(sim, "null sim");
+    sim->set_mask_height(sim->h[1] * (double)layer, true);
+    VF_API_END
+}
+
+int vf_sim_get_mask_info(vf_sim *sim, int64_t *firstMasked, int64_t *firstDetached, double *height) {
+    VF_API_BEGIN
+    need(sim, "null sim");
+    if (firstMasked) *firstMasked = sim->firstMasked;
+    if (firstDetached) *firstDetached = sim->firstDetached;
+    if (height) *height = sim->maskHeight;
+    VF_API_END
+}
+
+int vf_sim_apply_K(vf_sim *sim, const double *u, double *y, int where) {
+    VF_API_BEGIN
+    need(sim && u && y, "null argument");
+    sim->refresh_grid();
+    const size_t n = 3 * sim->nn();
+    Staged st{sim, where};
+    const double *du = st.in(u, n, sim->stage_in(n));
+    double *dy = st.out(y, sim->stage_out(n));
+    launch_fine_apply<double>(sim->g, sim->K0.data(), sim->E.data(), du, dy, nullptr, nullptr, AM_SET, sim->stream);
+    st.finish(y, dy, n);
+    VF_API_END
+}
+
+int vf_sim_build_load(vf_sim *sim, double *f, int where) {
+    VF_API_BEGIN
+    need(sim && f, "null argument");
+    sim->refresh_grid();
+    const size_t n = 3 * sim->nn();
+    Staged st{sim, where};
+    double *df = st.out(f, sim->stage_out(n));
+    if (sim->hasForce) d2d(df, sim->force.data(), n, sim->stream);
+    else launch_fill<double>(n, df, 0.0, sim->stream);
+    if (sim->grav[0] != 0 || sim->grav[1] != 0 || sim->grav[2] != 0)
+        launch_self_weight(sim->g, sim->rho.data(), sim->grav, sim->h[0] * sim->h[1] * sim->h[2], df, sim->stream);
+    st.finish(f, df, n);
+    VF_API_END
+}
+
+static int grad_impl(vf_sim *sim, const double *u, double *g, int where, int accumulate) {
+    VF_API_BEGIN
+    need(sim && u && g, "null argument");
+    sim->refresh_grid();
+    const size_t n = 3 * sim->nn(), ne = sim->nel();
+    Staged st{sim, where};
+    const double *du = st.in(u, n, sim->stage_in(n));
+    double *dg = where == VF_DEVICE ? g : sim->stage_out(ne);
+    if (where == VF_HOST && accumulate) h2d(dg, g, ne, sim->stream);
+    launch_compliance_gradient(sim->g, sim->K0.data(), sim->rho.data(), du, dg, sim->law, sim->E0, sim->Emin,
+                               sim->gamma, sim->q, sim->grav, sim->h[0] * sim->h[1] * sim->h[2], accumulate,
+                               sim->stream);
+    st.finish(g, dg, ne);
+    VF_API_END
+}
+
+int vf_sim_compliance_gradient(vf_sim *sim, const double *u, double *g, int where) { return grad_impl(sim, u, g, where, 0); }
+int vf_sim_accumulate_compliance_gradient(vf_sim *sim, const double *u, double *g, int where) {
+    return grad_impl(sim, u, g, where, 1);
+}
+
+int vf_sim_energy_density(vf_sim *sim, const double *u, double *out, int where) {
+    VF_API_BEGIN
+    need(sim && u && out, "null argument");
+    const size_t n = 3 * sim->nn(), ne = sim->nel();
+    Staged st{sim, where};
+    const double *du = st.in(u, n, sim->stage_in(n));
+    double *dout = st.out(out, sim->stage_out(ne));
+    launch_energy_density(sim->g, sim->K0.data(), sim->E.data(), du, dout, sim->stream);
+    st.finish(out, dout, ne);
+    VF_API_END
+}
+
+int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int where, double *out) {
+    VF_API_BEGIN
+    need(sim && a && b && out, "null argument");
+    sim->refresh_grid();
+    const size_t n = 3 * sim->nn();
+    Staged st{sim, where};
+    const double *da = st.in(a, n, sim->stage_in(n));
+    const double *db = st.in(b, n, sim->stage_aux(n));
+    double *sc = sim->scalars();
+    if (masked) {
+        launch_dot_active(sim->g, da, db, sim->red_partials(reduction_partials(sim->g)), sc + 9, sim->stream);
+        *out = sim->read_scalar(sc + 9);
+    } else {
+        launch_dot_full(n, da, db, sim->red_partials(reduction_partials_full(n)), sc + 9, sim->stream);
+        *out = 0.5 * sim->read_scalar(sc + 9);
+    }
+    VF_API_END
+}
+
+// ---------------------------------------------------------------------------------------------- MG
+int vf_mg_create(vf_mg **out, vf_sim *fine, int64_t levels) {
+    VF_API_BEGIN
+    need(out && fine, "null argument");
+    *out = nullptr;
+    *out = new vf_mg(fine, (int)levels);
+    VF_API_END
+}
+
+int vf_mg_destroy(vf_mg *mg) {
+    VF_API_BEGIN
+    if (mg) { (void)hipStreamSynchronize(mg->st()); delete mg; }
+    VF_API_END
+}
+
+int vf_mg_num_levels(vf_mg *mg, int64_t *levels) {
+    VF_API_BEGIN
+    need(mg && levels, "null argument");
+    *levels = mg->L;
+    VF_API_END
+}
+
+int vf_mg_level_nodes(vf_mg *mg, int64_t l, int64_t nn[3]) {
+    VF_API_BEGIN
+    need(mg && nn, "null argument");
+    need(l >= 0 && l <= mg->L, "level out of range");
+    for (int d = 0; d < 3; ++d) nn[d] = mg->lv[l]->g.nn[d];
+    VF_API_END
+}
+
+int vf_mg_level_dirichlet(vf_mg *mg, int64_t l, uint8_t *mask) {
+    VF_API_BEGIN
+    need(mg && mask, "null argument");
+    need(l >= 0 && l <= mg->L, "level out of range");
+    const auto &m = l == 0 ? mg->sim->hmask : mg->lv[l]->hmask;
+    std::copy(m.begin(), m.end(), mask);
+    VF_API_END
+}
+
+int vf_mg_set_symmetric_gs(vf_mg *mg, int symmetric) {
+    VF_API_BEGIN
+    need(mg, "null mg");
+    mg->symmetricGS = symmetric != 0;
+    VF_API_END
+}
+
+int vf_mg_update_stiffness(vf_mg *mg) {
+    VF_API_BEGIN
+    need(mg, "null mg");
+    mg->update_stiffness();
+    sync(mg->st());
+    VF_API_END
+}
+
+int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer) {
+    VF_API_BEGIN
+    need(mg, "null mg");
+    mg->sim->set_mask_height(mg->sim->h[1] * (double)layer, true);
+    mg->sync_masks();
+    mg->stiffnessReady = false;
+    VF_API_END
+}
+
+static size_t level_n(vf_mg *mg, int64_t l) { return 3 * (size_t)mg->lv[l]->g.nnodes; }
+
+int vf_mg_apply_K(vf_mg *mg, int64_t l, const double *u, double *y, int where) {
+    VF_API_BEGIN
+    need(mg && u && y, "null argument");
+    need(l >= 0 && l <= mg->L, "level out of range");
+    mg->sync_masks();
+    if (l >= 2) mg->ensure_stiffness();
+    const size_t n = level_n(mg, l);
+    Staged st{mg->sim, where};
+    const double *du = st.in(u, n, mg->sim->stage_in(n));
+    double *dy = st.out(y, mg->sim->stage_out(n));
+    launch_fill<double>(n, dy, 0.0, mg->st());
+    mg->apply<double>((int)l, du, dy, nullptr, AM_SET, false);
+    st.finish(y, dy, n);
+    VF_API_END
+}
+
+int vf_mg_residual(vf_mg *mg, int64_t l, const double *u, const double *b, double *r, int where) {
+    VF_API_BEGIN
+    need(mg && u && b && r, "null argument");
+    need(l >= 0 && l <= mg->L, "level out of range");
+    mg->sync_masks();
+    if (l >= 2) mg->ensure_stiffness();
+    const size_t n = level_n(mg, l);
+    Staged st{mg->sim, where};
+    const double *du = st.in(u, n, mg->sim->stage_in(n));
+    const double *db = st.in(b, n, mg->sim->stage_aux(n));
+    double *dr = st.out(r, mg->sim->stage_out(n));
+    launch_fill<double>(n, dr, 0.0, mg->st());
+    mg->apply<double>((int)l, du, dr, db, AM_RESIDUAL, true);
+    st.finish(r, dr, n);
+    VF_API_END
+}
+
+int vf_mg_smooth(vf_mg *mg, int64_t l, double *u, const double *b, int forward, int where) {
+    VF_API_BEGIN
+    need(mg && u && b, "null argument");
+    need(l >= 0 && l < mg->L, "smoothing needs a non-coarsest level");
+    mg->sync_masks();
+    if (l >= 2) mg->ensure_stiffness();
+    const size_t n = level_n(mg, l);
+    Staged st{mg->sim, where};
+    double *du = where == VF_DEVICE ? u : mg->sim->stage_out(n);
+    if (where == VF_HOST) h2d(du, u, n, mg->st());
+    const double *db = st.in(b, n, mg->sim->stage_aux(n));
+    mg->smooth<double>((int)l, du, db, forward != 0);
+    st.finish(u, du, n);
+    VF_API_END
+}
+
+int vf_mg_restrict(vf_mg *mg, int64_t l, const double *fine, double *coarse, int where) {
+    VF_API_BEGIN
+    need(mg && fine && coarse, "null argument");
+    need(l >= 0 && l < mg->L, "level out of range");
+    mg->sync_masks();
+    const size_t nf = level_n(mg, l), nc = level_n(mg, l + 1);
+    Staged st{mg->sim, where};
+    const double *df = st.in(fine, nf, mg->sim->stage_in(nf));
+    double *dc = st.out(coarse, mg->sim->stage_out(nc));
+    launch_restrict<double>(mg->lv[l]->g, mg->lv[l + 1]->g, df, dc, mg->st());
+    st.finish(coarse, dc, nc);
+    VF_API_END
+}
+
+int vf_mg_interpolate(vf_mg *mg, int64_t l, const double *coarse, double *fine, int accumulate, int where) {
+    VF_API_BEGIN
+    need(mg && fine && coarse, "null argument");
+    need(l >= 0 && l < mg->L, "level out of range");
+    mg->sync_masks();
+    const size_t nf = level_n(mg, l), nc = level_n(mg, l + 1);
+    Staged st{mg->sim, where};
+    const double *dc = st.in(coarse, nc, mg->sim->stage_in(nc));
+    double *df = where == VF_DEVICE ? fine : mg->sim->stage_out(nf);
+    if (where == VF_HOST) h2d(df, fine, nf, mg->st());
+    launch_interpolate<double>(mg->lv[l]->g, mg->lv[l + 1]->g, dc, df, accumulate, mg->st());
+    st.finish(fine, df, nf);
+    VF_API_END
+}
+
+int vf_mg_coarse_solve(vf_mg *mg, const double *b, double *x, int where) {
+    VF_API_BEGIN
+    need(mg && b && x, "null argument");
+    mg->sync_masks();
+    mg->ensure_stiffness();
+    const size_t n = level_n(mg, mg->L);
+    Staged st{mg->sim, where};
+    const double *db = st.in(b, n, mg->sim->stage_in(n));
+    double *dx = st.out(x, mg->sim->stage_out(n));
+    mg->coarse_solve<double>(db, dx);
+    st.finish(x, dx, n);
+    VF_API_END
+}
+
+int vf_mg_zero_dirichlet(vf_mg *mg, int64_t l, double *u, int where) {
+    VF_API_BEGIN
+    need(mg && u, "null argument");
+    need(l >= 0 && l <= mg->L, "level out of range");
+    const size_t n = level_n(mg, l);
+    double *du = where == VF_DEVICE ? u : mg->sim->stage_out(n);
+    if (where == VF_HOST) h2d(du, u, n, mg->st());
+    mg->zero_dirichlet<double>((int)l, du);
+    if (where == VF_HOST) { d2h(u, du, n, mg->st()); sync(mg->st()); }
+    VF_API_END
+}
+
+int vf_mg_solve(vf_mg *mg, double *x, const double *f, int64_t numSteps, int64_t smoothing, int stiffnessUpdated,
+                int zeroDirichlet, int fullMultigrid, vf_mg_callback cb, void *user, int where) {
+    VF_API_BEGIN
+    need(mg && x && f, "null argument");
+    const size_t n = level_n(mg, 0);
+    hipStream_t s = mg->st();
+    double *dx = x;
+    const double *df = f;
+    if (where == VF_HOST) {
+        mg->px.resize(n);
+        mg->pb.resize(n);
+        h2d(mg->px.data(), x, n, s);
+        h2d(mg->pb.data(), f, n, s);
+        dx = mg->px.data();
+        df = mg->pb.data();
+    }
+    mg->solve(dx, df, (int)numSteps, (int)smoothing, stiffnessUpdated != 0, zeroDirichlet != 0, fullMultigrid != 0, cb,
+              user);
+    if (where == VF_HOST) d2h(x, dx, n, s);
+    sync(s);
+    VF_API_END
+}
+
+int vf_mg_precondition(vf_mg *mg, const double *r, double *s, int64_t mgIterations, int64_t smoothing,
+                       int fullMultigrid, int precision, int where) {
+    VF_API_BEGIN
+    need(mg && r && s, "null argument");
+    const size_t n = level_n(mg, 0);
+    Staged st{mg->sim, where};
+    // the preconditioner reads r while writing s: keep them in distinct staging buffers
+    const double *dr = st.in(r, n, mg->sim->stage_in(n));
+    double *ds = st.out(s, mg->sim->stage_out(n));
+    mg->precondition(dr, ds, (int)mgIterations, (int)smoothing, fullMultigrid != 0, precision);
+    st.finish(s, ds, n);
+    VF_API_END
+}
+
+int vf_mg_pcg(vf_mg *mg, double *x, const double *b, int64_t maxIter, double tol, int64_t mgIterations,
+              int64_t smoothing, int fullMultigrid, int precision, int dirichletAlreadySatisfied, vf_pcg_callback cb,
+              void *user, int where, vf_pcg_stats *stats) {
+    VF_API_BEGIN
+    need(mg && x && b, "null argument");
+    need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "unknown precision");
+    const size_t n = level_n(mg, 0);
+    hipStream_t s = mg->st();
+    double *dx = x;
+    const double *db = b;
+    if (where == VF_HOST) {
+        mg->px.resize(n);
+        mg->pb.resize(n);
+        h2d(mg->px.data(), x, n, s);
+        h2d(mg->pb.data(), b, n, s);
+        dx = mg->px.data();
+        db = mg->pb.data();
+    }
+    mg->pcg(dx, db, maxIter, tol, (int)mgIterations, (int)smoothing, fullMultigrid != 0, precision,
+            dirichletAlreadySatisfied != 0, cb, user, stats);
+    if (where == VF_HOST) d2h(x, dx, n, s);
+    sync(s);
+    VF_API_END
+}
+
+int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                    int64_t smoothing, int fullMultigrid, int precision, double *objective, double *grad, int gradWhere,
+                    double *perLayerCompliance, int64_t *perLayerIterations) {
+    VF_API_BEGIN
+    need(mg && objective && grad, "null argument");
+    need(layerIncrement >= 1, "layerIncrement must be positive");
+    vf_sim *sim = mg->sim;
+    hipStream_t s = mg->st();
+    const int64_t numLayers = sim->ne[1];
+    const size_t n = 3 * sim->nn(), ne = sim->nel();
+    DBuf<double> f(n), u(n), g(ne);
+    launch_fill<double>(ne, g.data(), 0.0, s);
+    sim->set_mask_height(sim->h[1] * (double)numLayers, true);
+    double total = 0;
+    int64_t count = 0;
+    for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) {
+        if (l < numLayers) {
+            // decrementFabricationMaskHeightByLayer (TensorProductSimulator.hh:311-324)
+            sim->maskHeight -= (double)layerIncrement * sim->h[1];
+            sim->firstMasked -= layerIncrement;
+            sim->firstDetached = sim->firstMasked + 1;
+            sim->refresh_grid();
+            launch_zero_young_layers(sim->g, (int)sim->firstMasked, (int)(sim->firstMasked + layerIncrement),
+                                     sim->E.data(), sim->E32.data(), s);
+            ++sim->version;
+        }
+        mg->sync_masks();
+        if (sim->hasForce) d2d(f.data(), sim->force.data(), n, s);
+        else launch_fill<double>(n, f.data(), 0.0, s);
+        if (sim->grav[0] != 0 || sim->grav[1] != 0 || sim->grav[2] != 0)
+            launch_self_weight(sim->g, sim->rho.data(), sim->grav, sim->h[0] * sim->h[1] * sim->h[2], f.data(), s);
+        launch_fill<double>(n, u.data(), 0.0, s);
+        const int64_t its = mg->pcg(u.data(), f.data(), maxIter, tol, (int)mgIterations, (int)smoothing,
+                                    fullMultigrid != 0, precision, true, nullptr, nullptr, nullptr);
+        double *sc = sim->scalars();
+        launch_dot_active(sim->g, f.data(), u.data(), sim->red_partials(reduction_partials(sim->g)), sc + 10, s);
+        const double comp = sim->read_scalar(sc + 10);
+        if (perLayerCompliance) perLayerCompliance[count] = comp;
+        if (perLayerIterations) perLayerIterations[count] = its;
+        total += comp;
+        launch_compliance_gradient(sim->g, sim->K0.data(), sim->rho.data(), u.data(), g.data(), sim->law, sim->E0,
+                                   sim->Emin, sim->gamma, sim->q, sim->grav, sim->h[0] * sim->h[1] * sim->h[2], 1, s);
+        ++count;
+    }
+    *objective = 0.5 * total / (double)count;
+    std::vector<double> hg(ne);
+    d2h(hg.data(), g.data(), ne, s);
+    sync(s);
+    for (auto &v : hg) v /= (double)count;
+    if (gradWhere == VF_DEVICE) { h2d(grad, hg.data(), ne, s); sync(s); }
+    else std::copy(hg.begin(), hg.end(), grad);
+    VF_API_END
+}
+
+int vf_smoothing_filter(const int64_t dims[3], double rmin, int type, const double *in, double *out, int where,
+                        void *stream) {
+    VF_API_BEGIN
+    need(dims && in && out, "null argument");
+    need(rmin > 0, "filter radius must be positive");
+    need(in != out, "smoothing filter cannot run in place");
+    const int d[3] = {(int)dims[0], (int)dims[1], (int)dims[2]};
+    const size_t n = (size_t)d[0] * d[1] * d[2];
+    with_buffers(n, n, in, out, where, as_stream(stream), [&](const double *i, const double *, double *o) {
+        launch_smoothing_filter(d, rmin, type, i, o, as_stream(stream));
+    });
+    VF_API_END
+}
+
+int vf_projection(double beta, int64_t n, const double *in, double *out, int where, void *stream) {
+    VF_API_BEGIN
+    need(in && out, "null argument");
+    with_buffers(n, n, in, out, where, as_stream(stream), [&](const double *i, const double *, double *o) {
+        launch_projection(beta, n, i, o, as_stream(stream));
+    });
+    VF_API_END
+}
+
+int vf_projection_backprop(double beta, int64_t n, const double *g, const double *vars, double *out, int where,
+                           void *stream) {
+    VF_API_BEGIN
+    need(g && vars && out, "null argument");
+    with_buffers(
+        n, n, g, out, where, as_stream(stream),
+        [&](const double *i, const double *v, double *o) { launch_projection_backprop(beta, n, i, v, o, as_stream(stream)); },
+        vars);
+    VF_API_END
+}
+
+int vf_oc_update(int64_t n, const double *x0, const double *dJ, const double *dc, double lambda, double move, double p,
+                 double *out, int where, void *stream) {
+    VF_API_BEGIN
+    need(x0 && dJ && dc && out, "null argument");
+    hipStream_t s = as_stream(stream);
+    if (where == VF_DEVICE) {
+        launch_oc_update(n, x0, dJ, dc, lambda, move, p, out, s);
+    } else {
+        DBuf<double> a(n), b(n), c(n), o(n);
+        h2d(a.data(), x0, n, s); h2d(b.data(), dJ, n, s); h2d(c.data(), dc, n, s);
+        launch_oc_update(n, a.data(), b.data(), c.data(), lambda, move, p, o.data(), s);
+        d2h(out, o.data(), n, s);
+        sync(s);
+    }
+    VF_API_END
+}
+
+int vf_mean(int64_t n, const double *x, int where, void *stream, double *out) {
+    VF_API_BEGIN
+    need(x && out && n > 0, "invalid argument");
+    hipStream_t s = as_stream(stream);
+    DBuf<double> part(reduction_partials_full(n) + 1), res(1), tmp(where == VF_HOST ? n : 0);
+    const double *dx = x;
+    if (where == VF_HOST) { h2d(tmp.data(), x, n, s); dx = tmp.data(); }
+    launch_dot_full(n, dx, nullptr, part.data(), res.data(), s);
+    double v;
+    d2h(&v, res.data(), 1, s);
+    sync(s);
+    *out = v / (double)n;
+    VF_API_END
+}
+
+// ---------------------------------------------------------------------------------------------- bench
+int vf_bench_apply_K(vf_sim *sim, int64_t reps, int precision, double *avg_ms) {
+    VF_API_BEGIN
+    need(sim && avg_ms && reps > 0, "invalid argument");
+    sim->refresh_grid();
+    const size_t n = 3 * sim->nn();
+    hipStream_t s = sim->stream;
+    hipEvent_t a, b;
+    VF_HIP(hipEventCreate(&a));
+    VF_HIP(hipEventCreate(&b));
+    float ms = 0;
+    if (precision == VF_PREC_FP64) {
+        double *u = sim->stage_in(n), *y = sim->stage_out(n);
+        launch_fill<double>(n, u, 1e-3, s);
+        launch_fine_apply<double>(sim->g, sim->K0.data(), sim->E.data(), u, y, nullptr, nullptr, AM_SET, s);
+        VF_HIP(hipEventRecord(a, s));
+        for (int64_t i = 0; i < reps; ++i)
+            launch_fine_apply<double>(sim->g, sim->K0.data(), sim->E.data(), u, y, nullptr, nullptr, AM_SET, s);
+        VF_HIP(hipEventRecord(b, s));
+    } else {
+        sim->s32a.resize(n);
+        sim->s32b.resize(n);
+        float *u = sim->s32a.data(), *y = sim->s32b.data();
+        launch_fill<float>(n, u, 1e-3f, s);
+        launch_fine_apply<float>(sim->g, sim->K032.data(), sim->E32.data(), u, y, nullptr, nullptr, AM_SET, s);
+        VF_HIP(hipEventRecord(a, s));
+        for (int64_t i = 0; i < reps; ++i)
+            launch_fine_apply<float>(sim->g, sim->K032.data(), sim->E32.data(), u, y, nullptr, nullptr, AM_SET, s);
+        VF_HIP(hipEventRecord(b, s));
+    }
+    VF_HIP(hipEventSynchronize(b));
+    VF_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *avg_ms = (double)ms / (double)reps;
+    VF_API_END
+}
+
+}  // extern "C"

@@ -1,0 +1,188 @@
+// vf_internal.hpp -- shared declarations of the HIP implementation (kernels + host orchestration).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace vf {
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+struct NanError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define VF_HIP(call)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::vf::HipError(std::string(#call) + ": " + hipGetErrorString(e_));           \
+    } while (0)
+
+// Geometry of one grid level, passed to kernels by value.
+// Nodes with iy < ydet are active ("non-detached"); elements with ey < ymask are unmasked.
+struct Grid {
+    int ne[3];
+    int nn[3];
+    int nnodes;
+    int nelems;
+    int ydet;
+    int ymask;
+};
+
+inline Grid make_grid(const int64_t ne[3]) {
+    Grid g;
+    for (int d = 0; d < 3; ++d) {
+        g.ne[d] = (int)ne[d];
+        g.nn[d] = (int)ne[d] + 1;
+    }
+    g.nnodes = g.nn[0] * g.nn[1] * g.nn[2];
+    g.nelems = g.ne[0] * g.ne[1] * g.ne[2];
+    g.ydet = g.nn[1];
+    g.ymask = g.ne[1];
+    return g;
+}
+
+// Simple owning device buffer.
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    explicit DBuf(size_t count) { resize(count); }
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void resize(size_t count) {
+        if (count == n) return;
+        release();
+        if (count) VF_HIP(hipMalloc(&p, count * sizeof(T)));
+        n = count;
+    }
+    T *data() const { return p; }
+    size_t size() const { return n; }
+    bool empty() const { return n == 0; }
+};
+
+// ---------------------------------------------------------------------------------------------
+// Kernel launchers (vf_kernels_*.hip). Every launcher enqueues on `s` and does not synchronize.
+// T is double or float. Vector fields are SoA planes (3 * g.nnodes).
+// ---------------------------------------------------------------------------------------------
+enum ApplyMode { AM_SET = 0, AM_ADD = 1, AM_SUB = 2, AM_RESIDUAL = 3 };
+
+// fine level (TPSStencils.hh:431-728).  AM_SET: y = K u (detached rows zeroed);
+// AM_RESIDUAL: y = b - K u on active rows; dmask != nullptr zeroes constrained output components.
+template <typename T>
+void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y, const T *b, const uint8_t *dmask,
+                       int mode, hipStream_t s);
+// level 1 with on-the-fly Galerkin (MultigridSolver.hh:475-499)
+template <typename T>
+void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
+                     const uint8_t *dmask, int mode, hipStream_t s);
+// assembled 27-point block rows (levels >= 2)
+template <typename T>
+void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
+                        hipStream_t s);
+// cached per-element matrices (coarsest level operator; only used by applyK(l) on that level)
+template <typename T>
+void launch_elem_apply(const Grid &g, const T *Ke, const T *u, T *y, int mode, hipStream_t s);
+
+// one colour of the multicolour block Gauss-Seidel (MultigridSolver.hh:347-442)
+template <typename T>
+void launch_fine_gs(const Grid &g, const T *K0, const T *E, T *u, const T *b, const uint8_t *dmask, int color,
+                    int forward, hipStream_t s);
+template <typename T>
+void launch_l1_gs(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, T *u, const T *b, const uint8_t *dmask,
+                  int color, int forward, hipStream_t s);
+template <typename T>
+void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uint8_t *dmask, int color, int forward,
+                     hipStream_t s);
+
+// transfers (MultigridSolver.hh:178-275)
+template <typename T>
+void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s);
+template <typename T>
+void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int accumulate, hipStream_t s);
+
+// Dirichlet handling on explicit node lists (TensorProductSimulator.hh:575-593)
+template <typename T>
+void launch_zero_dirichlet(int count, const int *nodes, const uint8_t *masks, int nnodes, T *u, hipStream_t s);
+void launch_enforce_dirichlet(int count, const int *nodes, const uint8_t *masks, int nnodes, const double *vals,
+                              double *u, hipStream_t s);
+
+// vector helpers over the active (non-detached) region (+margin rows) of a level
+template <typename T>
+void launch_fill_active(const Grid &g, int margin, T *x, T value, hipStream_t s);
+template <typename T>
+void launch_fill(size_t n, T *x, T value, hipStream_t s);
+template <typename Tdst, typename Tsrc>
+void launch_convert(size_t n, const Tsrc *src, Tdst *dst, hipStream_t s);
+template <typename Tdst, typename Tsrc>
+void launch_copy_active(const Grid &g, int margin, const Tsrc *src, Tdst *dst, hipStream_t s);
+
+// deterministic reductions; result written to device scalar `out` (double)
+// dot over active region: out = sum a*b (3 components)
+void launch_dot_active(const Grid &g, const double *a, const double *b, double *partials, double *out, hipStream_t s);
+void launch_dot2_active(const Grid &g, const double *a, const double *b, const double *c, double *partials,
+                        double *out2, hipStream_t s);  // out2[0] = a.b, out2[1] = c.b
+void launch_dot_full(size_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s);
+int reduction_partials(const Grid &g);
+int reduction_partials_full(size_t n);
+
+// PCG vector kernels (MultigridSolver.hh:1119-1143); scalars live in device memory `sc`
+// (layout documented in vf_kernels_vec.hip).
+void launch_pcg_direction(const Grid &g, const double *s, double *d, const double *sc, int mode, hipStream_t st);
+void launch_pcg_update(const Grid &g, double *x, double *r, const double *d, const double *Ad, const double *sc,
+                       double *partials, double *rr_out, hipStream_t st);
+void launch_pcg_scalars(double *sc, int op, hipStream_t st);
+
+// setup-time Galerkin coarsening and assembly (MultigridSolver.hh:704-905)
+void launch_galerkin_l1_elems(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *Ke,
+                              hipStream_t s);
+void launch_galerkin_l2_elems(const Grid &g2, const Grid &gf, const double *M2, const double *Ef, double *Ke,
+                              hipStream_t s);
+void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, const double *Kf, double *Kc,
+                             hipStream_t s);
+void launch_assemble_blocks(const Grid &g, const double *Ke, double *blocks, hipStream_t s);
+void launch_assemble_dense(const Grid &g, const double *Ke, const int *red, int ld, double *A, hipStream_t s);
+// in-place blocked Gauss-Jordan inverse of an SPD matrix (n x n row-major, leading dim ld);
+// returns -A^-1 in A (the sweep-operator sign), scratch >= 3*ld*64 + 64*64 doubles
+void spd_inverse_inplace(int n, int ld, double *A, double *scratch, hipStream_t s);
+template <typename T>
+void launch_coarse_apply(int nred, const double *Ainv_neg, const int *redToFull, int nnodes, const T *b, T *x,
+                         double *tmp_in, double *tmp_out, hipStream_t s);
+
+// elementwise physics
+void launch_young(const Grid &g, const double *rho, double *E, float *E32, int law, double E0, double Emin,
+                  double gamma, double q, int maskActive, hipStream_t s);
+void launch_zero_young_layers(const Grid &g, int y0, int y1, double *E, float *E32, hipStream_t s);
+void launch_compliance_gradient(const Grid &g, const double *K0, const double *rho, const double *u, double *out,
+                                int law, double E0, double Emin, double gamma, double q, const double *grav, double vol,
+                                int accumulate, hipStream_t s);
+void launch_energy_density(const Grid &g, const double *K0, const double *E, const double *u, double *out,
+                           hipStream_t s);
+void launch_self_weight(const Grid &g, const double *rho, const double *grav, double vol, double *f, hipStream_t s);
+
+// filters / optimiser
+void launch_smoothing_filter(const int dims[3], double rmin, int type, const double *in, double *out, hipStream_t s);
+void launch_projection(double beta, size_t n, const double *in, double *out, hipStream_t s);
+void launch_projection_backprop(double beta, size_t n, const double *g, const double *vars, double *out,
+                                hipStream_t s);
+void launch_oc_update(size_t n, const double *x0, const double *dJ, const double *dc, double lambda, double move,
+                      double p, double *out, hipStream_t s);
+
+}  // namespace vf

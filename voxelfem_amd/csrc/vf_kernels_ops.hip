@@ -1,0 +1,575 @@
+// vf_kernels_ops.hip -- level operators of the multigrid hierarchy on gfx950:
+// matrix-free fine K u, level-1 on-the-fly Galerkin operator, assembled 27-point block rows,
+// multicolour block Gauss-Seidel and the grid transfers.
+//
+// Thread mapping: one work-item per output node, z (the contiguous index) fastest, so a
+// 64-wide wavefront reads 64 consecutive doubles of each component plane. Element matrices
+// (K0, the eight coarsened K0s) are indexed with compile-time offsets from a uniform pointer,
+// which the compiler turns into scalar loads shared by the whole wavefront.
+#include "vf_internal.hpp"
+
+namespace vf {
+namespace {
+
+constexpr int TPB = 256;
+
+__device__ __forceinline__ int eidx(const Grid &g, int ex, int ey, int ez) { return (ex * g.ne[1] + ey) * g.ne[2] + ez; }
+
+template <typename T>
+__device__ __forceinline__ void store_out(T *y, const T *b, int N, int n, const T f[3], int mode, uint8_t dm) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        T v;
+        if (mode == AM_SET) v = f[c];
+        else if (mode == AM_ADD) v = y[c * N + n] + f[c];
+        else if (mode == AM_SUB) v = y[c * N + n] - f[c];
+        else v = b[c * N + n] - f[c];
+        if (dm & (1 << c)) v = T(0);
+        y[c * N + n] = v;
+    }
+}
+
+// Gather the 3x3x3 neighbourhood of node (ix,iy,iz); out-of-grid values read as zero.
+template <typename T>
+__device__ __forceinline__ void gather27(const Grid &g, const T *__restrict__ u, int ix, int iy, int iz,
+                                         T un[27][3]) {
+    const int N = g.nnodes, sx = g.nn[1] * g.nn[2], sy = g.nn[2];
+    const int n = (ix * g.nn[1] + iy) * g.nn[2] + iz;
+#pragma unroll
+    for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+        for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) {
+                const int s = (dx + 1) * 9 + (dy + 1) * 3 + (dz + 1);
+                const bool ok = (unsigned)(ix + dx) < (unsigned)g.nn[0] && (unsigned)(iy + dy) < (unsigned)g.nn[1] &&
+                                (unsigned)(iz + dz) < (unsigned)g.nn[2];
+                const int m = n + dx * sx + dy * sy + dz;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) un[s][c] = ok ? u[c * N + m] : T(0);
+            }
+}
+
+// neighbour slot of local node m of the element whose local index of the centre node is (i,j,k)
+__device__ __forceinline__ constexpr int slot_of(int m, int i, int j, int k) {
+    return (((m >> 2) & 1) - i + 1) * 9 + (((m >> 1) & 1) - j + 1) * 3 + ((m & 1) - k + 1);
+}
+
+template <typename T>
+__device__ __forceinline__ void inv3_apply(const T M[9], const T r[3], T out[3]) {
+    const T c00 = M[4] * M[8] - M[5] * M[7];
+    const T c01 = M[5] * M[6] - M[3] * M[8];
+    const T c02 = M[3] * M[7] - M[4] * M[6];
+    const T det = M[0] * c00 + M[1] * c01 + M[2] * c02;
+    const T id = T(1) / det;
+    const T i00 = c00 * id, i01 = (M[2] * M[7] - M[1] * M[8]) * id, i02 = (M[1] * M[5] - M[2] * M[4]) * id;
+    const T i10 = c01 * id, i11 = (M[0] * M[8] - M[2] * M[6]) * id, i12 = (M[2] * M[3] - M[0] * M[5]) * id;
+    const T i20 = c02 * id, i21 = (M[1] * M[6] - M[0] * M[7]) * id, i22 = (M[0] * M[4] - M[1] * M[3]) * id;
+    out[0] = i00 * r[0] + i01 * r[1] + i02 * r[2];
+    out[1] = i10 * r[0] + i11 * r[1] + i12 * r[2];
+    out[2] = i20 * r[0] + i21 * r[1] + i22 * r[2];
+}
+
+// u_n += update from (b - S') and M (MultigridSolver.hh:347-378): block solve for free nodes,
+// sequential masked point updates for partially constrained nodes.
+template <typename T>
+__device__ __forceinline__ void gs_update(T *u, int N, int n, const T bms[3], const T M[9], uint8_t dm, int forward) {
+    T ud[3];
+    if (dm) {
+        ud[0] = ud[1] = ud[2] = T(0);
+        for (int t = 0; t < 3; ++t) {
+            const int i = forward ? t : 2 - t;
+            const T fr = (dm & (1 << i)) ? T(0) : T(1);
+            ud[i] = (bms[i] - (M[3 * i] * ud[0] + M[3 * i + 1] * ud[1] + M[3 * i + 2] * ud[2])) * (fr / M[4 * i]);
+        }
+    } else {
+        inv3_apply(M, bms, ud);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) u[c * N + n] += ud[c];
+}
+
+// ------------------------------------------------------------------ fine level (K0, E)
+// Sum over the 8 incident elements of E_e * K0[:, 3 ln + c]^T u_e (K0 symmetric, column access
+// as in TPSStencils.hh:453-471); optionally also the diagonal block M = sum E_e K0[3ln.., 3ln..].
+template <typename T, bool WANT_M>
+__device__ __forceinline__ void fine_row(const Grid &g, const T *__restrict__ K0, const T *__restrict__ E, int ix,
+                                         int iy, int iz, const T un[27][3], T f[3], T M[9]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
+        const int ex = ix - i, ey = iy - j, ez = iz - k;
+        const bool ok = (unsigned)ex < (unsigned)g.ne[0] && (unsigned)ey < (unsigned)g.ne[1] &&
+                        (unsigned)ez < (unsigned)g.ne[2];
+        if (!ok) continue;
+        const T Ee = E[eidx(g, ex, ey, ez)];
+        const int ln = 4 * i + 2 * j + k;
+        T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int s = slot_of(m, i, j, k);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T um = un[s][d];
+                const int row = (3 * m + d) * 24 + 3 * ln;
+                a0 += K0[row + 0] * um;
+                a1 += K0[row + 1] * um;
+                a2 += K0[row + 2] * um;
+            }
+        }
+        f[0] += Ee * a0;
+        f[1] += Ee * a1;
+        f[2] += Ee * a2;
+        if (WANT_M) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+#pragma unroll
+                for (int c2 = 0; c2 < 3; ++c2) M[3 * c + c2] += Ee * K0[(3 * ln + c) * 24 + 3 * ln + c2];
+        }
+    }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_fine_apply(Grid g, const T *__restrict__ K0, const T *__restrict__ E,
+                                                    const T *__restrict__ u, T *__restrict__ y,
+                                                    const T *__restrict__ b, const uint8_t *__restrict__ dmask) {
+    const int nz = g.nn[2], slab = g.ydet * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= g.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
+    const int n = (ix * g.nn[1] + iy) * nz + iz;
+    T un[27][3];
+    gather27(g, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+    T M[9];
+    fine_row<T, false>(g, K0, E, ix, iy, iz, un, f, M);
+    store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
+// Rows of detached nodes of an AM_SET product are zero (TPSStencils.hh:717-727).
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_zero_detached(Grid g, T *__restrict__ y) {
+    const int nz = g.nn[2], rows = g.nn[1] - g.ydet, slab = rows * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= g.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = g.ydet + r / nz, iz = r % nz;
+    const int n = (ix * g.nn[1] + iy) * nz + iz;
+    y[n] = T(0);
+    y[g.nnodes + n] = T(0);
+    y[2 * g.nnodes + n] = T(0);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_fine_gs(Grid g, const T *__restrict__ K0, const T *__restrict__ E,
+                                                 T *__restrict__ u, const T *__restrict__ b,
+                                                 const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
+                                                 int forward) {
+    const int mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int N = g.nnodes, n = (ix * g.nn[1] + iy) * g.nn[2] + iz;
+    const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
+    if (dm == 7) return;
+    T un[27][3];
+    gather27(g, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+    T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    fine_row<T, true>(g, K0, E, ix, iy, iz, un, f, M);
+    const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
+    gs_update(u, N, n, bms, M, dm, forward);
+}
+
+// ------------------------------------------------------------------ level 1 (fine E, Chat)
+// Coarse element e with centre-node local index ln: Ke = sum_fi E_f Chat_fi (MultigridSolver.hh:483-491).
+template <typename T, bool WANT_M, bool COLS>
+__device__ __forceinline__ void l1_row(const Grid &gc, const Grid &gf, const T *__restrict__ Chat,
+                                       const T *__restrict__ Ef, int ix, int iy, int iz, const T un[27][3], T f[3],
+                                       T M[9]) {
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
+        const int ex = ix - i, ey = iy - j, ez = iz - k;
+        const bool ok = (unsigned)ex < (unsigned)gc.ne[0] && (unsigned)ey < (unsigned)gc.ne[1] &&
+                        (unsigned)ez < (unsigned)gc.ne[2];
+        if (!ok) continue;
+        const int ln = 4 * i + 2 * j + k;
+        T ue[24];
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) ue[3 * m + d] = un[slot_of(m, i, j, k)][d];
+        const int fbase = eidx(gf, 2 * ex, 2 * ey, 2 * ez);
+#pragma unroll
+        for (int fi = 0; fi < 8; ++fi) {
+            const T Efi = Ef[fbase + ((fi >> 2) & 1) * gf.ne[1] * gf.ne[2] + ((fi >> 1) & 1) * gf.ne[2] + (fi & 1)];
+            const T *C = Chat + fi * 576;
+            T a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+            for (int bb = 0; bb < 24; ++bb) {
+                if (COLS) {  // Ke[:, 3ln+c]^T u (smoother form, MultigridSolver.hh:310-314)
+                    a0 += C[bb * 24 + 3 * ln + 0] * ue[bb];
+                    a1 += C[bb * 24 + 3 * ln + 1] * ue[bb];
+                    a2 += C[bb * 24 + 3 * ln + 2] * ue[bb];
+                } else {     // Ke[3ln+c, :] u (operator form, MultigridSolver.hh:488-491)
+                    a0 += C[(3 * ln + 0) * 24 + bb] * ue[bb];
+                    a1 += C[(3 * ln + 1) * 24 + bb] * ue[bb];
+                    a2 += C[(3 * ln + 2) * 24 + bb] * ue[bb];
+                }
+            }
+            f[0] += Efi * a0;
+            f[1] += Efi * a1;
+            f[2] += Efi * a2;
+            if (WANT_M) {
+#pragma unroll
+                for (int c = 0; c < 3; ++c)
+#pragma unroll
+                    for (int c2 = 0; c2 < 3; ++c2) M[3 * c + c2] += Efi * C[(3 * ln + c) * 24 + 3 * ln + c2];
+            }
+        }
+    }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_l1_apply(Grid gc, Grid gf, const T *__restrict__ Chat,
+                                                  const T *__restrict__ Ef, const T *__restrict__ u,
+                                                  T *__restrict__ y, const T *__restrict__ b,
+                                                  const uint8_t *__restrict__ dmask) {
+    const int nz = gc.nn[2], slab = gc.ydet * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= gc.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
+    const int n = (ix * gc.nn[1] + iy) * nz + iz;
+    T un[27][3];
+    gather27(gc, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+    T M[9];
+    l1_row<T, false, false>(gc, gf, Chat, Ef, ix, iy, iz, un, f, M);
+    store_out(y, b, gc.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_l1_gs(Grid gc, Grid gf, const T *__restrict__ Chat,
+                                               const T *__restrict__ Ef, T *__restrict__ u, const T *__restrict__ b,
+                                               const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
+                                               int forward) {
+    const int mx = (gc.nn[0] - cx + 1) / 2, my = (gc.ydet - cy + 1) / 2, mz = (gc.nn[2] - cz + 1) / 2;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int N = gc.nnodes, n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
+    const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
+    if (dm == 7) return;
+    T un[27][3];
+    gather27(gc, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+    T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    l1_row<T, true, true>(gc, gf, Chat, Ef, ix, iy, iz, un, f, M);
+    const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
+    gs_update(u, N, n, bms, M, dm, forward);
+}
+
+// ------------------------------------------------------------------ assembled block rows
+// blocks: plane-major SoA, blocks[(slot*9 + 3a + b) * nnodes + n] = K(n, n+slot)[a][b]
+template <typename T, bool WANT_M>
+__device__ __forceinline__ void block_row(const Grid &g, const T *__restrict__ blocks, int n, const T un[27][3],
+                                          T f[3], T M[9]) {
+    const int N = g.nnodes;
+#pragma unroll
+    for (int s = 0; s < 27; ++s) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T acc = 0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const T B = blocks[(size_t)(s * 9 + 3 * a + c) * N + n];
+                acc += B * un[s][c];
+                if (WANT_M && s == 13) M[3 * a + c] = B;
+            }
+            f[a] += acc;
+        }
+    }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_block_apply(Grid g, const T *__restrict__ blocks, const T *__restrict__ u,
+                                                     T *__restrict__ y, const T *__restrict__ b,
+                                                     const uint8_t *__restrict__ dmask) {
+    const int nz = g.nn[2], slab = g.ydet * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= g.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
+    const int n = (ix * g.nn[1] + iy) * nz + iz;
+    T un[27][3];
+    gather27(g, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+    T M[9];
+    block_row<T, false>(g, blocks, n, un, f, M);
+    store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_block_gs(Grid g, const T *__restrict__ blocks, T *__restrict__ u,
+                                                  const T *__restrict__ b, const uint8_t *__restrict__ dmask, int cx,
+                                                  int cy, int cz, int forward) {
+    const int mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int N = g.nnodes, n = (ix * g.nn[1] + iy) * g.nn[2] + iz;
+    const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
+    if (dm == 7) return;
+    T un[27][3];
+    gather27(g, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+    T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    block_row<T, true>(g, blocks, n, un, f, M);
+    const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
+    gs_update(u, N, n, bms, M, dm, forward);
+}
+
+// ------------------------------------------------------------------ cached element matrices
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_elem_apply(Grid g, const T *__restrict__ Ke, const T *__restrict__ u,
+                                                    T *__restrict__ y) {
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= g.nnodes) return;
+    const int nz = g.nn[2], ix = t / (g.nn[1] * nz), iy = (t / nz) % g.nn[1], iz = t % nz;
+    T un[27][3];
+    gather27(g, u, ix, iy, iz, un);
+    T f[3] = {0, 0, 0};
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
+        const int ex = ix - i, ey = iy - j, ez = iz - k;
+        if (!((unsigned)ex < (unsigned)g.ne[0] && (unsigned)ey < (unsigned)g.ne[1] && (unsigned)ez < (unsigned)g.ne[2]))
+            continue;
+        const int ln = 4 * i + 2 * j + k;
+        const T *K = Ke + (size_t)eidx(g, ex, ey, ez) * 576;
+#pragma unroll
+        for (int m = 0; m < 8; ++m)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T um = un[slot_of(m, i, j, k)][d];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) f[c] += K[(3 * ln + c) * 24 + 3 * m + d] * um;
+            }
+    }
+    store_out(y, (const T *)nullptr, g.nnodes, t, f, MODE, uint8_t(0));
+}
+
+// ------------------------------------------------------------------ transfers
+// r_c(n) = sum_{o in {-1,0,1}^3} prod_d (1 - |o_d|/2) r_f(2n + o) over active fine nodes
+// (MultigridSolver.hh:214-275); every detached coarse row is zeroed.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_restrict(Grid gf, Grid gc, const T *__restrict__ rf, T *__restrict__ rc) {
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= gc.nnodes) return;
+    const int nz = gc.nn[2], ix = t / (gc.nn[1] * nz), iy = (t / nz) % gc.nn[1], iz = t % nz;
+    T v0 = 0, v1 = 0, v2 = 0;
+    if (iy < gc.ydet) {
+        const int Nf = gf.nnodes;
+#pragma unroll
+        for (int ox = -1; ox <= 1; ++ox) {
+            const int fx = 2 * ix + ox;
+            if ((unsigned)fx >= (unsigned)gf.nn[0]) continue;
+#pragma unroll
+            for (int oy = -1; oy <= 1; ++oy) {
+                const int fy = 2 * iy + oy;
+                if ((unsigned)fy >= (unsigned)gf.ydet) continue;
+#pragma unroll
+                for (int oz = -1; oz <= 1; ++oz) {
+                    const int fz = 2 * iz + oz;
+                    if ((unsigned)fz >= (unsigned)gf.nn[2]) continue;
+                    const T w = T((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
+                    const int m = (fx * gf.nn[1] + fy) * gf.nn[2] + fz;
+                    v0 += w * rf[m];
+                    v1 += w * rf[Nf + m];
+                    v2 += w * rf[2 * Nf + m];
+                }
+            }
+        }
+    }
+    rc[t] = v0;
+    rc[gc.nnodes + t] = v1;
+    rc[2 * gc.nnodes + t] = v2;
+}
+
+// x_f (+)= P x_c, trilinear (MultigridSolver.hh:162-212). accumulate: active fine rows only
+// (accum_interpolation); otherwise every fine row is overwritten (interpolation).
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_interpolate(Grid gf, Grid gc, const T *__restrict__ xc, T *__restrict__ xf,
+                                                     int accumulate) {
+    const int nz = gf.nn[2];
+    const int rows = accumulate ? gf.ydet : gf.nn[1];
+    const int slab = rows * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= gf.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
+    const int id[3] = {ix, iy, iz};
+    int e[3];
+    T w0[3], w1[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        e[d] = min(id[d] >> 1, gc.ne[d]);
+        const int lf = id[d] - 2 * e[d];
+        w0[d] = lf ? T(0.5) : T(1);
+        w1[d] = lf ? T(0.5) : T(0);
+    }
+    const int Nc = gc.nnodes;
+    T v0 = 0, v1 = 0, v2 = 0;
+#pragma unroll
+    for (int lc = 0; lc < 8; ++lc) {
+        const int a = (lc >> 2) & 1, bb = (lc >> 1) & 1, cc = lc & 1;
+        const T coeff = (a ? w1[0] : w0[0]) * (bb ? w1[1] : w0[1]) * (cc ? w1[2] : w0[2]);
+        if (coeff == T(0)) continue;
+        const int m = ((e[0] + a) * gc.nn[1] + e[1] + bb) * gc.nn[2] + e[2] + cc;
+        v0 += coeff * xc[m];
+        v1 += coeff * xc[Nc + m];
+        v2 += coeff * xc[2 * Nc + m];
+    }
+    const int n = (ix * gf.nn[1] + iy) * nz + iz, Nf = gf.nnodes;
+    if (accumulate) {
+        xf[n] += v0;
+        xf[Nf + n] += v1;
+        xf[2 * Nf + n] += v2;
+    } else {
+        xf[n] = v0;
+        xf[Nf + n] = v1;
+        xf[2 * Nf + n] = v2;
+    }
+}
+
+inline unsigned nblocks(long count) { return (unsigned)((count + TPB - 1) / TPB); }
+
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+#define VF_DISPATCH_MODE(KERNEL, T, mode, grid, ...)                                                        \
+    switch (mode) {                                                                                         \
+    case AM_SET: hipLaunchKernelGGL((KERNEL<T, AM_SET>), grid, dim3(TPB), 0, s, __VA_ARGS__); break;         \
+    case AM_ADD: hipLaunchKernelGGL((KERNEL<T, AM_ADD>), grid, dim3(TPB), 0, s, __VA_ARGS__); break;         \
+    case AM_SUB: hipLaunchKernelGGL((KERNEL<T, AM_SUB>), grid, dim3(TPB), 0, s, __VA_ARGS__); break;         \
+    default: hipLaunchKernelGGL((KERNEL<T, AM_RESIDUAL>), grid, dim3(TPB), 0, s, __VA_ARGS__); break;        \
+    }
+
+template <typename T>
+void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y, const T *b, const uint8_t *dmask,
+                       int mode, hipStream_t s) {
+    const long count = (long)g.nn[0] * g.ydet * g.nn[2];
+    if (count > 0) { VF_DISPATCH_MODE(k_fine_apply, T, mode, dim3(nblocks(count)), g, K0, E, u, y, b, dmask); }
+    if (mode == AM_SET && g.ydet < g.nn[1]) {
+        const long cz = (long)g.nn[0] * (g.nn[1] - g.ydet) * g.nn[2];
+        hipLaunchKernelGGL(k_zero_detached<T>, dim3(nblocks(cz)), dim3(TPB), 0, s, g, y);
+    }
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
+                     const uint8_t *dmask, int mode, hipStream_t s) {
+    const long count = (long)gc.nn[0] * gc.ydet * gc.nn[2];
+    if (count > 0) { VF_DISPATCH_MODE(k_l1_apply, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask); }
+    if (mode == AM_SET && gc.ydet < gc.nn[1]) {
+        const long cz = (long)gc.nn[0] * (gc.nn[1] - gc.ydet) * gc.nn[2];
+        hipLaunchKernelGGL(k_zero_detached<T>, dim3(nblocks(cz)), dim3(TPB), 0, s, gc, y);
+    }
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
+                        hipStream_t s) {
+    const long count = (long)g.nn[0] * g.ydet * g.nn[2];
+    if (count > 0) { VF_DISPATCH_MODE(k_block_apply, T, mode, dim3(nblocks(count)), g, blocks, u, y, b, dmask); }
+    if (mode == AM_SET && g.ydet < g.nn[1]) {
+        const long cz = (long)g.nn[0] * (g.nn[1] - g.ydet) * g.nn[2];
+        hipLaunchKernelGGL(k_zero_detached<T>, dim3(nblocks(cz)), dim3(TPB), 0, s, g, y);
+    }
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_elem_apply(const Grid &g, const T *Ke, const T *u, T *y, int mode, hipStream_t s) {
+    if (mode == AM_SET) hipLaunchKernelGGL((k_elem_apply<T, AM_SET>), dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g, Ke, u, y);
+    else if (mode == AM_ADD) hipLaunchKernelGGL((k_elem_apply<T, AM_ADD>), dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g, Ke, u, y);
+    else if (mode == AM_SUB) hipLaunchKernelGGL((k_elem_apply<T, AM_SUB>), dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g, Ke, u, y);
+    else throw std::runtime_error("residual mode unsupported on cached element matrices");
+    VF_HIP(hipGetLastError());
+}
+
+static inline long color_count(const Grid &g, int cx, int cy, int cz) {
+    const long mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    return (mx > 0 && my > 0 && mz > 0) ? mx * my * mz : 0;
+}
+
+template <typename T>
+void launch_fine_gs(const Grid &g, const T *K0, const T *E, T *u, const T *b, const uint8_t *dmask, int color,
+                    int forward, hipStream_t s) {
+    const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
+    const long count = color_count(g, cx, cy, cz);
+    if (!count) return;
+    hipLaunchKernelGGL(k_fine_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, g, K0, E, u, b, dmask, cx, cy, cz, forward);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_l1_gs(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, T *u, const T *b, const uint8_t *dmask,
+                  int color, int forward, hipStream_t s) {
+    const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
+    const long count = color_count(gc, cx, cy, cz);
+    if (!count) return;
+    hipLaunchKernelGGL(k_l1_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx, cy, cz,
+                       forward);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uint8_t *dmask, int color, int forward,
+                     hipStream_t s) {
+    const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
+    const long count = color_count(g, cx, cy, cz);
+    if (!count) return;
+    hipLaunchKernelGGL(k_block_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy, cz,
+                       forward);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s) {
+    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(gc.nnodes)), dim3(TPB), 0, s, gf, gc, rf, rc);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int accumulate, hipStream_t s) {
+    const long count = (long)gf.nn[0] * (accumulate ? gf.ydet : gf.nn[1]) * gf.nn[2];
+    if (!count) return;
+    hipLaunchKernelGGL(k_interpolate<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gf, gc, xc, xf, accumulate);
+    VF_HIP(hipGetLastError());
+}
+
+#define VF_INST(T)                                                                                                     \
+    template void launch_fine_apply<T>(const Grid &, const T *, const T *, const T *, T *, const T *, const uint8_t *,  \
+                                       int, hipStream_t);                                                              \
+    template void launch_l1_apply<T>(const Grid &, const Grid &, const T *, const T *, const T *, T *, const T *,       \
+                                     const uint8_t *, int, hipStream_t);                                               \
+    template void launch_block_apply<T>(const Grid &, const T *, const T *, T *, const T *, const uint8_t *, int,       \
+                                        hipStream_t);                                                                  \
+    template void launch_elem_apply<T>(const Grid &, const T *, const T *, T *, int, hipStream_t);                     \
+    template void launch_fine_gs<T>(const Grid &, const T *, const T *, T *, const T *, const uint8_t *, int, int,      \
+                                    hipStream_t);                                                                      \
+    template void launch_l1_gs<T>(const Grid &, const Grid &, const T *, const T *, T *, const T *, const uint8_t *,    \
+                                  int, int, hipStream_t);                                                              \
+    template void launch_block_gs<T>(const Grid &, const T *, T *, const T *, const uint8_t *, int, int, hipStream_t); \
+    template void launch_restrict<T>(const Grid &, const Grid &, const T *, T *, hipStream_t);                        \
+    template void launch_interpolate<T>(const Grid &, const Grid &, const T *, T *, int, hipStream_t);
+
+VF_INST(double)
+VF_INST(float)
+
+}  // namespace vf

@@ -1,0 +1,550 @@
+// vf_kernels_vec.hip -- PCG vector kernels with deterministic reductions, Dirichlet handling,
+// Young's modulus interpolation, compliance sensitivity, self-weight loads, density filters and
+// the optimality-criterion update, for gfx950.
+//
+// Reductions are two-pass and order-fixed (per-workgroup partials in a fixed tree, then one
+// workgroup sums the partials in a fixed order), so every solve is bit-reproducible.
+#include "vf_internal.hpp"
+
+namespace vf {
+namespace {
+
+constexpr int TPB = 256;
+constexpr int ITEMS = 4;  // elements per work-item in the active-region kernels
+inline unsigned nblocks(long count) { return (unsigned)((count + TPB - 1) / TPB); }
+
+// The active region of a level: for component c and x-slab ix, the contiguous node range
+// [ix*sx, ix*sx + (ydet + margin)*nz). Launch: grid.y = 3 * nn0, grid.x covers one slab.
+struct Active {
+    int rowLen;  // (ydet + margin) * nz
+    int sx;      // nn1 * nz
+    int nn0;
+    int N;
+};
+inline Active active_of(const Grid &g, int margin) {
+    Active a;
+    a.rowLen = min(g.ydet + margin, g.nn[1]) * g.nn[2];
+    a.sx = g.nn[1] * g.nn[2];
+    a.nn0 = g.nn[0];
+    a.N = g.nnodes;
+    return a;
+}
+inline dim3 active_grid(const Active &a) {
+    return dim3((unsigned)((a.rowLen + TPB * ITEMS - 1) / (TPB * ITEMS)), (unsigned)(3 * a.nn0));
+}
+__device__ __forceinline__ size_t active_base(const Active &a) {
+    const int c = blockIdx.y / a.nn0, ix = blockIdx.y - c * a.nn0;
+    return (size_t)c * a.N + (size_t)ix * a.sx;
+}
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double r = 0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < TPB / 64; ++i) r += sh[i];
+    return r;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_fill_active(Active a, T *x, T v) {
+    const size_t base = active_base(a);
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) x[base + i] = v;
+    }
+}
+
+template <typename Td, typename Ts>
+__global__ void __launch_bounds__(TPB) k_copy_active(Active a, const Ts *__restrict__ s, Td *__restrict__ d) {
+    const size_t base = active_base(a);
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) d[base + i] = (Td)s[base + i];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_fill(size_t n, T *x, T v) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+template <typename Td, typename Ts>
+__global__ void __launch_bounds__(TPB) k_convert(size_t n, const Ts *__restrict__ s, Td *__restrict__ d) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) d[i] = (Td)s[i];
+}
+
+__global__ void __launch_bounds__(TPB) k_dot_active(Active a, const double *__restrict__ x,
+                                                    const double *__restrict__ y, double *__restrict__ partials) {
+    __shared__ double sh[TPB / 64];
+    const size_t base = active_base(a);
+    double s = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) s += x[base + i] * y[base + i];
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void __launch_bounds__(TPB) k_dot2_active(Active a, const double *__restrict__ x,
+                                                     const double *__restrict__ y, const double *__restrict__ z,
+                                                     double *__restrict__ partials, int stride) {
+    __shared__ double sh[TPB / 64];
+    const size_t base = active_base(a);
+    double s = 0, t = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            const double yy = y[base + i];
+            s += x[base + i] * yy;
+            t += z[base + i] * yy;
+        }
+    }
+    s = block_sum(s, sh);
+    __syncthreads();
+    t = block_sum(t, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+        partials[stride + blockIdx.y * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_dot_full(size_t n, const double *__restrict__ x,
+                                                  const double *__restrict__ y, double *__restrict__ partials) {
+    __shared__ double sh[TPB / 64];
+    double s = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const size_t i = ((size_t)blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < n) s += x[i] * (y ? y[i] : 1.0);
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+// single-workgroup fixed-order sum of `count` partials; writes out[0] (and out[1] for the second
+// set at partials + stride when nsets == 2)
+__global__ void __launch_bounds__(TPB) k_final_sum(const double *__restrict__ partials, int count, int stride,
+                                                   int nsets, double *__restrict__ out) {
+    __shared__ double sh[TPB / 64];
+    for (int set = 0; set < nsets; ++set) {
+        double s = 0;
+        for (int i = threadIdx.x; i < count; i += TPB) s += partials[set * stride + i];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) out[set] = s;
+        __syncthreads();
+    }
+}
+
+// ---- PCG (MultigridSolver.hh:1119-1143). Device scalars: sc[0]=<r,s>  sc[1]=<Ad_prev,s>
+// sc[2]=previous <r,s>  sc[3]=<d,Ad>  sc[4]=<r,r>  sc[5]=alpha of the last update
+__global__ void __launch_bounds__(TPB) k_pcg_direction(Active a, const double *__restrict__ s,
+                                                       double *__restrict__ d, const double *__restrict__ sc,
+                                                       int mode) {
+    // mode 0: Fletcher-Reeves beta = rz / rz_old ; mode 1: Polak-Ribiere (flexible) beta =
+    // <r_k - r_{k-1}, s_k>/rz_old = -alpha_prev <Ad_prev, s_k> / rz_old
+    const double beta = mode == 0 ? sc[0] / sc[2] : -sc[5] * sc[1] / sc[2];
+    const size_t base = active_base(a);
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) d[base + i] = s[base + i] + beta * d[base + i];
+    }
+}
+
+__global__ void __launch_bounds__(TPB) k_pcg_update(Active a, double *__restrict__ x, double *__restrict__ r,
+                                                    const double *__restrict__ d, const double *__restrict__ Ad,
+                                                    const double *__restrict__ sc, double *__restrict__ partials) {
+    __shared__ double sh[TPB / 64];
+    const double alpha = sc[0] / sc[3];
+    const size_t base = active_base(a);
+    double s = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            x[base + i] += alpha * d[base + i];
+            const double rn = r[base + i] - alpha * Ad[base + i];
+            r[base + i] = rn;
+            s += rn * rn;
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+__global__ void k_pcg_scalars(double *sc, int op) {
+    if (threadIdx.x != 0) return;
+    if (op == 0) sc[2] = sc[0];                 // previous <r,s> <- <r,s>
+    else if (op == 1) sc[5] = sc[0] / sc[3];    // alpha of this iteration
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_zero_dirichlet(int count, const int *__restrict__ nodes,
+                                                        const uint8_t *__restrict__ masks, int N, T *__restrict__ u) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= count) return;
+    const int n = nodes[i];
+    const uint8_t m = masks[i];
+    if (m & 1) u[n] = T(0);
+    if (m & 2) u[N + n] = T(0);
+    if (m & 4) u[2 * N + n] = T(0);
+}
+
+__global__ void __launch_bounds__(TPB) k_enforce_dirichlet(int count, const int *__restrict__ nodes,
+                                                           const uint8_t *__restrict__ masks, int N,
+                                                           const double *__restrict__ vals, double *__restrict__ u) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= count) return;
+    const int n = nodes[i];
+    const uint8_t m = masks[i];
+    for (int c = 0; c < 3; ++c)
+        if (m & (1 << c)) u[c * N + n] = vals[3 * i + c];
+}
+
+// ---- physics
+// E = Emin + rho^gamma (E0 - Emin) (SIMP) or Emin + rho (E0 - Emin)/(1 + q (1 - rho)) (RAMP);
+// 0 on masked layers (TensorProductSimulator.hh:2055-2060, 2088-2102)
+__global__ void __launch_bounds__(TPB) k_young(Grid g, const double *__restrict__ rho, double *__restrict__ E,
+                                               float *__restrict__ E32, int law, double E0, double Emin,
+                                               double gamma, double q, int maskActive) {
+    const int e = blockIdx.x * TPB + threadIdx.x;
+    if (e >= g.nelems) return;
+    const int ey = (e / g.ne[2]) % g.ne[1];
+    const double r = rho[e];
+    double v;
+    if (maskActive && ey >= g.ymask) v = 0.0;
+    else if (law == 0) v = Emin + pow(r, gamma) * (E0 - Emin);
+    else v = Emin + r * (E0 - Emin) / (1 + q * (1 - r));
+    E[e] = v;
+    E32[e] = (float)v;
+}
+
+__global__ void __launch_bounds__(TPB) k_zero_young_layers(Grid g, int y0, int y1, double *E, float *E32) {
+    const int e = blockIdx.x * TPB + threadIdx.x;
+    if (e >= g.nelems) return;
+    const int ey = (e / g.ne[2]) % g.ne[1];
+    if (ey >= y0 && ey < y1) {
+        E[e] = 0.0;
+        E32[e] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ void gather_elem(const Grid &g, const double *__restrict__ u, int ex, int ey, int ez,
+                                            double ue[24]) {
+    const int N = g.nnodes;
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        const int n = ((ex + ((l >> 2) & 1)) * g.nn[1] + ey + ((l >> 1) & 1)) * g.nn[2] + ez + (l & 1);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) ue[3 * l + c] = u[c * N + n];
+    }
+}
+
+__device__ __forceinline__ double quad_form(const double *__restrict__ K0, const double ue[24]) {
+    double s = 0;
+#pragma unroll
+    for (int a = 0; a < 24; ++a) {
+        double r = 0;
+#pragma unroll
+        for (int b = 0; b < 24; ++b) r += K0[a * 24 + b] * ue[b];
+        s += ue[a] * r;
+    }
+    return s;
+}
+
+// g_e = -1/2 dE/drho u_e^T K0 u_e (+ self-weight term) (TensorProductSimulator.hh:972-1040)
+__global__ void __launch_bounds__(TPB) k_compliance_gradient(Grid g, const double *__restrict__ K0,
+                                                             const double *__restrict__ rho,
+                                                             const double *__restrict__ u, double *__restrict__ out,
+                                                             int law, double E0, double Emin, double gamma, double q,
+                                                             double g0, double g1, double g2, double vol,
+                                                             int accumulate) {
+    const int e = blockIdx.x * TPB + threadIdx.x;
+    if (e >= g.nelems) return;
+    const int ez = e % g.ne[2], ey = (e / g.ne[2]) % g.ne[1], ex = e / (g.ne[1] * g.ne[2]);
+    if (ey >= g.ymask) {
+        if (!accumulate) out[e] = 0.0;
+        return;
+    }
+    double ue[24];
+    gather_elem(g, u, ex, ey, ez, ue);
+    const double r = rho[e];
+    const double dE = law == 0 ? gamma * pow(r, gamma - 1.0) * (E0 - Emin)
+                               : (1 + q) * (E0 - Emin) / ((1 + q * (1 - r)) * (1 + q * (1 - r)));
+    double val = -0.5 * dE * quad_form(K0, ue);
+    if (g0 != 0 || g1 != 0 || g2 != 0) {
+        for (int l = 0; l < 8; ++l) val += 0.125 * (g0 * ue[3 * l] + g1 * ue[3 * l + 1] + g2 * ue[3 * l + 2]) * vol;
+    }
+    if (accumulate) out[e] += val;
+    else out[e] = val;
+}
+
+__global__ void __launch_bounds__(TPB) k_energy_density(Grid g, const double *__restrict__ K0,
+                                                        const double *__restrict__ E, const double *__restrict__ u,
+                                                        double *__restrict__ out) {
+    const int e = blockIdx.x * TPB + threadIdx.x;
+    if (e >= g.nelems) return;
+    const int ez = e % g.ne[2], ey = (e / g.ne[2]) % g.ne[1], ex = e / (g.ne[1] * g.ne[2]);
+    double ue[24];
+    gather_elem(g, u, ex, ey, ez, ue);
+    out[e] = 0.5 * E[e] * quad_form(K0, ue);
+}
+
+// f_n += g * sum over incident unmasked elements of rho_e V / 8 (TensorProductSimulator.hh:1277-1286)
+__global__ void __launch_bounds__(TPB) k_self_weight(Grid g, const double *__restrict__ rho, double g0, double g1,
+                                                     double g2, double vol, double *__restrict__ f) {
+    const int n = blockIdx.x * TPB + threadIdx.x;
+    if (n >= g.nnodes) return;
+    const int nz = g.nn[2], ix = n / (g.nn[1] * nz), iy = (n / nz) % g.nn[1], iz = n % nz;
+    double w = 0;
+    for (int e = 0; e < 8; ++e) {
+        const int ex = ix - ((e >> 2) & 1), ey = iy - ((e >> 1) & 1), ez = iz - (e & 1);
+        if ((unsigned)ex >= (unsigned)g.ne[0] || (unsigned)ey >= (unsigned)g.ne[1] || (unsigned)ez >= (unsigned)g.ne[2])
+            continue;
+        if (ey >= g.ymask) continue;
+        w += 0.125 * rho[(ex * g.ne[1] + ey) * g.ne[2] + ez] * vol;
+    }
+    f[n] += g0 * w;
+    f[g.nnodes + n] += g1 * w;
+    f[2 * g.nnodes + n] += g2 * w;
+}
+
+// ---- filters (TopologyOptimizationFilter.hh:189-400)
+__device__ __forceinline__ int reflect_index(int i, int s) {
+    while (i < 0 || i >= s) {
+        if (i >= s) i = 2 * s - i - 1;
+        if (i < 0) i = -i - 1;
+    }
+    return i;
+}
+
+__global__ void __launch_bounds__(TPB) k_smoothing(int d0, int d1, int d2, int R, double rmin, int type, double W,
+                                                   const double *__restrict__ in, double *__restrict__ out) {
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= d0 * d1 * d2) return;
+    const int k = t % d2, j = (t / d2) % d1, i = t / (d1 * d2);
+    double s = 0;
+    for (int a = -R; a <= R; ++a) {
+        const int ii = reflect_index(i + a, d0);
+        for (int b = -R; b <= R; ++b) {
+            const int jj = reflect_index(j + b, d1);
+            const size_t rowb = ((size_t)ii * d1 + jj) * d2;
+            for (int c = -R; c <= R; ++c) {
+                const double w = type == 1 ? rmin - sqrt((double)(a * a + b * b + c * c)) : 1.0;
+                if (w <= 0) continue;
+                s += w * in[rowb + reflect_index(k + c, d2)];
+            }
+        }
+    }
+    out[t] = s / W;
+}
+
+__global__ void __launch_bounds__(TPB) k_projection(double beta, double th, size_t n, const double *__restrict__ in,
+                                                    double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) out[i] = (th + tanh(beta * (in[i] - 0.5))) / (2 * th);
+}
+
+__global__ void __launch_bounds__(TPB) k_projection_bp(double beta, double scale, size_t n,
+                                                       const double *__restrict__ g, const double *__restrict__ v,
+                                                       double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const double t = tanh(beta * (v[i] - 0.5));
+    out[i] = g[i] * (1.0 - t * t) * scale;
+}
+
+// OCOptimizer ceval update (OptimalityCriterion.hh:66-72)
+__global__ void __launch_bounds__(TPB) k_oc_update(size_t n, const double *__restrict__ x0,
+                                                   const double *__restrict__ dJ, const double *__restrict__ dc,
+                                                   double lambda, double m, double p, double *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i >= n) return;
+    const double x = x0[i];
+    double r = x * pow(dJ[i] / (dc[i] * lambda), p);
+    r = fmin(fmax(fmin(fmax(r, x - m), x + m), 0.0), 1.0);
+    if (!isfinite(r)) r = x;
+    out[i] = r;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ launchers
+template <typename T>
+void launch_fill_active(const Grid &g, int margin, T *x, T value, hipStream_t s) {
+    const Active a = active_of(g, margin);
+    if (a.rowLen <= 0) return;
+    hipLaunchKernelGGL(k_fill_active<T>, active_grid(a), dim3(TPB), 0, s, a, x, value);
+    VF_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_fill(size_t n, T *x, T value, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_fill<T>, dim3(nblocks(n)), dim3(TPB), 0, s, n, x, value);
+    VF_HIP(hipGetLastError());
+}
+template <typename Td, typename Ts>
+void launch_convert(size_t n, const Ts *src, Td *dst, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_convert<Td, Ts>), dim3(nblocks(n)), dim3(TPB), 0, s, n, src, dst);
+    VF_HIP(hipGetLastError());
+}
+template <typename Td, typename Ts>
+void launch_copy_active(const Grid &g, int margin, const Ts *src, Td *dst, hipStream_t s) {
+    const Active a = active_of(g, margin);
+    if (a.rowLen <= 0) return;
+    hipLaunchKernelGGL((k_copy_active<Td, Ts>), active_grid(a), dim3(TPB), 0, s, a, src, dst);
+    VF_HIP(hipGetLastError());
+}
+
+int reduction_partials(const Grid &g) {
+    const Active a = active_of(g, 0);
+    const dim3 gr = active_grid(a);
+    return (int)(gr.x * gr.y);
+}
+int reduction_partials_full(size_t n) { return (int)((n + TPB * ITEMS - 1) / (TPB * ITEMS)); }
+
+void launch_dot_active(const Grid &g, const double *a, const double *b, double *partials, double *out, hipStream_t s) {
+    const Active ac = active_of(g, 0);
+    if (ac.rowLen <= 0) { launch_fill<double>(1, out, 0.0, s); return; }
+    const dim3 gr = active_grid(ac);
+    hipLaunchKernelGGL(k_dot_active, gr, dim3(TPB), 0, s, ac, a, b, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, partials, (int)(gr.x * gr.y), 0, 1, out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_dot2_active(const Grid &g, const double *a, const double *b, const double *c, double *partials,
+                        double *out2, hipStream_t s) {
+    const Active ac = active_of(g, 0);
+    if (ac.rowLen <= 0) { launch_fill<double>(2, out2, 0.0, s); return; }
+    const dim3 gr = active_grid(ac);
+    const int cnt = (int)(gr.x * gr.y);
+    hipLaunchKernelGGL(k_dot2_active, gr, dim3(TPB), 0, s, ac, a, b, c, partials, cnt);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, partials, cnt, cnt, 2, out2);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_dot_full(size_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s) {
+    if (!n) { launch_fill<double>(1, out, 0.0, s); return; }
+    const int cnt = reduction_partials_full(n);
+    hipLaunchKernelGGL(k_dot_full, dim3(cnt), dim3(TPB), 0, s, n, a, b, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, partials, cnt, 0, 1, out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_pcg_direction(const Grid &g, const double *s, double *d, const double *sc, int mode, hipStream_t st) {
+    const Active a = active_of(g, 0);
+    if (a.rowLen <= 0) return;
+    hipLaunchKernelGGL(k_pcg_direction, active_grid(a), dim3(TPB), 0, st, a, s, d, sc, mode);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_pcg_update(const Grid &g, double *x, double *r, const double *d, const double *Ad, const double *sc,
+                       double *partials, double *rr_out, hipStream_t st) {
+    const Active a = active_of(g, 0);
+    if (a.rowLen <= 0) { launch_fill<double>(1, rr_out, 0.0, st); return; }
+    const dim3 gr = active_grid(a);
+    hipLaunchKernelGGL(k_pcg_update, gr, dim3(TPB), 0, st, a, x, r, d, Ad, sc, partials);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, st, partials, (int)(gr.x * gr.y), 0, 1, rr_out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_pcg_scalars(double *sc, int op, hipStream_t st) {
+    hipLaunchKernelGGL(k_pcg_scalars, dim3(1), dim3(64), 0, st, sc, op);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_zero_dirichlet(int count, const int *nodes, const uint8_t *masks, int nnodes, T *u, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_zero_dirichlet<T>, dim3(nblocks(count)), dim3(TPB), 0, s, count, nodes, masks, nnodes, u);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_enforce_dirichlet(int count, const int *nodes, const uint8_t *masks, int nnodes, const double *vals,
+                              double *u, hipStream_t s) {
+    if (!count) return;
+    hipLaunchKernelGGL(k_enforce_dirichlet, dim3(nblocks(count)), dim3(TPB), 0, s, count, nodes, masks, nnodes, vals, u);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_young(const Grid &g, const double *rho, double *E, float *E32, int law, double E0, double Emin,
+                  double gamma, double q, int maskActive, hipStream_t s) {
+    hipLaunchKernelGGL(k_young, dim3(nblocks(g.nelems)), dim3(TPB), 0, s, g, rho, E, E32, law, E0, Emin, gamma, q,
+                       maskActive);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_zero_young_layers(const Grid &g, int y0, int y1, double *E, float *E32, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_young_layers, dim3(nblocks(g.nelems)), dim3(TPB), 0, s, g, y0, y1, E, E32);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_compliance_gradient(const Grid &g, const double *K0, const double *rho, const double *u, double *out,
+                                int law, double E0, double Emin, double gamma, double q, const double *grav, double vol,
+                                int accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(k_compliance_gradient, dim3(nblocks(g.nelems)), dim3(TPB), 0, s, g, K0, rho, u, out, law, E0,
+                       Emin, gamma, q, grav[0], grav[1], grav[2], vol, accumulate);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_energy_density(const Grid &g, const double *K0, const double *E, const double *u, double *out,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_energy_density, dim3(nblocks(g.nelems)), dim3(TPB), 0, s, g, K0, E, u, out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_self_weight(const Grid &g, const double *rho, const double *grav, double vol, double *f, hipStream_t s) {
+    hipLaunchKernelGGL(k_self_weight, dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g, rho, grav[0], grav[1], grav[2], vol, f);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_smoothing_filter(const int dims[3], double rmin, int type, const double *in, double *out, hipStream_t s) {
+    const int R = (int)ceil(rmin) - 1;
+    double W = 0;
+    for (int a = -R; a <= R; ++a)
+        for (int b = -R; b <= R; ++b)
+            for (int c = -R; c <= R; ++c) {
+                const double w = type == 1 ? rmin - sqrt((double)(a * a + b * b + c * c)) : 1.0;
+                if (w > 0) W += w;
+            }
+    const long n = (long)dims[0] * dims[1] * dims[2];
+    hipLaunchKernelGGL(k_smoothing, dim3(nblocks(n)), dim3(TPB), 0, s, dims[0], dims[1], dims[2], R, rmin, type, W, in,
+                       out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_projection(double beta, size_t n, const double *in, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_projection, dim3(nblocks(n)), dim3(TPB), 0, s, beta, tanh(0.5 * beta), n, in, out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_projection_backprop(double beta, size_t n, const double *g, const double *vars, double *out,
+                                hipStream_t s) {
+    const double scale = 1.0 / (2 * tanh(0.5 * beta) / beta);
+    hipLaunchKernelGGL(k_projection_bp, dim3(nblocks(n)), dim3(TPB), 0, s, beta, scale, n, g, vars, out);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_oc_update(size_t n, const double *x0, const double *dJ, const double *dc, double lambda, double move,
+                      double p, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_oc_update, dim3(nblocks(n)), dim3(TPB), 0, s, n, x0, dJ, dc, lambda, move, p, out);
+    VF_HIP(hipGetLastError());
+}
+
+template void launch_fill_active<double>(const Grid &, int, double *, double, hipStream_t);
+template void launch_fill_active<float>(const Grid &, int, float *, float, hipStream_t);
+template void launch_fill<double>(size_t, double *, double, hipStream_t);
+template void launch_fill<float>(size_t, float *, float, hipStream_t);
+template void launch_convert<float, double>(size_t, const double *, float *, hipStream_t);
+template void launch_convert<double, float>(size_t, const float *, double *, hipStream_t);
+template void launch_copy_active<double, double>(const Grid &, int, const double *, double *, hipStream_t);
+template void launch_copy_active<float, double>(const Grid &, int, const double *, float *, hipStream_t);
+template void launch_copy_active<double, float>(const Grid &, int, const float *, double *, hipStream_t);
+template void launch_zero_dirichlet<double>(int, const int *, const uint8_t *, int, double *, hipStream_t);
+template void launch_zero_dirichlet<float>(int, const int *, const uint8_t *, int, float *, hipStream_t);
+
+}  // namespace vf
