@@ -321,3 +321,69 @@ def oc_update(x0, dJ, dc, lam, m=0.2, p=0.5):
     _check(lib().orc_oc_update(ctypes.c_long(x0.size), _p(x0), _p(dJ), _p(dc), ctypes.c_double(lam),
                                ctypes.c_double(m), ctypes.c_double(p), _p(out)))
     return out
+
+
+# ------------------------------------------------------------------------------------------------
+# Filter chain, volume constraint and one optimality-criterion step, composed from the primitives
+# above (TopologyOptimizationFilter.hh:142-175, TopologyOptimizationConstraint.hh:30-37,
+# OptimalityCriterion.hh:51-138).
+class Chain:
+    def __init__(self, dims, rmin=0.0, linear=True, beta=0.0):
+        self.dims, self.rmin, self.linear, self.beta = tuple(dims), rmin, linear, beta
+
+    def forward(self, x):
+        s = smoothing_filter(x, self.dims, self.rmin, self.linear) if self.rmin > 0 else np.asarray(x, float).copy()
+        p = projection(s, self.beta) if self.beta > 0 else s
+        return s, p
+
+    def backprop(self, g, x):
+        s, _ = self.forward(x)
+        if self.beta > 0:
+            g = projection_backprop(g, s, self.beta)
+        if self.rmin > 0:
+            g = smoothing_filter(g, self.dims, self.rmin, self.linear)
+        return g
+
+
+class OC:
+    """OCOptimizer restated: the lambda bracket persists across steps (OptimalityCriterion.hh:47-48)."""
+
+    def __init__(self):
+        self.lmin, self.lmax = 1.0, 2.0
+
+    def step(self, x0, dJ, dc, vf, chain, m=0.2, p=0.5, ctol=1e-6, guard=100):
+        def ceval(lam):
+            xs = oc_update(x0, dJ, dc, lam, m, p)
+            return 1.0 - chain.forward(xs)[1].mean() / vf, xs
+
+        dil = 32.0
+        mid = 0.5 * (self.lmin + self.lmax)
+        self.lmax = dil * self.lmax + (1 - dil) * mid
+        self.lmin = max(dil * self.lmin + (1 - dil) * mid, 0.01)
+        nit = 0
+        while nit < guard:
+            if ceval(self.lmin)[0] < 0:
+                break
+            self.lmax = self.lmin
+            self.lmin /= 2
+            nit += 1
+        if nit == guard:
+            raise RuntimeError("Bracketing constraint(lambda_min) < 0 failed")
+        if nit == 0:
+            while nit < guard:
+                if ceval(self.lmax)[0] > 0:
+                    break
+                self.lmin = self.lmax
+                self.lmax *= 2
+                nit += 1
+        if nit == guard:
+            raise RuntimeError("Bracketing constraint(lambda_max) > 0 failed")
+        while True:
+            mid = 0.5 * (self.lmin + self.lmax)
+            v, xs = ceval(mid)
+            if abs(v) <= ctol:
+                return xs
+            if v < 0:
+                self.lmin = mid
+            if v > 0:
+                self.lmax = mid

@@ -194,20 +194,28 @@ __global__ void __launch_bounds__(256) k_pivot_inverse(int p, int ld, const doub
     for (int w = t; w < BS * BS; w += 256) D[w / BS][w % BS] = A[(base + w / BS) * ld + base + w % BS];
     __syncthreads();
     for (int k = 0; k < BS; ++k) {
-        const double d = D[k][k];
+        const double dinv = 1.0 / D[k][k];
         if (t < BS) {
             colk[t] = D[t][k];
-            rowk[t] = D[k][t];
+            rowk[t] = D[k][t] * dinv;
         }
         __syncthreads();
-        for (int w = t; w < BS * BS; w += 256) {
-            const int i = w / BS, j = w % BS;
-            double v;
-            if (i == k && j == k) v = -1.0 / d;
-            else if (i == k) v = rowk[j] / d;
-            else if (j == k) v = colk[i] / d;
-            else v = D[i][j] - colk[i] * rowk[j] / d;
-            D[i][j] = v;
+        // each work-item owns a 4x4 sub-block: rows (t/16)*4.., columns (t%16)*4..
+        const int i0 = (t >> 4) * 4, j0 = (t & 15) * 4;
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const int i = i0 + a;
+            const double ci = colk[i];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const int j = j0 + b;
+                double v;
+                if (i == k && j == k) v = -dinv;
+                else if (i == k) v = rowk[j];
+                else if (j == k) v = ci * dinv;
+                else v = D[i][j] - ci * rowk[j];
+                D[i][j] = v;
+            }
         }
         __syncthreads();
     }

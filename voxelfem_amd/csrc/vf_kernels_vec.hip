@@ -10,7 +10,7 @@ namespace vf {
 namespace {
 
 constexpr int TPB = 256;
-constexpr int ITEMS = 4;  // elements per work-item in the active-region kernels
+constexpr int ITEMS = 8;  // elements per work-item in the active-region kernels
 inline unsigned nblocks(long count) { return (unsigned)((count + TPB - 1) / TPB); }
 
 // The active region of a level: for component c and x-slab ix, the contiguous node range
@@ -126,16 +126,33 @@ __global__ void __launch_bounds__(TPB) k_dot_full(size_t n, const double *__rest
     if (threadIdx.x == 0) partials[blockIdx.x] = s;
 }
 
-// single-workgroup fixed-order sum of `count` partials; writes out[0] (and out[1] for the second
-// set at partials + stride when nsets == 2)
-__global__ void __launch_bounds__(TPB) k_final_sum(const double *__restrict__ partials, int count, int stride,
-                                                   int nsets, double *__restrict__ out) {
-    __shared__ double sh[TPB / 64];
+// Fixed-order sum of `count` partials (per set; set k starts at partials + k*stride): 1024 work-items
+// each add a fixed strided subset, then a fixed tree. Writes out[set].
+constexpr int FTPB = 1024;
+__global__ void __launch_bounds__(FTPB) k_final_sum(const double *__restrict__ partials, int count, int stride,
+                                                    int nsets, double *__restrict__ out) {
+    __shared__ double sh[FTPB / 64];
     for (int set = 0; set < nsets; ++set) {
-        double s = 0;
-        for (int i = threadIdx.x; i < count; i += TPB) s += partials[set * stride + i];
-        s = block_sum(s, sh);
-        if (threadIdx.x == 0) out[set] = s;
+        const double *p = partials + (size_t)set * stride;
+        double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+        int i = threadIdx.x;
+        for (; i + 3 * FTPB < count; i += 4 * FTPB) {
+            s0 += p[i];
+            s1 += p[i + FTPB];
+            s2 += p[i + 2 * FTPB];
+            s3 += p[i + 3 * FTPB];
+        }
+        for (; i < count; i += FTPB) s0 += p[i];
+        double s = (s0 + s1) + (s2 + s3);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        if (lane == 0) sh[w] = s;
+        __syncthreads();
+        if (threadIdx.x < 64) {
+            double t = threadIdx.x < FTPB / 64 ? sh[threadIdx.x] : 0.0;
+            for (int o = 32; o > 0; o >>= 1) t += __shfl_xor(t, o, 64);
+            if (threadIdx.x == 0) out[set] = t;
+        }
         __syncthreads();
     }
 }
@@ -412,7 +429,7 @@ void launch_dot_active(const Grid &g, const double *a, const double *b, double *
     if (ac.rowLen <= 0) { launch_fill<double>(1, out, 0.0, s); return; }
     const dim3 gr = active_grid(ac);
     hipLaunchKernelGGL(k_dot_active, gr, dim3(TPB), 0, s, ac, a, b, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, partials, (int)(gr.x * gr.y), 0, 1, out);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, s, partials, (int)(gr.x * gr.y), 0, 1, out);
     VF_HIP(hipGetLastError());
 }
 
@@ -423,7 +440,7 @@ void launch_dot2_active(const Grid &g, const double *a, const double *b, const d
     const dim3 gr = active_grid(ac);
     const int cnt = (int)(gr.x * gr.y);
     hipLaunchKernelGGL(k_dot2_active, gr, dim3(TPB), 0, s, ac, a, b, c, partials, cnt);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, partials, cnt, cnt, 2, out2);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, s, partials, cnt, cnt, 2, out2);
     VF_HIP(hipGetLastError());
 }
 
@@ -431,7 +448,7 @@ void launch_dot_full(size_t n, const double *a, const double *b, double *partial
     if (!n) { launch_fill<double>(1, out, 0.0, s); return; }
     const int cnt = reduction_partials_full(n);
     hipLaunchKernelGGL(k_dot_full, dim3(cnt), dim3(TPB), 0, s, n, a, b, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, s, partials, cnt, 0, 1, out);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, s, partials, cnt, 0, 1, out);
     VF_HIP(hipGetLastError());
 }
 
@@ -448,7 +465,7 @@ void launch_pcg_update(const Grid &g, double *x, double *r, const double *d, con
     if (a.rowLen <= 0) { launch_fill<double>(1, rr_out, 0.0, st); return; }
     const dim3 gr = active_grid(a);
     hipLaunchKernelGGL(k_pcg_update, gr, dim3(TPB), 0, st, a, x, r, d, Ad, sc, partials);
-    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(TPB), 0, st, partials, (int)(gr.x * gr.y), 0, 1, rr_out);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, st, partials, (int)(gr.x * gr.y), 0, 1, rr_out);
     VF_HIP(hipGetLastError());
 }
 

@@ -76,41 +76,21 @@ def write_json(obj, directory=None, name="bc.json"):
 
 
 def configure(pv, ne, bc, rho, directory=None, Emin=1e-9, symmetry_x=False):
-    """Create a pyVoxelFEM simulator with the standard material/law, `bc` (a .bc dict) and densities."""
+    """Create a pyVoxelFEM simulator with the standard material/law, `bc` (a .bc dict), optional
+    u_x = 0 symmetry on the x = 0 face, and densities."""
     tps = pv.TensorProductSimulator([1, 1, 1], [[0.0, 0.0, 0.0], [float(n) for n in ne]], list(ne))
     d = directory or tempfile.mkdtemp(prefix="vf_wl_")
     tps.readMaterial(write_json(MATERIAL, d, "material.json"))
     tps.E_0 = 1.0
     tps.E_min = Emin
     tps.gamma = 3.0
+    if bc is not None:
+        tps.applyDisplacementsAndLoadsFromFile(write_json(bc, d))
     if symmetry_x:
         tps.applySymmetryConditions([True, False, False], [False, False, False])
-    if bc is not None:
-        if symmetry_x:
-            # the .bc loader refuses to extend existing conditions (as the reference does); merge by
-            # applying the file first on a fresh simulator is not possible, so add the regions directly
-            _apply_regions(tps, ne, bc)
-        else:
-            tps.applyDisplacementsAndLoadsFromFile(write_json(bc, d))
     if rho is not None:
         tps.setDensities(np.asarray(rho, dtype=np.float64))
     return tps
-
-
-def _apply_regions(tps, ne, bc):
-    f = np.zeros((tps.numNodes(), 3))
-    X = _node_positions(ne)
-    size = np.asarray(ne, dtype=np.float64)
-    for reg in bc["regions"]:
-        lo = np.asarray(reg["box%"]["minCorner"]) * size
-        hi = np.asarray(reg["box%"]["maxCorner"]) * size
-        inside = np.all((X >= lo) & (X <= hi), axis=1)
-        if reg["type"] == "force":
-            f[inside] = np.asarray(reg["value"], dtype=np.float64) / inside.sum()
-        else:
-            comps = reg["type"][len("dirichlet"):] or "xyz"
-            tps.addDirichletCondition(reg["value"], lo.tolist(), hi.tolist(), comps)
-    tps.setForces(f)
 
 
 def _node_positions(ne):
@@ -131,3 +111,19 @@ def cantilever_load(ne):
 def cantilever_mask(ne):
     X = _node_positions(ne)
     return np.where(X[:, 0] == 0, 7, 0).astype(np.uint8)
+
+
+def mbb_mask(ne):
+    """u_x = 0 on x = 0 (symmetry) and (u_y, u_z) = 0 on the edge x = L, y = 0 (roller)."""
+    X = _node_positions(ne)
+    m = np.where(X[:, 0] == 0, 1, 0).astype(np.uint8)
+    m[(X[:, 0] == ne[0]) & (X[:, 1] == 0)] |= 6
+    return m
+
+
+def mbb_load(ne):
+    X = _node_positions(ne)
+    top = (X[:, 0] == 0) & (X[:, 1] == ne[1])
+    f = np.zeros((len(X), 3))
+    f[top, 1] = -1.0 / top.sum()
+    return f
