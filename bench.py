@@ -2,7 +2,8 @@
 cantilever (43,022,595 DoF), FP32-preconditioned (mixed) vs all-FP64, plus fine-level K.u GDoF/s.
 
 One step = one full MG-PCG solve from a zero initial guess (Galerkin hierarchy setup, coarse FP64
-factorisation and all PCG iterations included) with the load and densities already resident in HBM.
+factorisation and all PCG iterations included, as in every iteration of a design loop) with the load
+and densities already resident in HBM; setup and PCG-only times are reported beside it.
 Prints one JSON line.
 """
 import argparse
@@ -109,28 +110,42 @@ def main():
     x = torch.zeros_like(f)
     torch.cuda.synchronize()
 
+    L = capi.lib()
+
+    def setup():
+        # Galerkin hierarchy (level-1 operator, assembled block rows, coarsest dense FP64 inverse)
+        capi.check(L.vf_mg_update_stiffness(mg_h))
+
     def solve(prec):
         x.zero_()
         return capi.pcg(mg_h, x.data_ptr(), f.data_ptr(), 1000, args.tol, 1, 1, False, prec)
 
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = None
+        for _ in range(reps):
+            out = fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps, out
+
     results = {}
     for name, prec in (("fp64", capi.VF_PREC_FP64), ("mixed", capi.VF_PREC_FP32)):
         for _ in range(args.warmup):
+            setup()
             solve(prec)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        stats = None
-        for _ in range(args.steps):
-            stats = solve(prec)
-        torch.cuda.synchronize()
-        dt = (time.perf_counter() - t0) / args.steps
+        # one step = hierarchy setup + PCG solve, as in every design iteration of an optimisation
+        dt, stats = timed(lambda: (setup(), solve(prec))[1], args.steps)
         comp = 0.5 * float(torch.dot(f, x).item())
-        results[name] = {"s": dt, "iterations": int(stats.iterations),
+        t_setup, _ = timed(setup, 1)
+        t_solve, _ = timed(lambda: solve(prec), args.steps)
+        results[name] = {"s": dt, "setup_s": t_setup, "solve_s": t_solve, "iterations": int(stats.iterations),
                          "rel_residual": stats.residual_norm / stats.rhs_norm, "compliance": comp}
 
     # fine-level K.u kernel, timed with HIP events on the simulator's stream
-    ku_ms64 = capi.bench_apply_K(sim_h, args.ku_reps, capi.VF_PREC_FP64)
-    ku_ms32 = capi.bench_apply_K(sim_h, args.ku_reps, capi.VF_PREC_FP32)
+    tuned = {"fp64": capi.tuning(mg_h, capi.VF_PREC_FP64), "fp32": capi.tuning(mg_h, capi.VF_PREC_FP32)}
+    ku_ms64 = capi.bench_mg_kernel(mg_h, 0, 0, args.ku_reps, capi.VF_PREC_FP64)
+    ku_ms32 = capi.bench_mg_kernel(mg_h, 0, 0, args.ku_reps, capi.VF_PREC_FP32)
     bytes64 = 48.0 * nn + 8.0 * nel
     flops = 1152.0 * nel
     t64 = ku_ms64 * 1e-3
@@ -141,8 +156,17 @@ def main():
         roof = {"bound": "hbm", "achieved": bytes64 / t64 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
     roof["traffic"] = None
-    roof["kernel"] = "k_fine_apply<double> (fine K.u, FP64)"
+    roof["kernel"] = "fine K.u, FP64, " + {"tiled": "k_ku_tiled<double,0> (LDS-tiled node rows)",
+                                           "walsh": "k_ku_walsh<double,0> (element-centric, Walsh basis)",
+                                           "gather": "k_fine_apply<double,0> (register-gather)"}[tuned["fp64"][0]]
     roof["avg_us"] = ku_ms64 * 1e3
+    # fine smoothing sweep (8 colours): 2 x (576 + 72) flops per node update in the dense count
+    sweep = {}
+    for pname, prec, peak in (("fp64", capi.VF_PREC_FP64, PEAK_FP64_TFLOPS), ("fp32", capi.VF_PREC_FP32,
+                                                                               PEAK_FP32_TFLOPS)):
+        ms = capi.bench_mg_kernel(mg_h, 2, 0, args.ku_reps, prec)
+        tf = 1296.0 * nn / (ms * 1e-3) / 1e12
+        sweep[pname] = {"avg_us": ms * 1e3, "tflops": tf, "frac_of_peak": tf / peak}
 
     line = {
         "metric": "MG-PCG solve s to 1e-8 rel. residual @43M DoF, mixed vs FP64; K·u GDoF/s",
@@ -160,9 +184,15 @@ def main():
         "config": {"workload": f"cantilever {ne[0]}x{ne[1]}x{ne[2]} Q1 hex, uniform density 0.3, E0=1, "
                                f"Emin=1e-9, nu=0.3, SIMP p=3", "dof": 3 * nn, "mg_levels": levels,
                    "smoother": "8-colour block Gauss-Seidel, 1 pre + 1 post sweep", "tol": args.tol},
-        "fp64_solve_s": results["fp64"]["s"],
-        "mixed_solve_s": results["mixed"]["s"],
+        "fp64_step_s": results["fp64"]["s"],
+        "mixed_step_s": results["mixed"]["s"],
         "speedup_mixed_vs_fp64": results["fp64"]["s"] / results["mixed"]["s"],
+        "setup_s": {"fp64": results["fp64"]["setup_s"], "mixed": results["mixed"]["setup_s"]},
+        "pcg_only_s": {"fp64": results["fp64"]["solve_s"], "mixed": results["mixed"]["solve_s"]},
+        "speedup_pcg_only": results["fp64"]["solve_s"] / results["mixed"]["solve_s"],
+        "fine_sweep": sweep,
+        "fine_kernels": {p: {"K.u": t[0], "sweep": t[1], "tuner_ms": [round(v, 4) for v in t[2]]}
+                         for p, t in tuned.items()},
         "pcg_iterations": {"fp64": results["fp64"]["iterations"], "mixed": results["mixed"]["iterations"]},
         "final_rel_residual": {"fp64": results["fp64"]["rel_residual"], "mixed": results["mixed"]["rel_residual"]},
         "compliance": {"fp64": results["fp64"]["compliance"], "mixed": results["mixed"]["compliance"],

@@ -183,10 +183,22 @@ int vf_oc_update(int64_t n, const double *x0, const double *dJ, const double *dc
 /* mean of n values (TotalVolumeConstraint::evaluate, TopologyOptimizationConstraint.hh:30-32) */
 int vf_mean(int64_t n, const double *x, int where, void *stream, double *out);
 
-/* ---- measurement helpers (no reference counterpart: used by bench.py) ---------------------- */
-/* Launch `reps` fine-level K u products of the simulator on its stream and return the average
- * kernel time measured with HIP events recorded on that stream. precision: VF_PREC_FP64/FP32. */
-int vf_bench_apply_K(vf_sim *sim, int64_t reps, int precision, double *avg_ms);
+/* ---- measurement helpers (no reference counterpart: used by bench.py and the tests) ---------- */
+/* Select the fine-level kernels process-wide: 0 (default) = per solver and precision, the fastest
+ * family on the solver's grid (timed once, on first use); 1 = register-gather kernels with one launch
+ * per colour; 2 = LDS-tiled kernels on the structured element matrix with the two-launch smoother;
+ * 3 = element-centric Walsh-basis K u with the LDS-tiled smoother. All families compute the same
+ * operators; the tests compare them. Variant 1 also keeps level 1 matrix-free. */
+int vf_set_kernel_variant(int variant);
+/* Fine-level kernel families the solver uses in `precision` (running the autotuner if it has not run):
+ * choice[0] for K u (0 register-gather, 1 LDS-tiled, 2 Walsh), choice[1] for the smoother
+ * (0 register-gather, 1 LDS-tiled); ms[] = the autotuner's timings (gather, tiled, Walsh K u; gather,
+ * tiled sweep; 0 where it did not run). */
+int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[5]);
+/* Average time of one level-`level` operation of the hierarchy over `reps` launches (HIP events on
+ * the solver's stream, vectors allocated and filled beforehand): which = 0 K u, 1 residual b - K u,
+ * 2 one forward smoothing sweep (all eight colours). */
+int vf_bench_mg_kernel(vf_mg *mg, int which, int64_t level, int64_t reps, int precision, double *avg_ms);
 
 #ifdef __cplusplus
 }

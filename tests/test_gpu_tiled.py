@@ -1,0 +1,123 @@
+"""GPU tests of the LDS-tiled fine-level kernels (structured element matrix, two-launch smoother) against
+the register-gather kernels (one launch per colour) and against the oracle.
+
+Grids are chosen so the (y, z) tiles split unevenly (node counts 39 and 71 against tiles of 16 and 32),
+with fabrication mask layers (detached rows that the smoother must leave untouched) and with partially
+constrained nodes (the MBB roller and symmetry plane) that take the sequential point update.
+
+The element-centric Walsh-basis K u (variant 3) is checked the same way. Tolerances: K u and one
+smoothing sweep agree to 1e-13 relative between kernel families (they round in a different order); a
+V-cycle in FP64 to 1e-12; in FP32 to 1e-5 (single-precision rounding).
+"""
+import ctypes
+
+import numpy as np
+import pytest
+
+from oracle import oracle as orc
+from voxelfem_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(1234)
+
+
+@pytest.fixture(scope="module")
+def vf():
+    import voxelfem_amd
+
+    yield voxelfem_amd
+    _variant(0)
+
+
+def _variant(v):
+    from voxelfem_amd import capi
+
+    L = capi.lib()
+    L.vf_set_kernel_variant.argtypes = [ctypes.c_int]
+    capi.check(L.vf_set_kernel_variant(v))
+
+
+def rel(a, b):
+    return np.abs(np.asarray(a) - np.asarray(b)).max() / max(np.abs(np.asarray(b)).max(), 1e-300)
+
+
+def both(fn, variant=2):
+    """fn() evaluated with the tiled (2) or Walsh (3) fine kernels and with the register-gather ones (1)."""
+    _variant(variant)
+    a = fn()
+    _variant(1)
+    try:
+        b = fn()
+    finally:
+        _variant(0)
+    return a, b
+
+
+CASES = [
+    ((10, 38, 70), wl.CANTILEVER_BC, False, None),
+    ((10, 38, 70), wl.MBB_BC, True, None),
+    ((12, 38, 22), wl.CANTILEVER_BC, False, 17),
+    ((8, 20, 40), wl.MBB_BC, True, 9),
+]
+
+
+@pytest.mark.parametrize("ne,bc,sym,layer", CASES)
+def test_tiled_fine_operators_match_gather(vf, ne, bc, sym, layer):
+    pv = vf.pyVoxelFEM
+    rho = wl.noise_density(ne, 3)
+    tps = wl.configure(pv, ne, bc, rho, symmetry_x=sym)
+    mg = tps.multigridSolver(1)
+    if layer is not None:
+        tps.setFabricationMaskHeightByLayer(layer)
+        mg.setFabricationMaskHeightByLayer(layer)
+    nn = tps.numNodes()
+    u = RNG.normal(size=(nn, 3))
+    b = RNG.normal(size=(nn, 3))
+    for var in (2, 3):
+        ku_t, ku_g = both(lambda: tps.applyK(u), var)
+        assert rel(ku_t, ku_g) < 1e-13, var
+        r_t, r_g = both(lambda: mg.computeResidual(0, u, b), var)
+        assert rel(r_t, r_g) < 1e-13, var
+    for fwd in (True, False):
+        s_t, s_g = both(lambda: mg._smooth(0, u, b, fwd))
+        assert rel(s_t, s_g) < 1e-13, fwd
+        if layer is not None:
+            # rows of detached nodes keep their input values
+            ny = ne[1] + 1
+            iy = (np.arange(nn) // (ne[2] + 1)) % ny
+            det = iy >= layer + 1
+            assert np.array_equal(s_t[det], u[det])
+
+
+@pytest.mark.parametrize("ne,bc,sym,layer", CASES[:2])
+def test_tiled_smoother_matches_oracle(vf, ne, bc, sym, layer):
+    pv = vf.pyVoxelFEM
+    rho = wl.noise_density(ne, 4)
+    tps = wl.configure(pv, ne, bc, rho, symmetry_x=sym)
+    mg = tps.multigridSolver(1)
+    s = orc.Sim(ne)
+    s.set_etensor(1.0, 0.3)
+    s.set_law(0, 1.0, 1e-9, 3.0, 3.0)
+    s.set_densities(rho)
+    m = tps.getDirichletMask().astype(np.uint8)
+    s.set_dirichlet((m[:, 0] | (m[:, 1] << 1) | (m[:, 2] << 2)).astype(np.uint8))
+    om = orc.MG(s, 1)
+    nn = tps.numNodes()
+    u = RNG.normal(size=(nn, 3))
+    b = RNG.normal(size=(nn, 3))
+    for fwd in (True, False):
+        assert rel(mg._smooth(0, u, b, fwd), om.smooth(0, u, b, fwd)) < 1e-12, fwd
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+@pytest.mark.parametrize("variant", [2, 3])
+def test_tiled_vcycle_matches_gather(vf, prec, tol, variant):
+    pv = vf.pyVoxelFEM
+    ne = (16, 24, 40)
+    tps = wl.configure(pv, ne, wl.MBB_BC, wl.noise_density(ne, 5), symmetry_x=True)
+    mg = tps.multigridSolver(3)
+    r = RNG.normal(size=(tps.numNodes(), 3))
+    for nu in (1, 2):
+        a, b = both(lambda: mg._precondition(r, 1, nu, False, prec), variant)
+        assert rel(a, b) < tol, nu

@@ -24,10 +24,14 @@ def lib():
         vp, i64, dbl, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_double, ctypes.c_int
         L.vf_mg_pcg.argtypes = [vp, vp, vp, i64, dbl, i64, i64, i32, i32, i32, vp, vp, i32, ctypes.POINTER(PcgStats)]
         L.vf_bench_apply_K.argtypes = [vp, i64, i32, ctypes.POINTER(dbl)]
+        L.vf_bench_mg_kernel.argtypes = [vp, i32, i64, i64, i32, ctypes.POINTER(dbl)]
+        L.vf_set_kernel_variant.argtypes = [i32]
+        L.vf_mg_tuning.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(dbl)]
         L.vf_sim_set_densities.argtypes = [vp, vp, i32]
         L.vf_sim_build_load.argtypes = [vp, vp, i32]
         L.vf_sim_compliance_gradient.argtypes = [vp, vp, vp, i32]
         L.vf_sim_dot.argtypes = [vp, vp, vp, i32, i32, ctypes.POINTER(dbl)]
+        L.vf_sim_apply_K.argtypes = [vp, vp, vp, i32]
         L.vf_smoothing_filter.argtypes = [ctypes.POINTER(i64), dbl, i32, vp, vp, i32, vp]
         L.vf_projection.argtypes = [dbl, i64, vp, vp, i32, vp]
         L.vf_projection_backprop.argtypes = [dbl, i64, vp, vp, vp, i32, vp]
@@ -49,6 +53,28 @@ def pcg(mg_handle, x_ptr, b_ptr, max_iter, tol, mg_iterations=1, smoothing=1, fm
                           int(precision), int(bool(dirichlet_satisfied)), None, None, VF_DEVICE,
                           ctypes.byref(st)))
     return st
+
+
+def bench_mg_kernel(mg_handle, which, level, reps, precision=VF_PREC_FP64):
+    """which: 0 K u, 1 residual, 2 one forward smoothing sweep; returns the average ms per call."""
+    ms = ctypes.c_double()
+    check(lib().vf_bench_mg_kernel(ctypes.c_void_p(mg_handle), int(which), int(level), int(reps), int(precision),
+                                   ctypes.byref(ms)))
+    return ms.value
+
+
+def set_kernel_variant(variant):
+    """0: autotuned per solver and precision (default); 1: register-gather; 2: LDS-tiled fine kernels;
+    3: Walsh-basis K u with the LDS-tiled smoother."""
+    check(lib().vf_set_kernel_variant(int(variant)))
+
+
+def tuning(mg_handle, precision=VF_PREC_FP64):
+    """(K u family, smoother family, timings ms [gather, tiled, Walsh K u; gather, tiled sweep])."""
+    ch = (ctypes.c_int * 2)()
+    ms = (ctypes.c_double * 5)()
+    check(lib().vf_mg_tuning(ctypes.c_void_p(mg_handle), int(precision), ch, ms))
+    return ("gather", "tiled", "walsh")[ch[0]], ("gather", "tiled")[ch[1]], list(ms)
 
 
 def bench_apply_K(sim_handle, reps, precision=VF_PREC_FP64):

@@ -24,6 +24,10 @@ namespace {
 
 thread_local std::string g_err;
 constexpr int64_t LAYER_NONE = INT64_MAX;
+// Fine-level kernel selection: 0 = chosen per solver and precision by timing both families on the
+// solver's own grid (autotuned), 1 = register-gather kernels, 2 = LDS-tiled structured kernels (where
+// the element matrix has the structure they need).
+int g_kernel_variant = 0;
 
 void sync(hipStream_t s) { VF_HIP(hipStreamSynchronize(s)); }
 
@@ -126,6 +130,12 @@ struct vf_sim {
     double K0h[576];
     DBuf<double> K0;
     DBuf<float> K032;
+    DBuf<double> Ftab;  // structured element matrix (see update_K0)
+    DBuf<float> Ftab32;
+    bool structured = false;
+    DBuf<double> Wtab;  // Walsh-basis element matrix entries (see update_K0)
+    DBuf<float> Wtab32;
+    bool walsh = false;
     DBuf<double> rho, E;
     DBuf<float> E32;
     std::vector<uint8_t> hmask;
@@ -157,6 +167,66 @@ struct vf_sim {
         for (int i = 0; i < 576; ++i) f[i] = (float)K0h[i];
         h2d(K0.data(), K0h, 576, stream);
         h2d(K032.data(), f, 576, stream);
+        // Structured form used by the LDS-tiled fine kernels: K0[(l,c),(l^delta,d)] equals
+        // F[delta][c][d] = K0[(0,c),(delta,d)] times s_c(l) s_d(l) for c != d (s_a(l) = -1 or +1
+        // for the low or high side of axis a), which holds for box elements of an orthotropic tensor.
+        double F[128] = {0};
+        float F32[128] = {0};
+        for (int dl = 0; dl < 8; ++dl)
+            for (int c = 0; c < 3; ++c)
+                for (int d = 0; d < 3; ++d) F[16 * dl + 3 * c + d] = K0h[c * 24 + 3 * dl + d];
+        double kmax = 0, dev = 0;
+        for (int i = 0; i < 576; ++i) kmax = std::max(kmax, std::fabs(K0h[i]));
+        for (int l = 0; l < 8; ++l)
+            for (int dl = 0; dl < 8; ++dl)
+                for (int c = 0; c < 3; ++c)
+                    for (int d = 0; d < 3; ++d) {
+                        const int sc = ((l >> (2 - c)) & 1) ? 1 : -1, sd = ((l >> (2 - d)) & 1) ? 1 : -1;
+                        const double sg = c == d ? 1.0 : double(sc * sd);
+                        const double v = K0h[(3 * l + c) * 24 + 3 * (l ^ dl) + d];
+                        dev = std::max(dev, std::fabs(v - sg * F[16 * dl + 3 * c + d]));
+                    }
+        structured = dev <= 1e-13 * kmax;
+        for (int i = 0; i < 128; ++i) F32[i] = (float)F[i];
+        h2d(Ftab.data(), F, 128, stream);
+        h2d(Ftab32.data(), F32, 128, stream);
+        // Walsh basis: Khat = W K0 W^T with W[k][m] = prod_a s_a(m)^{k_a}; the element-centric kernel
+        // uses Khat[(k,d),(k,d)] and Khat[(k,d),(k^e_d^e_c,c)] (c != d, k_d != k_c), divided by 64.
+        double Kh[576] = {0};
+        auto Wkm = [](int k, int m) {
+            int sgn = 1;
+            for (int a = 0; a < 3; ++a)
+                if ((k >> (2 - a)) & 1) sgn *= ((m >> (2 - a)) & 1) ? 1 : -1;
+            return sgn;
+        };
+        for (int k = 0; k < 8; ++k)
+            for (int l = 0; l < 8; ++l)
+                for (int d = 0; d < 3; ++d)
+                    for (int c = 0; c < 3; ++c) {
+                        double acc = 0;
+                        for (int m = 0; m < 8; ++m)
+                            for (int q = 0; q < 8; ++q)
+                                acc += Wkm(k, m) * K0h[(3 * m + d) * 24 + 3 * q + c] * Wkm(l, q);
+                        Kh[(3 * k + d) * 24 + 3 * l + c] = acc;
+                    }
+        double C[128] = {0}, khmax = 0, off = 0;
+        for (int i = 0; i < 576; ++i) khmax = std::max(khmax, std::fabs(Kh[i]));
+        for (int k = 0; k < 8; ++k)
+            for (int d = 0; d < 3; ++d)
+                for (int l = 0; l < 8; ++l)
+                    for (int c = 0; c < 3; ++c) {
+                        const int bd = 1 << (2 - d), bc = 1 << (2 - c);
+                        const bool kd = (k & bd) != 0, kc = (k & bc) != 0;
+                        const bool used = (c == d && l == k) || (c != d && kd != kc && l == (k ^ bd ^ bc));
+                        const double v = Kh[(3 * k + d) * 24 + 3 * l + c];
+                        if (used) C[16 * k + 3 * d + c] = v / 64.0;
+                        else off = std::max(off, std::fabs(v));
+                    }
+        walsh = off <= 1e-13 * khmax;
+        float C32[128];
+        for (int i = 0; i < 128; ++i) C32[i] = (float)C[i];
+        h2d(Wtab.data(), C, 128, stream);
+        h2d(Wtab32.data(), C32, 128, stream);
         sync(stream);
     }
     void update_E() {
@@ -238,6 +308,23 @@ void upload_dirichlet(const std::vector<uint8_t> &mask, DBuf<uint8_t> &dmask, DB
 
 }  // namespace
 
+template <typename T>
+void sim_apply_K(vf_sim *sim, const T *u, T *y, hipStream_t s) {
+    const T *E, *K0, *F;
+    const T *Wc;
+    if constexpr (std::is_same<T, double>::value) {
+        E = sim->E.data(); K0 = sim->K0.data(); F = sim->Ftab.data(); Wc = sim->Wtab.data();
+    } else {
+        E = sim->E32.data(); K0 = sim->K032.data(); F = sim->Ftab32.data(); Wc = sim->Wtab32.data();
+    }
+    if (sim->walsh && (g_kernel_variant == 0 || g_kernel_variant == 3))
+        launch_fine_apply_walsh<T>(sim->g, Wc, E, u, y, nullptr, nullptr, AM_SET, s);
+    else if (sim->structured && g_kernel_variant != 1)
+        launch_fine_apply_tiled<T>(sim->g, F, E, u, y, nullptr, nullptr, AM_SET, s);
+    else
+        launch_fine_apply<T>(sim->g, K0, E, u, y, nullptr, nullptr, AM_SET, s);
+}
+
 struct vf_mg {
     vf_sim *sim;
     int L;
@@ -247,6 +334,7 @@ struct vf_mg {
     DBuf<double> dChat, dphi, dM2, dKe0;
     DBuf<float> dChat32;
     bool symmetricGS = true;
+    bool l1Blocks = false;  // level 1 held as assembled block rows (set by update_stiffness)
     // coarsest operator
     DBuf<double> Ainv, invScratch, ctmp_in, ctmp_out;
     DBuf<int> dRedToFull, dRed;
@@ -378,10 +466,102 @@ struct vf_mg {
     const T *blocksp(int l) const;
 
     template <typename T>
+    const T *Fp() const;
+    template <typename T>
+    const T *Wp() const;
+    // Fine-level kernel family per precision (index 0 FP64, 1 FP32). K u: 0 register-gather, 1 LDS-tiled
+    // node rows, 2 element-centric Walsh; smoother: 0 register-gather (one launch per colour), 1 LDS-tiled
+    // two-launch sweep. -1 = not tuned yet. See tune().
+    int kuChoice[2] = {-1, -1}, gsChoice[2] = {-1, -1};
+    template <typename T>
+    static constexpr int pidx() { return std::is_same<T, double>::value ? 0 : 1; }
+    template <typename T>
+    int ku_family() const {
+        switch (g_kernel_variant) {
+        case 1: return 0;
+        case 2: return sim->structured ? 1 : 0;
+        case 3: return sim->walsh ? 2 : (sim->structured ? 1 : 0);
+        default: {
+            const int c = kuChoice[pidx<T>()];
+            if (c >= 0) return c;
+            return sim->walsh ? 2 : (sim->structured ? 1 : 0);
+        }
+        }
+    }
+    template <typename T>
+    bool tiled_gs() const {
+        if (!sim->structured || g_kernel_variant == 1) return false;
+        return g_kernel_variant >= 2 || gsChoice[pidx<T>()] != 0;
+    }
+    // Autotuner: time each applicable kernel family on this grid (HIP events on the solver's stream,
+    // best of two runs after a warm-up) and keep the fastest per operation and precision.
+    template <typename T>
+    void tune() {
+        const int p = pidx<T>();
+        if (g_kernel_variant != 0 || !sim->structured || (kuChoice[p] >= 0 && gsChoice[p] >= 0)) return;
+        const Grid &g = lv[0]->g;
+        const size_t n = 3 * (size_t)g.nnodes;
+        hipStream_t s = st();
+        DBuf<T> u(n), y(n), rhs(n);
+        launch_fill<T>(n, u.data(), T(1e-3), s);
+        launch_fill<T>(n, rhs.data(), T(1), s);
+        hipEvent_t a, b;
+        VF_HIP(hipEventCreate(&a));
+        VF_HIP(hipEventCreate(&b));
+        auto best = [&](auto &&fn) {
+            float ms = 1e30f;
+            fn();
+            for (int r = 0; r < 2; ++r) {
+                VF_HIP(hipEventRecord(a, s));
+                fn();
+                VF_HIP(hipEventRecord(b, s));
+                VF_HIP(hipEventSynchronize(b));
+                float t = 0;
+                VF_HIP(hipEventElapsedTime(&t, a, b));
+                ms = std::min(ms, t);
+            }
+            return ms;
+        };
+        const uint8_t *mask = dm(0);
+        float ku[3] = {1e30f, 1e30f, 1e30f};
+        ku[0] = best([&] { launch_fine_apply<T>(g, K0p<T>(), Ep<T>(), u.data(), y.data(), rhs.data(), mask,
+                                                AM_RESIDUAL, s); });
+        ku[1] = best([&] { launch_fine_apply_tiled<T>(g, Fp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(), mask,
+                                                      AM_RESIDUAL, s); });
+        if (sim->walsh)
+            ku[2] = best([&] { launch_fine_apply_walsh<T>(g, Wp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(), mask,
+                                                          AM_RESIDUAL, s); });
+        const float gsT = best([&] {
+            launch_fine_gs_phase<T>(g, Fp<T>(), Ep<T>(), u.data(), u.data(), y.data(), rhs.data(), mask, 0, 1, s);
+            launch_fine_gs_phase<T>(g, Fp<T>(), Ep<T>(), u.data(), y.data(), y.data(), rhs.data(), mask, 1, 1, s);
+        });
+        const float gsG = best([&] {
+            for (int c = 0; c < 8; ++c) launch_fine_gs<T>(g, K0p<T>(), Ep<T>(), y.data(), rhs.data(), mask, c, 1, s);
+        });
+        (void)hipEventDestroy(a);
+        (void)hipEventDestroy(b);
+        int kbest = 0;
+        for (int i = 1; i < 3; ++i)
+            if (ku[i] < ku[kbest]) kbest = i;
+        kuChoice[p] = kbest;
+        gsChoice[p] = gsT <= gsG ? 1 : 0;
+        tuneMs[p][0] = ku[0];
+        tuneMs[p][1] = ku[1];
+        tuneMs[p][2] = sim->walsh ? ku[2] : 0.0f;
+        tuneMs[p][3] = gsG;
+        tuneMs[p][4] = gsT;
+    }
+    float tuneMs[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+
+    template <typename T>
     void apply(int l, const T *u, T *y, const T *b, int mode, bool useMask) {
         const uint8_t *mask = useMask ? dm(l) : nullptr;
-        if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
-        else if (l == 1) launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
+        const int fam = l == 0 && (mode == AM_SET || mode == AM_RESIDUAL) ? ku_family<T>() : 0;
+        if (l == 0 && fam == 2) launch_fine_apply_walsh<T>(lv[0]->g, Wp<T>(), Ep<T>(), u, y, b, mask, mode, st());
+        else if (l == 0 && fam == 1) launch_fine_apply_tiled<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, y, b, mask, mode, st());
+        else if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
+        else if (l == 1 && !(l1Blocks && L >= 2))
+            launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l < L) launch_block_apply<T>(lv[l]->g, blocksp<T>(l), u, y, b, mask, mode, st());
         else {
             if constexpr (std::is_same<T, double>::value) {
@@ -397,12 +577,28 @@ struct vf_mg {
             }
         }
     }
+    // one sweep from uin into uout (uout != uin) on levels with a two-launch smoother
+    template <typename T>
+    void sweep_to(int l, const T *uin, T *uout, const T *b, bool forward) {
+        const int p0 = forward ? 0 : 1;
+        launch_fine_gs_phase<T>(lv[0]->g, Fp<T>(), Ep<T>(), uin, uin, uout, b, dm(0), p0, forward, st());
+        launch_fine_gs_phase<T>(lv[0]->g, Fp<T>(), Ep<T>(), uin, uout, uout, b, dm(0), 1 - p0, forward, st());
+    }
+    template <typename T>
+    bool paired(int l) const { return l == 0 && tiled_gs<T>(); }
     template <typename T>
     void smooth(int l, T *u, const T *b, bool forward) {
+        if (paired<T>(l)) {
+            T *w = R<T>(l);
+            sweep_to<T>(l, u, w, b, forward);
+            d2d(u, w, 3 * (size_t)lv[l]->g.nnodes, st());
+            return;
+        }
         for (int t = 0; t < 8; ++t) {
             const int color = forward ? t : 7 - t;
             if (l == 0) launch_fine_gs<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
-            else if (l == 1) launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, st());
+            else if (l == 1 && !(l1Blocks && L >= 2))
+                launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, st());
             else launch_block_gs<T>(lv[l]->g, blocksp<T>(l), u, b, dm(l), color, forward, st());
         }
     }
@@ -433,6 +629,20 @@ struct vf_mg {
         T *x = X<T>(l);
         const T *b = B<T>(l);
         dirichlet_for_cycle<T>(l, x, residual);
+        if (paired<T>(l)) {
+            // sweeps alternate between x and the residual buffer; the residual goes to whichever of
+            // the two does not hold the iterate, and 2 nu sweeps bring the iterate back to x
+            T *cur = x, *alt = R<T>(l);
+            for (int i = 0; i < nu; ++i) { sweep_to<T>(l, cur, alt, b, true); std::swap(cur, alt); }
+            apply<T>(l, cur, alt, b, AM_RESIDUAL, true);
+            launch_restrict<T>(lv[l]->g, lv[l + 1]->g, alt, B<T>(l + 1), st());
+            launch_fill<T>(3 * (size_t)lv[l + 1]->g.nnodes, X<T>(l + 1), T(0), st());
+            vcycle<T>(l + 1, nu);
+            launch_interpolate<T>(lv[l]->g, lv[l + 1]->g, X<T>(l + 1), cur, 1, st());
+            for (int i = 0; i < nu; ++i) { sweep_to<T>(l, cur, alt, b, !symmetricGS); std::swap(cur, alt); }
+            if (cur != x) d2d(x, cur, 3 * (size_t)lv[l]->g.nnodes, st());
+            return;
+        }
         for (int i = 0; i < nu; ++i) smooth<T>(l, x, b, true);
         apply<T>(l, x, R<T>(l), b, AM_RESIDUAL, true);
         launch_restrict<T>(lv[l]->g, lv[l + 1]->g, R<T>(l), B<T>(l + 1), st());
@@ -468,6 +678,7 @@ struct vf_mg {
             return;
         }
         ensure_buffers<double>(1);
+        tune<double>();
         b0v = const_cast<double *>(f);
         x0v = x;
         for (int i = 0; i < numSteps; ++i) {
@@ -490,6 +701,7 @@ struct vf_mg {
         ensure_stiffness();
         if (prec == VF_PREC_FP64) {
             ensure_buffers<double>(1);
+            tune<double>();
             b0v = const_cast<double *>(r);
             x0v = s;
             launch_fill<double>(n0, s, 0.0, st());
@@ -499,6 +711,7 @@ struct vf_mg {
             }
         } else {
             ensure_buffers<float>(0);
+            tune<float>();
             launch_convert<float, double>(n0, r, lv[0]->b32.data(), st());
             launch_fill<float>(n0, lv[0]->x32.data(), 0.0f, st());
             for (int i = 0; i < mgIter; ++i) {
@@ -533,6 +746,20 @@ struct vf_mg {
             c.Ke.resize((size_t)c.g.nelems * 576);
             launch_galerkin_l1_elems(c.g, lv[0]->g, dChat.data(), sim->E.data(), c.Ke.data(), s);
         } else {
+            // level 1 as assembled block rows (HBM holds them easily; the reference applies this level
+            // on the fly to save memory), unless the register-gather family is forced
+            l1Blocks = g_kernel_variant != 1;
+            if (l1Blocks) {
+                LevelData &c1 = *lv[1];
+                const size_t nb = (size_t)c1.g.nnodes * 243;
+                c1.blocks.resize(nb);
+                c1.blocks32.resize(nb);
+                launch_assemble_l1_blocks(c1.g, lv[0]->g, dChat.data(), sim->E.data(), c1.blocks.data(), s);
+                launch_convert<float, double>(nb, c1.blocks.data(), c1.blocks32.data(), s);
+            } else {
+                lv[1]->blocks.release();
+                lv[1]->blocks32.release();
+            }
             LevelData &c2 = *lv[2];
             c2.Ke.resize((size_t)c2.g.nelems * 576);
             launch_galerkin_l2_elems(c2.g, lv[0]->g, dM2.data(), sim->E.data(), c2.Ke.data(), s);
@@ -690,6 +917,10 @@ template <> double *vf_mg::R<double>(int l) {
 template <> float *vf_mg::X<float>(int l) { return lv[l]->x32.data(); }
 template <> float *vf_mg::B<float>(int l) { return lv[l]->b32.data(); }
 template <> float *vf_mg::R<float>(int l) { return lv[l]->r32.data(); }
+template <> const double *vf_mg::Wp<double>() const { return sim->Wtab.data(); }
+template <> const float *vf_mg::Wp<float>() const { return sim->Wtab32.data(); }
+template <> const double *vf_mg::Fp<double>() const { return sim->Ftab.data(); }
+template <> const float *vf_mg::Fp<float>() const { return sim->Ftab32.data(); }
 template <> const double *vf_mg::K0p<double>() const { return sim->K0.data(); }
 template <> const float *vf_mg::K0p<float>() const { return sim->K032.data(); }
 template <> const double *vf_mg::Ep<double>() const { return sim->E.data(); }
@@ -784,6 +1015,14 @@ int vf_stream_sync(void *stream) {
 }
 const char *vf_version(void) { return "voxelfem-mi355x 0.1 (gfx950)"; }
 
+int vf_set_kernel_variant(int variant) {
+    VF_API_BEGIN
+    need(variant >= 0 && variant <= 3,
+         "kernel variant must be 0 (autotuned), 1 (register-gather), 2 (LDS-tiled) or 3 (Walsh K u, LDS-tiled smoother)");
+    g_kernel_variant = variant;
+    VF_API_END
+}
+
 int vf_device_count(int *count) {
     VF_API_BEGIN
     need(count != nullptr, "null count");
@@ -812,6 +1051,10 @@ int vf_sim_create(vf_sim **out, const int64_t ne[3], const double minCorner[3], 
     s->g = make_grid(ne);
     s->K0.resize(576);
     s->K032.resize(576);
+    s->Ftab.resize(128);
+    s->Ftab32.resize(128);
+    s->Wtab.resize(128);
+    s->Wtab32.resize(128);
     s->rho.resize(s->nel());
     s->E.resize(s->nel());
     s->E32.resize(s->nel());
@@ -961,7 +1204,7 @@ int vf_sim_apply_K(vf_sim *sim, const double *u, double *y, int where) {
     Staged st{sim, where};
     const double *du = st.in(u, n, sim->stage_in(n));
     double *dy = st.out(y, sim->stage_out(n));
-    launch_fine_apply<double>(sim->g, sim->K0.data(), sim->E.data(), du, dy, nullptr, nullptr, AM_SET, sim->stream);
+    sim_apply_K<double>(sim, du, dy, sim->stream);
     st.finish(y, dy, n);
     VF_API_END
 }
@@ -1398,23 +1641,85 @@ int vf_bench_apply_K(vf_sim *sim, int64_t reps, int precision, double *avg_ms) {
     if (precision == VF_PREC_FP64) {
         double *u = sim->stage_in(n), *y = sim->stage_out(n);
         launch_fill<double>(n, u, 1e-3, s);
-        launch_fine_apply<double>(sim->g, sim->K0.data(), sim->E.data(), u, y, nullptr, nullptr, AM_SET, s);
+        sim_apply_K<double>(sim, u, y, s);
         VF_HIP(hipEventRecord(a, s));
         for (int64_t i = 0; i < reps; ++i)
-            launch_fine_apply<double>(sim->g, sim->K0.data(), sim->E.data(), u, y, nullptr, nullptr, AM_SET, s);
+            sim_apply_K<double>(sim, u, y, s);
         VF_HIP(hipEventRecord(b, s));
     } else {
         sim->s32a.resize(n);
         sim->s32b.resize(n);
         float *u = sim->s32a.data(), *y = sim->s32b.data();
         launch_fill<float>(n, u, 1e-3f, s);
-        launch_fine_apply<float>(sim->g, sim->K032.data(), sim->E32.data(), u, y, nullptr, nullptr, AM_SET, s);
+        sim_apply_K<float>(sim, u, y, s);
         VF_HIP(hipEventRecord(a, s));
         for (int64_t i = 0; i < reps; ++i)
-            launch_fine_apply<float>(sim->g, sim->K032.data(), sim->E32.data(), u, y, nullptr, nullptr, AM_SET, s);
+            sim_apply_K<float>(sim, u, y, s);
         VF_HIP(hipEventRecord(b, s));
     }
     VF_HIP(hipEventSynchronize(b));
+    VF_HIP(hipEventElapsedTime(&ms, a, b));
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    *avg_ms = (double)ms / (double)reps;
+    VF_API_END
+}
+
+int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[5]) {
+    VF_API_BEGIN
+    need(mg && choice && ms, "null argument");
+    need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
+    mg->sync_masks();
+    mg->ensure_stiffness();
+    const int p = precision == VF_PREC_FP64 ? 0 : 1;
+    if (p == 0) {
+        mg->tune<double>();
+        choice[0] = mg->ku_family<double>();
+        choice[1] = mg->tiled_gs<double>() ? 1 : 0;
+    } else {
+        mg->tune<float>();
+        choice[0] = mg->ku_family<float>();
+        choice[1] = mg->tiled_gs<float>() ? 1 : 0;
+    }
+    for (int i = 0; i < 5; ++i) ms[i] = mg->tuneMs[p][i];
+    VF_API_END
+}
+
+int vf_bench_mg_kernel(vf_mg *mg, int which, int64_t level, int64_t reps, int precision, double *avg_ms) {
+    VF_API_BEGIN
+    need(mg && avg_ms && reps > 0, "invalid argument");
+    need(level >= 0 && level < mg->L, "level must be a smoothed level (0 <= l < numLevels)");
+    need(which >= 0 && which <= 2, "which must be 0 (K u), 1 (residual) or 2 (one smoothing sweep)");
+    need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
+    mg->sync_masks();
+    mg->ensure_stiffness();
+    const int l = (int)level;
+    const size_t n = 3 * (size_t)mg->lv[l]->g.nnodes;
+    hipStream_t s = mg->st();
+    hipEvent_t a, b;
+    VF_HIP(hipEventCreate(&a));
+    VF_HIP(hipEventCreate(&b));
+    auto run = [&](auto zero) {
+        using T = decltype(zero);
+        mg->template tune<T>();
+        DBuf<T> u(n), y(n), rhs(n);
+        launch_fill<T>(n, u.data(), T(1e-3), s);
+        launch_fill<T>(n, rhs.data(), T(1), s);
+        auto once = [&]() {
+            if (which == 0) mg->apply<T>(l, u.data(), y.data(), nullptr, AM_SET, false);
+            else if (which == 1) mg->apply<T>(l, u.data(), y.data(), rhs.data(), AM_RESIDUAL, true);
+            else if (mg->template paired<T>(l)) mg->sweep_to<T>(l, u.data(), y.data(), rhs.data(), true);
+            else mg->smooth<T>(l, u.data(), rhs.data(), true);
+        };
+        once();
+        VF_HIP(hipEventRecord(a, s));
+        for (int64_t i = 0; i < reps; ++i) once();
+        VF_HIP(hipEventRecord(b, s));
+        VF_HIP(hipEventSynchronize(b));
+    };
+    if (precision == VF_PREC_FP64) run(0.0);
+    else run(0.0f);
+    float ms = 0;
     VF_HIP(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);

@@ -84,12 +84,54 @@ struct DBuf {
 // ---------------------------------------------------------------------------------------------
 enum ApplyMode { AM_SET = 0, AM_ADD = 1, AM_SUB = 2, AM_RESIDUAL = 3 };
 
+// Assembled block rows (levels 2..L-1) are stored colour-major: the nodes of colour
+// c = 4(ix%2) + 2(iy%2) + (iz%2) form a class of cnt_c nodes numbered j = ((ix/2) my + iy/2) mz + iz/2
+// (m_ = nodes of that parity along each axis), classes follow each other in colour order, and inside a
+// class entry k of neighbour slot s sits at (s*9 + k) * cnt_c + j. A smoothing pass over one colour then
+// streams contiguous memory.
+struct BlockSlot {
+    size_t base;  // first entry of the node's colour class
+    int cnt;      // nodes in the class
+    int j;        // node index inside the class
+};
+__host__ __device__ inline BlockSlot block_slot(const Grid &g, int ix, int iy, int iz) {
+    const int cx = ix & 1, cy = iy & 1, cz = iz & 1;
+    const int c = 4 * cx + 2 * cy + cz;
+    size_t before = 0;
+    for (int q = 0; q < c; ++q) {
+        const long mx = (g.nn[0] - ((q >> 2) & 1) + 1) / 2, my = (g.nn[1] - ((q >> 1) & 1) + 1) / 2,
+                   mz = (g.nn[2] - (q & 1) + 1) / 2;
+        before += (size_t)(mx * my * mz);
+    }
+    const int my = (g.nn[1] - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2, mx = (g.nn[0] - cx + 1) / 2;
+    BlockSlot r;
+    r.base = before * 243;
+    r.cnt = mx * my * mz;
+    r.j = ((ix >> 1) * my + (iy >> 1)) * mz + (iz >> 1);
+    return r;
+}
+
 // fine level (TPSStencils.hh:431-728).  AM_SET: y = K u (detached rows zeroed);
 // AM_RESIDUAL: y = b - K u on active rows; dmask != nullptr zeroes constrained output components.
 template <typename T>
 void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y, const T *b, const uint8_t *dmask,
                        int mode, hipStream_t s);
 // level 1 with on-the-fly Galerkin (MultigridSolver.hh:475-499)
+// LDS-tiled fine K u (AM_SET or AM_RESIDUAL only); same values as launch_fine_apply.
+template <typename T>
+void launch_fine_apply_tiled(const Grid &g, const T *K0, const T *E, const T *u, T *y, const T *b,
+                             const uint8_t *dmask, int mode, hipStream_t s);
+// Element-centric fine K u in the Walsh basis (AM_SET or AM_RESIDUAL); C = the 8 x 16 table of the
+// nonzero Walsh-basis element stiffness entries divided by 64 (see vf_kernels_walsh.hip).
+template <typename T>
+void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
+                             const uint8_t *dmask, int mode, hipStream_t s);
+// One x-parity of a fine block Gauss-Seidel sweep: planes of that parity read from uin, neighbour
+// planes from unb, result planes written to uout (uout != uin). A forward sweep is parity 0 with
+// unb = uin then parity 1 with unb = uout; a backward sweep runs parity 1 first.
+template <typename T>
+void launch_fine_gs_phase(const Grid &g, const T *K0, const T *E, const T *uin, const T *unb, T *uout, const T *b,
+                          const uint8_t *dmask, int parity, int forward, hipStream_t s);
 template <typename T>
 void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
                      const uint8_t *dmask, int mode, hipStream_t s);
@@ -158,6 +200,9 @@ void launch_galerkin_l2_elems(const Grid &g2, const Grid &gf, const double *M2, 
 void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, const double *Kf, double *Kc,
                              hipStream_t s);
 void launch_assemble_blocks(const Grid &g, const double *Ke, double *blocks, hipStream_t s);
+// level-1 block rows assembled from the fine moduli and the eight coarsened element matrices
+void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *blocks,
+                               hipStream_t s);
 void launch_assemble_dense(const Grid &g, const double *Ke, const int *red, int ld, double *A, hipStream_t s);
 // in-place blocked Gauss-Jordan inverse of an SPD matrix (n x n row-major, leading dim ld);
 // returns -A^-1 in A (the sweep-operator sign), scratch >= 3*ld*64 + 64*64 doubles

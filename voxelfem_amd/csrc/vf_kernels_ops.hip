@@ -272,23 +272,39 @@ __global__ void __launch_bounds__(TPB) k_l1_gs(Grid gc, Grid gf, const T *__rest
 }
 
 // ------------------------------------------------------------------ assembled block rows
-// blocks: plane-major SoA, blocks[(slot*9 + 3a + b) * nnodes + n] = K(n, n+slot)[a][b]
+// Colour-major block storage (see BlockSlot). The neighbours are visited three at a time with the
+// accumulators pinned after each group, which bounds how many of the 243 block entries and 81 u values
+// the compiler keeps in flight (and so the register count) while still overlapping memory latency.
 template <typename T, bool WANT_M>
-__device__ __forceinline__ void block_row(const Grid &g, const T *__restrict__ blocks, int n, const T un[27][3],
-                                          T f[3], T M[9]) {
-    const int N = g.nnodes;
+__device__ __forceinline__ void block_row(const Grid &g, const T *__restrict__ blocks, const BlockSlot &bs, int ix,
+                                          int iy, int iz, const T *__restrict__ u, T f[3], T M[9]) {
+    const int N = g.nnodes, sx = g.nn[1] * g.nn[2], sy = g.nn[2];
+    const int n = (ix * g.nn[1] + iy) * g.nn[2] + iz;
+    const T *__restrict__ bp = blocks + bs.base + bs.j;
+    const size_t st = (size_t)bs.cnt;
 #pragma unroll
     for (int s = 0; s < 27; ++s) {
+        const int dx = s / 9 - 1, dy = (s / 3) % 3 - 1, dz = s % 3 - 1;
+        const bool ok = (unsigned)(ix + dx) < (unsigned)g.nn[0] && (unsigned)(iy + dy) < (unsigned)g.nn[1] &&
+                        (unsigned)(iz + dz) < (unsigned)g.nn[2];
+        const int m = n + dx * sx + dy * sy + dz;
+        T un[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) un[c] = ok ? u[(size_t)c * N + m] : T(0);
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             T acc = 0;
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const T B = blocks[(size_t)(s * 9 + 3 * a + c) * N + n];
-                acc += B * un[s][c];
+                const T B = bp[(size_t)(s * 9 + 3 * a + c) * st];
+                acc += B * un[c];
                 if (WANT_M && s == 13) M[3 * a + c] = B;
             }
             f[a] += acc;
+        }
+        if (s % 3 == 2) {
+            asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]));
+            __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
@@ -302,11 +318,10 @@ __global__ void __launch_bounds__(TPB) k_block_apply(Grid g, const T *__restrict
     if (t >= g.nn[0] * slab) return;
     const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
     const int n = (ix * g.nn[1] + iy) * nz + iz;
-    T un[27][3];
-    gather27(g, u, ix, iy, iz, un);
+    const BlockSlot bs = block_slot(g, ix, iy, iz);
     T f[3] = {0, 0, 0};
     T M[9];
-    block_row<T, false>(g, blocks, n, un, f, M);
+    block_row<T, false>(g, blocks, bs, ix, iy, iz, u, f, M);
     store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
@@ -322,11 +337,10 @@ __global__ void __launch_bounds__(TPB) k_block_gs(Grid g, const T *__restrict__ 
     const int N = g.nnodes, n = (ix * g.nn[1] + iy) * g.nn[2] + iz;
     const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
     if (dm == 7) return;
-    T un[27][3];
-    gather27(g, u, ix, iy, iz, un);
+    const BlockSlot bs = block_slot(g, ix, iy, iz);
     T f[3] = {0, 0, 0};
     T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    block_row<T, true>(g, blocks, n, un, f, M);
+    block_row<T, true>(g, blocks, bs, ix, iy, iz, u, f, M);
     const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
     gs_update(u, N, n, bms, M, dm, forward);
 }

@@ -138,7 +138,49 @@ __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *_
                         for (int c = 0; c < 3; ++c) B[3 * a + c] += K[(3 * ln + a) * 24 + 3 * lm + c];
                 }
     }
-    for (int k = 0; k < 9; ++k) blocks[(size_t)(slot * 9 + k) * g.nnodes + n] = B[k];
+    const BlockSlot bs = block_slot(g, ix, iy, iz);
+    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = B[k];
+}
+
+// ---- level-1 block rows straight from the fine moduli: the block of coarse node n and neighbour slot
+// is sum over the coarse elements holding both nodes (unmasked: ey < ymask) of sum_fi E_child(fi) *
+// Chat_fi[(3 ln + a), (3 lm + c)], i.e. the assembled form of the level-1 operator that
+// MultigridSolver.hh:483-491 applies on the fly. Colour-major output layout (BlockSlot).
+__global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, const double *__restrict__ Chat,
+                                                            const double *__restrict__ Ef,
+                                                            double *__restrict__ blocks) {
+    const long t = (long)blockIdx.x * TPB + threadIdx.x;
+    if (t >= (long)gc.nnodes * 27) return;
+    const int n = (int)(t % gc.nnodes), slot = (int)(t / gc.nnodes);
+    const int nz = gc.nn[2], ix = n / (gc.nn[1] * nz), iy = (n / nz) % gc.nn[1], iz = n % nz;
+    const int off[3] = {slot / 9 - 1, (slot / 3) % 3 - 1, slot % 3 - 1};
+    const int id[3] = {ix, iy, iz};
+    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    bool inside = true;
+    for (int d = 0; d < 3; ++d) inside &= (unsigned)(id[d] + off[d]) < (unsigned)gc.nn[d];
+    if (inside) {
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = max(max(id[d] - 1, id[d] + off[d] - 1), 0);
+            hi[d] = min(min(id[d], id[d] + off[d]), gc.ne[d] - 1);
+        }
+        hi[1] = min(hi[1], gc.ymask - 1);
+        for (int ex = lo[0]; ex <= hi[0]; ++ex)
+            for (int ey = lo[1]; ey <= hi[1]; ++ey)
+                for (int ez = lo[2]; ez <= hi[2]; ++ez) {
+                    const int ln = 4 * (ix - ex) + 2 * (iy - ey) + (iz - ez);
+                    const int lm = 4 * (ix + off[0] - ex) + 2 * (iy + off[1] - ey) + (iz + off[2] - ez);
+                    for (int fi = 0; fi < 8; ++fi) {
+                        const int fx = 2 * ex + ((fi >> 2) & 1), fy = 2 * ey + ((fi >> 1) & 1), fz = 2 * ez + (fi & 1);
+                        const double Ee = Ef[(fx * gf.ne[1] + fy) * gf.ne[2] + fz];
+                        const double *C = Chat + fi * 576 + (3 * ln) * 24 + 3 * lm;
+                        for (int a = 0; a < 3; ++a)
+                            for (int c = 0; c < 3; ++c) B[3 * a + c] += Ee * C[a * 24 + c];
+                    }
+                }
+    }
+    const BlockSlot bs = block_slot(gc, ix, iy, iz);
+    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = B[k];
 }
 
 // ---- reduced dense coarsest operator: one work-item per (row DoF, neighbour slot)
@@ -370,6 +412,13 @@ void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, 
 
 void launch_assemble_blocks(const Grid &g, const double *Ke, double *blocks, hipStream_t s) {
     hipLaunchKernelGGL(k_assemble_blocks, dim3(nblocks((long)g.nnodes * 27)), dim3(TPB), 0, s, g, Ke, blocks);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *blocks,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(k_assemble_l1_blocks, dim3(nblocks((long)gc.nnodes * 27)), dim3(TPB), 0, s, gc, gf, Chat, Ef,
+                       blocks);
     VF_HIP(hipGetLastError());
 }
 

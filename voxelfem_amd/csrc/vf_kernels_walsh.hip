@@ -1,0 +1,284 @@
+// vf_kernels_walsh.hip -- element-centric fine K u in the Walsh (Hadamard) basis of the Q1 element.
+//
+// With W[k][m] = prod_a s_a(m)^{k_a} (s_a(m) = -1 or +1 for the low or high side of axis a; k, m in
+// {0,1}^3, index 4x+2y+z), the element stiffness of an axis-aligned box with an orthotropic (in
+// particular isotropic) elasticity tensor is K0 = (1/64) W^T Khat W where Khat = W K0 W^T has only 48
+// nonzero entries: Khat[(k,d),(k,d)] and Khat[(k,d),(k ^ e_d ^ e_c, c)] for c != d with k_d != k_c
+// (the monomials of the element's displacement basis are orthogonal under 2-point Gauss integration).
+// One element product y_e = E_e K0 u_e therefore costs two 3-stage butterflies per component (72 + 72
+// additions), 48 products with the Khat entries and 24 scalings by E_e, instead of the 576
+// multiply-adds of the dense 24 x 24 product (TPSStencils.hh:453-471). The host verifies the sparsity
+// on the actual K0 and selects this kernel only where it holds.
+//
+// Work layout: a workgroup owns a (y,z) tile of 16 x 16 element columns and marches along x. The node
+// plane ahead is staged in LDS; each work-item forms its element's 24 outputs, keeps the four of the
+// face ahead in registers (they belong to the next node plane), and publishes the four of the current
+// plane to LDS, where the owner of each node adds the four (y,z)-neighbouring elements' values in a
+// fixed order. The tile's outer element row/column only serves its neighbours: a tile owns 15 x 15
+// node columns. Every node value is summed in the same order whatever the tiling, so results are
+// deterministic.
+#include "vf_internal.hpp"
+
+#include <algorithm>
+
+namespace vf {
+namespace {
+
+template <typename T>
+using CT = __attribute__((address_space(4))) T;
+
+constexpr int TE = 16;       // element columns per tile side
+constexpr int TN = TE - 1;   // node columns owned per tile side
+constexpr int RN = TE + 1;   // node columns staged per tile side
+constexpr int NTW = TE * TE; // work-items
+
+__device__ __forceinline__ constexpr int bitof(int a) { return 1 << (2 - a); }
+
+// y = W^T diag/offdiag(Khat/64) W u for one element, u and y as [8 nodes][3 components] (y may alias u)
+template <typename T>
+__device__ __forceinline__ void walsh_apply(const T *__restrict__ C, T Ee, T v[8][3]) {
+    // forward butterflies: v[m][d] -> z[k][d]
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int bb = bitof(a);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            if (m & bb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T lo = v[m][d], hi = v[m | bb][d];
+                v[m][d] = lo + hi;
+                v[m | bb][d] = hi - lo;
+            }
+        }
+    }
+    // zh[k][d] = E (Khat[(k,d),(k,d)] z[k][d] + sum_{c != d, k_d != k_c} Khat[(k,d),(k^e_d^e_c,c)] z[..][c]) / 64
+    T zh[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        // coefficients of row group k (nine values, loaded through an opaque pointer so they are not
+        // hoisted out of the element loop and spilled)
+        const T *Cg = C + 16 * k;
+        asm volatile("" : "+s"(Cg));
+        const CT<T> *Ck = (const CT<T> *)Cg;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            T acc = Ck[3 * d + d] * v[k][d];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (c == d) continue;
+                const int kd = (k & bitof(d)) ? 1 : 0, kc = (k & bitof(c)) ? 1 : 0;
+                if (kd == kc) continue;
+                acc += Ck[3 * d + c] * v[k ^ bitof(d) ^ bitof(c)][c];
+            }
+            zh[k][d] = Ee * acc;
+        }
+    }
+    // backward butterflies: y[m][d] = sum_k W[k][m] zh[k][d]
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int bb = bitof(a);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            if (k & bb) continue;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const T z0 = zh[k][d], z1 = zh[k | bb][d];
+                zh[k][d] = z0 - z1;
+                zh[k | bb][d] = z0 + z1;
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) v[m][d] = zh[m][d];
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ C, const T *__restrict__ E,
+                                                  const T *__restrict__ u, T *__restrict__ y, const T *__restrict__ b,
+                                                  const uint8_t *__restrict__ dmask, int ntZ, int xchunk) {
+    __shared__ T su[2][3 * RN * RN];  // node planes, double-buffered
+    __shared__ T sp[4][3 * NTW];      // per element column: its values at the four corners of the plane
+    const int N = g.nnodes, nx = g.nn[0], ny = g.nn[1], nz = g.nn[2];
+    const int tz = blockIdx.x % ntZ, ty = blockIdx.x / ntZ;
+    const int sy = TN * ty - 1, sz = TN * tz - 1;  // first element (and staged node) of the tile
+    const int x0 = blockIdx.y * xchunk, x1 = min(x0 + xchunk, nx);
+    const int t = threadIdx.x, ly = t / TE, lz = t - ly * TE;
+    const int ey = sy + ly, ez = sz + lz;
+    const bool elemOk = (unsigned)ey < (unsigned)g.ne[1] && (unsigned)ez < (unsigned)g.ne[2];
+    const int Y = ey + 1, Z = ez + 1;  // node owned by this work-item
+    const bool owner = ly < TN && lz < TN && Y < g.ydet && Z < nz;
+
+    // Node planes are fetched one step ahead into registers (PF values per work-item) and written to
+    // LDS at the start of the next step, so the global-load latency overlaps a whole step of compute.
+    constexpr int PLANE = 3 * RN * RN;
+    constexpr int PF = (PLANE + NTW - 1) / NTW;
+    T pf[PF];
+    auto fetch_plane = [&](int ix) {
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int w = t + q * NTW;
+            T v = T(0);
+            if (w < PLANE) {
+                const int c = w / (RN * RN), r = w - c * (RN * RN), ry = r / RN, rz = r - ry * RN;
+                const int gy = sy + ry, gz = sz + rz;
+                if ((unsigned)ix < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz)
+                    v = u[(size_t)c * N + ((size_t)ix * ny + gy) * nz + gz];
+            }
+            pf[q] = v;
+        }
+    };
+    auto store_plane = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < PF; ++q) {
+            const int w = t + q * NTW;
+            if (w < PLANE) su[buf][w] = pf[q];
+        }
+    };
+    auto fetch_E = [&](int ex) {
+        T v = T(0);
+        if (elemOk && (unsigned)ex < (unsigned)g.ne[0]) v = E[((size_t)ex * g.ne[1] + ey) * g.ne[2] + ez];
+        return v;
+    };
+    auto read_face = [&](int buf, T f[4][3]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int j = q >> 1, k = q & 1;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[q][c] = su[buf][c * RN * RN + (ly + j) * RN + lz + k];
+        }
+    };
+
+    T uf[4][3];       // u at the element's face on the current node plane
+    T carry[4][3];    // contributions of the previous element to the current node plane
+#pragma unroll
+    for (int q = 0; q < 4; ++q) carry[q][0] = carry[q][1] = carry[q][2] = T(0);
+    fetch_plane(x0 - 1);
+    store_plane((x0 + 1) & 1);
+    fetch_plane(x0);
+    T Enext = fetch_E(x0 - 1);
+    __syncthreads();
+    read_face((x0 + 1) & 1, uf);
+    for (int ex = x0 - 1; ex < x1; ++ex) {
+        // element plane ex: node planes ex (face i = 0) and ex + 1 (face i = 1)
+        const int buf = (ex + 1) & 1;
+        store_plane(buf);     // plane ex + 1, fetched during the previous step
+        fetch_plane(ex + 2);  // consumed by the next step
+        const T Ee = Enext;
+        Enext = fetch_E(ex + 1);
+        // right-hand side and mask of the node finished in this step, requested early
+        const int ix = ex;
+        const size_t n = ((size_t)ix * ny + Y) * nz + Z;
+        uint8_t dm = 0;
+        T bv[3] = {0, 0, 0};
+        if (owner && ix >= x0) {
+            if (dmask) dm = dmask[n];
+            if (MODE == AM_RESIDUAL) {
+                bv[0] = b[n];
+                bv[1] = b[(size_t)N + n];
+                bv[2] = b[(size_t)2 * N + n];
+            }
+        }
+        __syncthreads();
+        T ub[4][3];
+        read_face(buf, ub);
+        T v[8][3];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                v[q][c] = uf[q][c];      // local node 0jk
+                v[4 + q][c] = ub[q][c];  // local node 1jk
+                uf[q][c] = ub[q][c];
+            }
+        walsh_apply<T>(C, Ee, v);
+        if (ex >= x0) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) sp[q][c * NTW + t] = carry[q][c] + v[q][c];
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) carry[q][c] = v[4 + q][c];
+        __syncthreads();
+        if (ex >= x0 && owner) {
+            // corners (j,k) of the elements at (ly+1-j, lz+1-k) that hold node (Y, Z)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                T f = sp[3][c * NTW + ly * TE + lz];
+                f += sp[2][c * NTW + ly * TE + lz + 1];
+                f += sp[1][c * NTW + (ly + 1) * TE + lz];
+                f += sp[0][c * NTW + (ly + 1) * TE + lz + 1];
+                T out = MODE == AM_RESIDUAL ? bv[c] - f : f;
+                if (dm & (1 << c)) out = T(0);
+                y[(size_t)c * N + n] = out;
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_zero_rows_w(Grid g, T *__restrict__ y) {
+    const int nz = g.nn[2], rows = g.nn[1] - g.ydet, slab = rows * nz;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= g.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = g.ydet + r / nz, iz = r % nz;
+    const int n = (ix * g.nn[1] + iy) * nz + iz;
+    y[n] = T(0);
+    y[g.nnodes + n] = T(0);
+    y[(size_t)2 * g.nnodes + n] = T(0);
+}
+
+int cu_count_w() {
+    static int n = 0;
+    if (!n) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            n = v;
+        else
+            n = 256;
+    }
+    return n;
+}
+
+inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+}  // namespace
+
+template <typename T>
+void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
+                             const uint8_t *dmask, int mode, hipStream_t s) {
+    if (mode != AM_SET && mode != AM_RESIDUAL) throw std::runtime_error("Walsh K u supports the SET and RESIDUAL modes");
+    if (g.ydet > 0) {
+        // tiles cover node columns [TN*t, TN*t + TN); rows beyond ydet are skipped by the owners
+        const int ntY = cdiv(g.ydet, TN), ntZ = cdiv(g.nn[2], TN);
+        const int tiles = ntY * ntZ;
+        const int want = 4 * cu_count_w();
+        int chunks = std::max(1, std::min(g.nn[0], cdiv(want, tiles)));
+        const int xchunk = cdiv(g.nn[0], chunks);
+        chunks = cdiv(g.nn[0], xchunk);
+        if (chunks > 65535) throw std::runtime_error("grid too large for the Walsh K u");
+        const dim3 grid(tiles, chunks);
+        if (mode == AM_RESIDUAL)
+            hipLaunchKernelGGL((k_ku_walsh<T, AM_RESIDUAL>), grid, dim3(NTW), 0, s, g, C, E, u, y, b, dmask, ntZ, xchunk);
+        else
+            hipLaunchKernelGGL((k_ku_walsh<T, AM_SET>), grid, dim3(NTW), 0, s, g, C, E, u, y, b, dmask, ntZ, xchunk);
+    }
+    if (mode == AM_SET && g.ydet < g.nn[1]) {
+        const long cz = (long)g.nn[0] * (g.nn[1] - g.ydet) * g.nn[2];
+        hipLaunchKernelGGL(k_zero_rows_w<T>, dim3((unsigned)((cz + 255) / 256)), dim3(256), 0, s, g, y);
+    }
+    VF_HIP(hipGetLastError());
+}
+
+template void launch_fine_apply_walsh<double>(const Grid &, const double *, const double *, const double *, double *,
+                                              const double *, const uint8_t *, int, hipStream_t);
+template void launch_fine_apply_walsh<float>(const Grid &, const float *, const float *, const float *, float *,
+                                             const float *, const uint8_t *, int, hipStream_t);
+
+}  // namespace vf
