@@ -62,6 +62,10 @@ int vf_malloc(void **ptr, size_t bytes);
 int vf_free(void *ptr);
 int vf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream);
 int vf_stream_sync(void *stream);
+/* create / destroy a stream of default priority that orders itself with the null stream (work on the
+ * null stream, e.g. a framework's default stream, and work on this stream do not overlap) */
+int vf_stream_create(void **stream);
+int vf_stream_destroy(void *stream);
 
 /* ---- simulator: TensorProductSimulator (TensorProductSimulator.hh:209-285, factory
  *      python_bindings/VoxelFEM.cc:423-430). stream: a hipStream_t or NULL for the null stream. */
@@ -166,6 +170,16 @@ int vf_mg_pcg(vf_mg *mg, double *x, const double *b, int64_t maxIter, double tol
 int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
                     int64_t smoothing, int fullMultigrid, int precision, double *objective, double *grad,
                     int gradWhere, double *perLayerCompliance, int64_t *perLayerIterations);
+/* The same evaluation with the layer solves batched: the layers are spread over `pipelines`
+ * concurrent device pipelines (each a copy of the simulator and hierarchy on its own stream, driven by
+ * its own host thread; 0 = 8), and with useGraphs the steady-state PCG iteration of each layer solve
+ * is captured once into a HIP graph and replayed. Results are summed in layer order, so objective and
+ * gradient equal the sequential evaluator's (LayerByLayer.hh:223-300). No reference counterpart for
+ * the two extra arguments. */
+int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                            int64_t smoothing, int fullMultigrid, int precision, int64_t pipelines, int useGraphs,
+                            double *objective, double *grad, int gradWhere, double *perLayerCompliance,
+                            int64_t *perLayerIterations);
 
 /* ---- filters and optimiser kernels (TopologyOptimizationFilter.hh, OptimalityCriterion.hh) --- */
 /* SmoothingFilter::apply == backprop (TopologyOptimizationFilter.hh:285-400). rmin is the kernel
@@ -187,14 +201,20 @@ int vf_mean(int64_t n, const double *x, int where, void *stream, double *out);
 /* Select the fine-level kernels process-wide: 0 (default) = per solver and precision, the fastest
  * family on the solver's grid (timed once, on first use); 1 = register-gather kernels with one launch
  * per colour; 2 = LDS-tiled kernels on the structured element matrix with the two-launch smoother;
- * 3 = element-centric Walsh-basis K u with the LDS-tiled smoother. All families compute the same
- * operators; the tests compare them. Variant 1 also keeps level 1 matrix-free. */
+ * 3 = element-centric Walsh-basis K u with the LDS-tiled smoother; 4 = Walsh-basis K u with the
+ * register-gather smoother on the structured element matrix. All families compute the same operators;
+ * the tests compare them. Variant 1 also keeps level 1 matrix-free. */
 int vf_set_kernel_variant(int variant);
 /* Fine-level kernel families the solver uses in `precision` (running the autotuner if it has not run):
  * choice[0] for K u (0 register-gather, 1 LDS-tiled, 2 Walsh), choice[1] for the smoother
- * (0 register-gather, 1 LDS-tiled); ms[] = the autotuner's timings (gather, tiled, Walsh K u; gather,
- * tiled sweep; 0 where it did not run). */
-int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[5]);
+ * (0 register-gather, 1 LDS-tiled two-launch, 2 register-gather on the structured element matrix);
+ * ms[] = the autotuner's timings (gather, tiled, Walsh K u; gather, tiled, structured-gather sweep; 0
+ * where it did not run). */
+int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]);
+/* Replay the steady-state PCG iteration of vf_mg_pcg from a HIP graph captured on its second iteration
+ * (enable = 1) instead of launching its kernels one by one (0, the default). Needs the solver's stream
+ * to be a created stream, not the null stream; otherwise the iterations run eagerly. Same results. */
+int vf_mg_set_use_graphs(vf_mg *mg, int enable);
 /* Average time of one level-`level` operation of the hierarchy over `reps` launches (HIP events on
  * the solver's stream, vectors allocated and filled beforehand): which = 0 K u, 1 residual b - K u,
  * 2 one forward smoothing sweep (all eight colours). */

@@ -65,16 +65,17 @@ def bench_mg_kernel(mg_handle, which, level, reps, precision=VF_PREC_FP64):
 
 def set_kernel_variant(variant):
     """0: autotuned per solver and precision (default); 1: register-gather; 2: LDS-tiled fine kernels;
-    3: Walsh-basis K u with the LDS-tiled smoother."""
+    3: Walsh-basis K u with the LDS-tiled smoother; 4: Walsh-basis K u with the structured gather smoother."""
     check(lib().vf_set_kernel_variant(int(variant)))
 
 
 def tuning(mg_handle, precision=VF_PREC_FP64):
-    """(K u family, smoother family, timings ms [gather, tiled, Walsh K u; gather, tiled sweep])."""
+    """(K u family, smoother family, timings ms [gather, tiled, Walsh K u; gather, tiled, structured-gather
+    sweep])."""
     ch = (ctypes.c_int * 2)()
-    ms = (ctypes.c_double * 5)()
+    ms = (ctypes.c_double * 6)()
     check(lib().vf_mg_tuning(ctypes.c_void_p(mg_handle), int(precision), ch, ms))
-    return ("gather", "tiled", "walsh")[ch[0]], ("gather", "tiled")[ch[1]], list(ms)
+    return ("gather", "tiled", "walsh")[ch[0]], ("gather", "tiled", "gather-structured")[ch[1]], list(ms)
 
 
 def bench_apply_K(sim_handle, reps, precision=VF_PREC_FP64):

@@ -1,0 +1,228 @@
+"""Reports for the five seeded benchmark workloads of the solve-and-sensitivity path.
+
+    python -m voxelfem_amd.configs c0 c1 c2 c3 c4 [--out FILE] [--c4-iters N]
+
+Each workload prints one JSON object (and appends it to --out). Inputs are the synthetic fields of
+``voxelfem_amd.workloads`` (seed k for workload k, C-order element draws); every run goes through the
+pyVoxelFEM API, on the device.
+
+  c0  64x32x32 cantilever, noise density: FP64 MG-PCG to 1e-8, compliance, sensitivity.
+  c1  256x128x128 cantilever, smoothed density: FP64 PCG with an FP64 vs an FP32 V-cycle.
+  c2  128x64x64 MBB half-beam, hat filter rmin=2.5 + projection beta=8: autograd gradient against
+      central differences (h=1e-6, 20 seeded N(0,1) directions), FP64 and FP32 V-cycle.
+  c3  160x80x80 layer-by-layer (40 layers of 2 along the build axis y, clamped base, self-weight):
+      batched evaluator vs the 40 sequential solves, FP64 and FP32 V-cycle.
+  c4  384x192x192 cantilever, OC loop (filter radius 3, vf=0.3, m=0.2, p=0.5) with the FP32 V-cycle in
+      the FP64 PCG, the FP64-only loop beside it, and the FP64 gradient at every mixed iterate.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+from . import workloads as wl
+from .mg_levels import optMGLevel
+
+
+def _pv():
+    from . import pyVoxelFEM
+    return pyVoxelFEM
+
+
+def _sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def _solve(tps, levels, precision, tol=1e-8, nu=1):
+    mg = tps.multigridSolver(levels)
+    f = tps.buildLoadVector()
+    its = []
+    mg.preconditionedConjugateGradient(np.zeros_like(f), f, 1000, tol, mgIterations=1, mgSmoothingIterations=nu,
+                                       fullMultigrid=False, precision=precision)  # warm-up: setup and tuning
+    t0 = time.perf_counter()
+    u = mg.preconditionedConjugateGradient(np.zeros_like(f), f, 1000, tol,
+                                           lambda it, x, r: its.append(it), mgIterations=1,
+                                           mgSmoothingIterations=nu, fullMultigrid=False, precision=precision)
+    dt = time.perf_counter() - t0
+    return u, f, dt, len(its)
+
+
+def c0():
+    pv = _pv()
+    ne = wl.GRIDS["cantilever_64x32x32"]
+    tps = wl.configure(pv, ne, wl.CANTILEVER_BC, wl.noise_density(ne, 0))
+    u, f, dt, its = _solve(tps, optMGLevel(ne, 3), "fp64")
+    comp = 0.5 * float(np.sum(f * u))
+    g = tps.complianceGradient(u)
+    return {"config": "c0", "grid": ne, "dof": int(3 * np.prod([n + 1 for n in ne])), "pcg_iterations": its,
+            "solve_s_incl_callback": dt, "compliance": comp, "sensitivity_sum": float(np.sum(g)),
+            "sensitivity_min": float(np.min(g))}
+
+
+def c1():
+    pv = _pv()
+    ne = wl.GRIDS["cantilever_256x128x128"]
+    tps = wl.configure(pv, ne, wl.CANTILEVER_BC, wl.smoothed_density(ne, 1))
+    levels = optMGLevel(ne, 3)
+    out = {"config": "c1", "grid": ne, "dof": int(3 * np.prod([n + 1 for n in ne])), "mg_levels": levels}
+    ref = None
+    for prec in ("fp64", "fp32"):
+        u, f, dt, its = _solve(tps, levels, prec)
+        comp = 0.5 * float(np.sum(f * u))
+        out[prec] = {"pcg_iterations": its, "solve_s_incl_callback": dt, "compliance": comp}
+        if ref is None:
+            ref = (comp, u)
+        else:
+            out[prec]["compliance_rel_err_vs_fp64"] = abs(comp - ref[0]) / abs(ref[0])
+            out[prec]["u_rel_err_vs_fp64"] = float(np.abs(u - ref[1]).max() / np.abs(ref[1]).max())
+    out["speedup_fp32_vcycle"] = out["fp64"]["solve_s_incl_callback"] / out["fp32"]["solve_s_incl_callback"]
+    return out
+
+
+def c2(directions=20):
+    import torch
+
+    from . import optMGLevel as lv
+    from .autograd import CompliancePipeline
+
+    pv = _pv()
+    ne = wl.GRIDS["mbb_128x64x64"]
+    n = int(np.prod(ne))
+    x0 = np.random.default_rng(2).uniform(0.2, 0.4, n)
+    out = {"config": "c2", "grid": ne, "h": 1e-6, "directions": directions}
+    for prec in ("fp64", "fp32"):
+        tps = wl.configure(pv, ne, wl.MBB_BC, None, symmetry_x=True)
+        pipe = CompliancePipeline(tps, lv(ne, 3), rmin=2.5, filter_type=1, beta=8.0, tol=1e-12, max_iter=3000,
+                                  precision=prec)
+        x = torch.tensor(x0, device="cuda:0", requires_grad=True)
+        c = pipe(x)
+        c.backward()
+        g = x.grad.detach()
+        gn = float(torch.linalg.vector_norm(g).item())
+        rng = np.random.default_rng(20)
+        e_norm, e_dir = [], []
+        for _ in range(directions):
+            v = torch.tensor(rng.normal(size=n), device="cuda:0")
+            fd = (pipe(x.detach() + 1e-6 * v).item() - pipe(x.detach() - 1e-6 * v).item()) / 2e-6
+            gv = float(torch.dot(g, v).item())
+            e_norm.append(abs(fd - gv) / gn)
+            e_dir.append(abs(fd - gv) / abs(gv))
+        out[prec] = {"compliance": float(c.item()), "max_err_over_gnorm": max(e_norm),
+                     "median_err_over_abs_gv": float(np.median(e_dir)), "max_err_over_abs_gv": max(e_dir)}
+    return out
+
+
+def c3():
+    pv = _pv()
+    ne = wl.GRIDS["lbl_160x80x80"]
+    rho = wl.smoothed_density(ne, 3)
+    full = wl.configure(pv, ne, None, rho)
+    sim = full.getIntermediateFabricationShape(1.0, False, pv.InterpolationLaw.SIMP)
+    levels = optMGLevel(ne, 3)
+    mg = sim.multigridSolver(levels)
+    ev = pv.LayerByLayerEvaluator(sim)
+    ev.selectInitMethod("zero")
+    out = {"config": "c3", "grid": ne, "layers": ne[1] // 2, "layer_increment": 2, "mg_levels": levels}
+    ref = None
+    for prec in ("fp64", "fp32"):
+        ev.precision = prec
+        res = {}
+        for name, batched, pipes, graphs in (("sequential", False, 1, False), ("batched", True, 0, True)):
+            ev.batched, ev.pipelines, ev.useGraphs = batched, pipes, graphs
+            ev.run(mg, True, 2, 1000, 1e-8)  # warm-up (hierarchy memory, kernel tuning)
+            ev.run(mg, True, 2, 1000, 1e-8)
+            res[name] = {"s": ev.lastRunSeconds, "objective": ev.objective(),
+                         "pcg_iterations_total": int(np.sum(ev.perLayerIterations()))}
+            if ref is None:
+                ref = (ev.objective(), ev.gradient())
+            else:
+                res[name]["objective_rel_err_vs_fp64_sequential"] = abs(ev.objective() - ref[0]) / abs(ref[0])
+                res[name]["gradient_rel_err_vs_fp64_sequential"] = float(
+                    np.abs(ev.gradient() - ref[1]).max() / np.abs(ref[1]).max())
+        res["speedup_batched"] = res["sequential"]["s"] / res["batched"]["s"]
+        out[prec] = res
+    return out
+
+
+def c4(iters=50):
+    pv = _pv()
+    ne = wl.GRIDS["cantilever_384x192x192"]
+    n = int(np.prod(ne))
+    vf = 0.3
+    levels = optMGLevel(ne, 3)
+
+    def problem(prec):
+        tps = wl.configure(pv, ne, wl.CANTILEVER_BC, np.full(n, vf))
+        mg = tps.multigridSolver(levels)
+        obj = pv.MultigridComplianceObjective(mg)
+        obj.tol = 1e-8
+        obj.cgIter = 1000
+        obj.precision = prec
+        top = pv.TopologyOptimizationProblem(tps, obj, [pv.TotalVolumeConstraint(vf)],
+                                             [pv.SmoothingFilter(3, pv.SmoothingFilter.Type.Linear)])
+        top.setVars(np.full(n, vf))
+        return top, obj
+
+    out = {"config": "c4", "grid": ne, "dof": int(3 * np.prod([k + 1 for k in ne])), "oc_iterations": iters,
+           "mg_levels": levels, "objective_settings": "1 FMG V-cycle, 2+2 sweeps, tol 1e-8, warm start"}
+    runs = {}
+    for prec in ("fp64", "fp32"):
+        top, obj = problem(prec)
+        shadow = None
+        if prec == "fp32":
+            shadow, _ = problem("fp64")  # FP64 gradient at each mixed iterate
+        oc = pv.OCOptimizer(top)
+        hist, gerr, its = [], [], []
+        _sync()
+        t0 = time.perf_counter()
+        t_shadow = 0.0
+        for k in range(iters):
+            hist.append(top.evaluateObjective())
+            its.append(int(obj.lastIterations))
+            if shadow is not None:
+                ts = time.perf_counter()
+                g = top.evaluateObjectiveGradient()
+                shadow.setVars(top.getVars())
+                gs = shadow.evaluateObjectiveGradient()
+                gerr.append(float(np.abs(g - gs).max() / np.abs(gs).max()))
+                t_shadow += time.perf_counter() - ts
+            oc.step()
+        final = top.evaluateObjective()
+        _sync()
+        dt = time.perf_counter() - t0 - t_shadow
+        runs[prec] = {"s": dt, "s_per_iteration": dt / iters, "final_compliance": final,
+                      "compliance_history": hist, "pcg_iterations": its,
+                      "volume_constraint": float(top.evaluateConstraints()[0])}
+        if gerr:
+            runs[prec]["gradient_rel_err_vs_fp64"] = {"max": max(gerr), "median": float(np.median(gerr)),
+                                                      "per_iteration": gerr}
+    out.update(runs)
+    out["final_compliance_rel_diff"] = abs(runs["fp32"]["final_compliance"] - runs["fp64"]["final_compliance"]) \
+        / abs(runs["fp64"]["final_compliance"])
+    out["speedup_mixed_loop"] = runs["fp64"]["s"] / runs["fp32"]["s"]
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("configs", nargs="+", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--c4-iters", type=int, default=50)
+    args = ap.parse_args(argv)
+    fns = {"c0": c0, "c1": c1, "c2": c2, "c3": c3, "c4": lambda: c4(args.c4_iters)}
+    for c in args.configs:
+        print(f"running {c}", file=sys.stderr, flush=True)
+        res = fns[c]()
+        line = json.dumps(res, default=lambda o: list(o) if isinstance(o, tuple) else str(o))
+        print(line, flush=True)
+        if args.out:
+            with open(args.out, "a") as fh:
+                fh.write(line + "\n")
+
+
+if __name__ == "__main__":
+    main()

@@ -14,6 +14,7 @@
 #include <array>
 #include <cmath>
 #include <cstring>
+#include <chrono>
 #include <fstream>
 #include <iostream>
 #include <limits>
@@ -122,9 +123,12 @@ struct TPS : std::enable_shared_from_this<TPS> {
     std::array<double, 3> grav{0, 0, 0};
     std::shared_ptr<struct MG> directSolver;  // 0-level hierarchy backing solve(f)
 
+    void *stream = nullptr;  // the simulator's own stream (ordered with the null stream)
+
     TPS(const std::array<double, 3> &minC, const std::array<double, 3> &maxC, const std::array<int64_t, 3> &n)
         : ne(n), lo(minC), hi(maxC) {
-        check(vf_sim_create(&sim, ne.data(), lo.data(), hi.data(), nullptr));
+        check(vf_stream_create(&stream));
+        check(vf_sim_create(&sim, ne.data(), lo.data(), hi.data(), stream));
         nn = (size_t)(ne[0] + 1) * (ne[1] + 1) * (ne[2] + 1);
         nel = (size_t)ne[0] * ne[1] * ne[2];
         dmask.assign(nn, 0);
@@ -132,7 +136,10 @@ struct TPS : std::enable_shared_from_this<TPS> {
         forces.assign(3 * nn, 0.0);
         push_law();
     }
-    ~TPS() { if (sim) vf_sim_destroy(sim); }
+    ~TPS() {
+        if (sim) vf_sim_destroy(sim);
+        if (stream) vf_stream_destroy(stream);
+    }
 
     std::array<int64_t, 3> nnodes3() const { return {ne[0] + 1, ne[1] + 1, ne[2] + 1}; }
     std::array<double, 3> h() const { return {(hi[0] - lo[0]) / ne[0], (hi[1] - lo[1]) / ne[1], (hi[2] - lo[2]) / ne[2]}; }
@@ -207,6 +214,7 @@ struct MG {
     TPSPtr tps;
     vf_mg *mg = nullptr;
     int64_t levels = 0;
+    bool useGraphs = false;
     MG(TPSPtr t, int64_t L) : tps(std::move(t)), levels(L) { check(vf_mg_create(&mg, tps->sim, L)); }
     ~MG() { if (mg) vf_mg_destroy(mg); }
     size_t nodes(int64_t l) const {
@@ -618,6 +626,12 @@ struct LBL {
     std::vector<double> perLayer;
     std::vector<int64_t> perLayerIters;
     int precision = VF_PREC_FP64;
+    // batched evaluation (extension): concurrent pipelines (0 = default) and graph-replayed PCG iterations;
+    // batched = false runs the layer solves one after the other on the caller's solver
+    bool batched = true;
+    int64_t pipelines = 0;
+    bool useGraphs = true;
+    double seconds = 0;
     explicit LBL(TPSPtr t) : tps(std::move(t)) {}
     void selectInitMethod(const std::string &m) {
         if (m == "zero" || m == "constant" || m == "fd" || m.rfind("N=", 0) == 0) { method = m; return; }
@@ -634,9 +648,17 @@ struct LBL {
         perLayer.assign(nl + 1, 0.0);
         perLayerIters.assign(nl + 1, 0);
         double obj = 0;
-        check(vf_lbl_run_zero(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol, (int64_t)mgIterations,
-                              (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision, &obj, grad.data(), VF_HOST,
-                              perLayer.data(), perLayerIters.data()));
+        const auto t0 = std::chrono::steady_clock::now();
+        if (batched)
+            check(vf_lbl_run_zero_batched(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol,
+                                          (int64_t)mgIterations, (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision,
+                                          pipelines, useGraphs ? 1 : 0, &obj, grad.data(), VF_HOST, perLayer.data(),
+                                          perLayerIters.data()));
+        else
+            check(vf_lbl_run_zero(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol, (int64_t)mgIterations,
+                                  (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision, &obj, grad.data(), VF_HOST,
+                                  perLayer.data(), perLayerIters.data()));
+        seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         total = obj;
         if (verbose) {
             size_t k = 0;
@@ -1072,6 +1094,8 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
             }, py::arg("u"), py::arg("f"), py::arg("numSteps"), py::arg("numSmoothingSteps"),
             py::arg("stiffnessUpdated") = false, py::arg("zeroDirichlet") = false, py::arg("it_callback") = nullptr,
             py::arg("fullMultigrid") = false)
+        .def_property("useGraphs", [](const MG &mg) { return mg.useGraphs; },
+                      [](MG &mg, bool v) { check(vf_mg_set_use_graphs(mg.mg, v ? 1 : 0)); mg.useGraphs = v; })
         .def("setFabricationMaskHeightByLayer", [](MG &mg, size_t h) { check(vf_mg_set_mask_layer(mg.mg, (int64_t)h)); }, py::arg("h"))
         .def("preconditionedConjugateGradient",
              [](MG &mg, const Arr &u, const Arr &b, size_t maxIter, double tol,
@@ -1284,7 +1308,11 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def("perLayerCompliance", [](const LBL &l) { return l.perLayer; })
         .def("perLayerIterations", [](const LBL &l) { return l.perLayerIters; })
         .def_property("precision", [](const LBL &l) { return l.precision == VF_PREC_FP32 ? "fp32" : "fp64"; },
-                      [](LBL &l, const std::string &p) { l.precision = p == "fp32" ? VF_PREC_FP32 : VF_PREC_FP64; });
+                      [](LBL &l, const std::string &p) { l.precision = p == "fp32" ? VF_PREC_FP32 : VF_PREC_FP64; })
+        .def_readwrite("batched", &LBL::batched)
+        .def_readwrite("pipelines", &LBL::pipelines)
+        .def_readwrite("useGraphs", &LBL::useGraphs)
+        .def_readonly("lastRunSeconds", &LBL::seconds);
 
     // ------------------------------------------------------------------ factories (VoxelFEM.cc:221-298, 423-430)
     m.def("TensorProductSimulator", [](const std::vector<size_t> &degrees, const std::array<std::vector<double>, 2> &bbox,

@@ -17,6 +17,7 @@
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <thread>
 
 using namespace vf;
 
@@ -265,6 +266,40 @@ struct vf_sim {
         if (scal.empty()) scal.resize(16);
         return scal.data();
     }
+    // deep copy of the simulator's state (device arrays copied on this simulator's stream) bound to `st`
+    std::unique_ptr<vf_sim> clone(hipStream_t st) const {
+        auto c = std::make_unique<vf_sim>();
+        c->stream = st;
+        std::copy(ne, ne + 3, c->ne);
+        std::copy(lo, lo + 3, c->lo);
+        std::copy(hi, hi + 3, c->hi);
+        std::copy(h, h + 3, c->h);
+        c->g = g;
+        c->young = young; c->poisson = poisson; c->law = law;
+        c->E0 = E0; c->Emin = Emin; c->gamma = gamma; c->q = q;
+        std::copy(grav, grav + 3, c->grav);
+        std::copy(K0h, K0h + 576, c->K0h);
+        c->structured = structured;
+        c->walsh = walsh;
+        c->hmask = hmask;
+        c->hvals = hvals;
+        c->ndir = ndir;
+        c->hforce = hforce;
+        c->hasForce = hasForce;
+        c->firstMasked = firstMasked;
+        c->firstDetached = firstDetached;
+        c->maskHeight = maskHeight;
+        c->version = version;
+        auto dup = [&](auto &dst, const auto &src) {
+            dst.resize(src.size());
+            d2d(dst.data(), src.data(), src.size(), stream);
+        };
+        dup(c->K0, K0); dup(c->K032, K032); dup(c->Ftab, Ftab); dup(c->Ftab32, Ftab32); dup(c->Wtab, Wtab);
+        dup(c->Wtab32, Wtab32); dup(c->rho, rho); dup(c->E, E); dup(c->E32, E32); dup(c->dmask, dmask);
+        dup(c->dnodes, dnodes); dup(c->dnmask, dnmask); dup(c->dvals, dvals); dup(c->force, force);
+        sync(stream);
+        return c;
+    }
     double read_scalar(const double *dev) {
         double v;
         d2h(&v, dev, 1, stream);
@@ -317,13 +352,25 @@ void sim_apply_K(vf_sim *sim, const T *u, T *y, hipStream_t s) {
     } else {
         E = sim->E32.data(); K0 = sim->K032.data(); F = sim->Ftab32.data(); Wc = sim->Wtab32.data();
     }
-    if (sim->walsh && (g_kernel_variant == 0 || g_kernel_variant == 3))
+    if (sim->walsh && (g_kernel_variant == 0 || g_kernel_variant >= 3))
         launch_fine_apply_walsh<T>(sim->g, Wc, E, u, y, nullptr, nullptr, AM_SET, s);
     else if (sim->structured && g_kernel_variant != 1)
         launch_fine_apply_tiled<T>(sim->g, F, E, u, y, nullptr, nullptr, AM_SET, s);
     else
         launch_fine_apply<T>(sim->g, K0, E, u, y, nullptr, nullptr, AM_SET, s);
 }
+
+namespace {
+// owner of a captured graph and its executable instance
+struct GraphGuard {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphGuard() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+}  // namespace
 
 struct vf_mg {
     vf_sim *sim;
@@ -334,9 +381,12 @@ struct vf_mg {
     DBuf<double> dChat, dphi, dM2, dKe0;
     DBuf<float> dChat32;
     bool symmetricGS = true;
+    bool useGraphs = false;  // replay the steady-state PCG iteration from a captured HIP graph
     bool l1Blocks = false;  // level 1 held as assembled block rows (set by update_stiffness)
     // coarsest operator
-    DBuf<double> Ainv, invScratch, ctmp_in, ctmp_out;
+    DBuf<double> Ainv, invScratch, ctmp_in, ctmp_out, band;  // Ainv holds X = L^-1 (see factor_coarsest)
+    DBuf<int> cholFlag;
+    int bandKb = 1;
     DBuf<int> dRedToFull, dRed;
     int nred = 0, ldA = 0;
     bool stiffnessReady = false;
@@ -480,7 +530,8 @@ struct vf_mg {
         switch (g_kernel_variant) {
         case 1: return 0;
         case 2: return sim->structured ? 1 : 0;
-        case 3: return sim->walsh ? 2 : (sim->structured ? 1 : 0);
+        case 3:
+        case 4: return sim->walsh ? 2 : (sim->structured ? 1 : 0);
         default: {
             const int c = kuChoice[pidx<T>()];
             if (c >= 0) return c;
@@ -488,11 +539,18 @@ struct vf_mg {
         }
         }
     }
+    // smoother family: 0 register gather (dense K0), 1 LDS-tiled two-launch sweep, 2 register gather on
+    // the structured element matrix
     template <typename T>
-    bool tiled_gs() const {
-        if (!sim->structured || g_kernel_variant == 1) return false;
-        return g_kernel_variant >= 2 || gsChoice[pidx<T>()] != 0;
+    int gs_family() const {
+        if (!sim->structured || g_kernel_variant == 1) return 0;
+        if (g_kernel_variant == 4) return 2;
+        if (g_kernel_variant >= 2) return 1;
+        const int c = gsChoice[pidx<T>()];
+        return c >= 0 ? c : 2;
     }
+    template <typename T>
+    bool tiled_gs() const { return gs_family<T>() == 1; }
     // Autotuner: time each applicable kernel family on this grid (HIP events on the solver's stream,
     // best of two runs after a warm-up) and keep the fastest per operation and precision.
     template <typename T>
@@ -538,20 +596,25 @@ struct vf_mg {
         const float gsG = best([&] {
             for (int c = 0; c < 8; ++c) launch_fine_gs<T>(g, K0p<T>(), Ep<T>(), y.data(), rhs.data(), mask, c, 1, s);
         });
+        const float gsF = best([&] {
+            for (int c = 0; c < 8; ++c)
+                launch_fine_gs_gather_f<T>(g, Fp<T>(), Ep<T>(), y.data(), rhs.data(), mask, c, 1, s);
+        });
         (void)hipEventDestroy(a);
         (void)hipEventDestroy(b);
         int kbest = 0;
         for (int i = 1; i < 3; ++i)
             if (ku[i] < ku[kbest]) kbest = i;
         kuChoice[p] = kbest;
-        gsChoice[p] = gsT <= gsG ? 1 : 0;
+        gsChoice[p] = gsT <= gsG ? (gsT <= gsF ? 1 : 2) : (gsG <= gsF ? 0 : 2);
         tuneMs[p][0] = ku[0];
         tuneMs[p][1] = ku[1];
         tuneMs[p][2] = sim->walsh ? ku[2] : 0.0f;
         tuneMs[p][3] = gsG;
         tuneMs[p][4] = gsT;
+        tuneMs[p][5] = gsF;
     }
-    float tuneMs[2][5] = {{0, 0, 0, 0, 0}, {0, 0, 0, 0, 0}};
+    float tuneMs[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
 
     template <typename T>
     void apply(int l, const T *u, T *y, const T *b, int mode, bool useMask) {
@@ -596,7 +659,9 @@ struct vf_mg {
         }
         for (int t = 0; t < 8; ++t) {
             const int color = forward ? t : 7 - t;
-            if (l == 0) launch_fine_gs<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
+            if (l == 0 && gs_family<T>() == 2)
+                launch_fine_gs_gather_f<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
+            else if (l == 0) launch_fine_gs<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
             else if (l == 1 && !(l1Blocks && L >= 2))
                 launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, st());
             else launch_block_gs<T>(lv[l]->g, blocksp<T>(l), u, b, dm(l), color, forward, st());
@@ -605,8 +670,8 @@ struct vf_mg {
     template <typename T>
     void coarse_solve(const T *b, T *x) {
         ensure_factor();
-        launch_coarse_apply<T>(nred, Ainv.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
-                               ctmp_out.data(), st());
+        launch_coarse_chol_apply<T>(nred, Ainv.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
+                                    ctmp_out.data(), st());
     }
 
     // ---------------------------------------------------------------- cycles (MultigridSolver.hh:587-658)
@@ -808,21 +873,48 @@ struct vf_mg {
         }
         nred = (int)redToFull.size();
         ldA = std::max(64, (nred + 63) / 64 * 64);
-        // the coarsest operator is held as a dense FP64 inverse: bound its size (24576^2 doubles = 4.8 GB)
+        // the solves use the explicit dense L^-1: bound its size (24576^2 doubles = 4.8 GB)
         if (ldA > 24576)
             throw std::runtime_error("direct solve of " + std::to_string(nred) +
-                                     " unknowns exceeds the dense coarse-solver limit (24576); use more coarsening "
-                                     "levels");
+                                     " unknowns exceeds the coarse-solver limit (24576); use more coarsening levels");
+        // half-bandwidth of the reduced operator in the natural order, in blocks of 64
+        int bw = 0;
+        for (int n = 0; n < g.nnodes; ++n) {
+            const int ix = n / (g.nn[1] * g.nn[2]), iy = (n / g.nn[2]) % g.nn[1], iz = n % g.nn[2];
+            for (int k = 0; k < 3; ++k) {
+                const int rr = red[3 * (size_t)n + k];
+                if (rr < 0) continue;
+                for (int dx = -1; dx <= 1; ++dx)
+                    for (int dy = -1; dy <= 1; ++dy)
+                        for (int dz = -1; dz <= 1; ++dz) {
+                            const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                            if (jx < 0 || jy < 0 || jz < 0 || jx >= g.nn[0] || jy >= g.nn[1] || jz >= g.nn[2]) continue;
+                            const int m = (jx * g.nn[1] + jy) * g.nn[2] + jz;
+                            for (int c2 = 0; c2 < 3; ++c2) {
+                                const int cc = red[3 * (size_t)m + c2];
+                                if (cc >= 0 && cc < rr) bw = std::max(bw, rr - cc);
+                            }
+                        }
+            }
+        }
+        bandKb = std::max(1, (bw + 63) / 64);
         dRed.resize(red.size());
         dRedToFull.resize(std::max<size_t>(redToFull.size(), 1));
         h2d(dRed.data(), red.data(), red.size(), s);
         h2d(dRedToFull.data(), redToFull.data(), redToFull.size(), s);
         Ainv.resize((size_t)ldA * ldA);
-        invScratch.resize(64 * 64 + 3 * (size_t)64 * ldA);
+        band.resize(band_storage(std::max(nred, 1), bandKb));
+        invScratch.resize((size_t)(ldA / 64) * 64 * 64);
         ctmp_in.resize(ldA);
         ctmp_out.resize(ldA);
-        launch_assemble_dense(g, c.Ke.data(), dRed.data(), ldA, Ainv.data(), s);
-        spd_inverse_inplace(nred, ldA, Ainv.data(), invScratch.data(), s);
+        cholFlag.resize(1);
+        if (nred == 0) return;  // nothing to solve: the coarse correction is zero
+        band_cholesky_inverse(g, c.Ke.data(), dRed.data(), std::max(nred, 1), bandKb, band.data(), invScratch.data(),
+                              Ainv.data(), cholFlag.data(), s);
+        int flag = 0;
+        d2h(&flag, cholFlag.data(), 1, s);
+        sync(s);
+        if (flag) throw std::runtime_error("coarsest-level operator is not positive definite");
     }
 
     // ---------------------------------------------------------------- PCG (MultigridSolver.hh:1046-1152)
@@ -875,24 +967,47 @@ struct vf_mg {
         int64_t it = 0;
         bool updated = false, first = true;
         const bool flexible = prec == VF_PREC_FP32;
-        while (it < maxIter && rr > tol * tol * bb) {
-            ++it;
-            if (mgIter > 0) {
-                if (!updated) { update_stiffness(); updated = true; }
-                precondition(r, sv, mgIter, nu, fullMG, prec);
-            } else {
-                d2d(sv, r, n0, s);
-            }
+        // one PCG iteration, device work only (every launch is on s, no host synchronisation)
+        auto body = [&](bool firstIt) {
+            if (mgIter > 0) precondition(r, sv, mgIter, nu, fullMG, prec);
+            else d2d(sv, r, n0, s);
             zero_dirichlet<double>(0, sv);
             launch_pcg_scalars(sc, 0, s);  // rz_old <- rz
-            if (flexible && !first) launch_dot2_active(g, r, sv, Ad, part, sc, s);  // sc[0]=r.s, sc[1]=Ad_prev.s
+            if (flexible && !firstIt) launch_dot2_active(g, r, sv, Ad, part, sc, s);  // sc[0]=r.s, sc[1]=Ad_prev.s
             else launch_dot_active(g, r, sv, part, sc, s);
-            if (first) d2d(d, sv, n0, s);
+            if (firstIt) d2d(d, sv, n0, s);
             else launch_pcg_direction(g, sv, d, sc, flexible ? 1 : 0, s);
             apply<double>(0, d, Ad, nullptr, AM_SET, true);
             launch_dot_active(g, d, Ad, part, sc + 3, s);
             launch_pcg_scalars(sc, 1, s);  // alpha of this iteration
             launch_pcg_update(g, x, r, d, Ad, sc, part, sc + 4, s);
+        };
+        // With useGraphs, the steady-state iteration (second and later) is captured once into a HIP graph and
+        // replayed: the first iteration runs eagerly (it also builds the hierarchy, tunes kernels and
+        // allocates work buffers, none of which may happen under capture).
+        GraphGuard graph;
+        while (it < maxIter && rr > tol * tol * bb) {
+            ++it;
+            if (mgIter > 0 && !updated) { update_stiffness(); updated = true; }
+            if (useGraphs && !first && s != nullptr) {
+                if (!graph.exec) {
+                    hipGraph_t gr = nullptr;
+                    VF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                    try {
+                        body(false);
+                    } catch (...) {
+                        (void)hipStreamEndCapture(s, &gr);
+                        if (gr) (void)hipGraphDestroy(gr);
+                        throw;
+                    }
+                    VF_HIP(hipStreamEndCapture(s, &gr));
+                    graph.graph = gr;
+                    VF_HIP(hipGraphInstantiate(&graph.exec, gr, nullptr, nullptr, 0));
+                }
+                VF_HIP(hipGraphLaunch(graph.exec, s));
+            } else {
+                body(first);
+            }
             rr = sim->read_scalar(sc + 4);
             if (std::isnan(rr)) throw NanError("NaN encountered at iteration" + std::to_string(it));
             callback(it);
@@ -1008,6 +1123,21 @@ int vf_memcpy(void *dst, const void *src, size_t bytes, int kind, void *stream) 
     VF_API_END
 }
 
+int vf_stream_create(void **stream) {
+    VF_API_BEGIN
+    need(stream != nullptr, "null argument");
+    hipStream_t s = nullptr;
+    VF_HIP(hipStreamCreateWithFlags(&s, hipStreamDefault));
+    *stream = (void *)s;
+    VF_API_END
+}
+
+int vf_stream_destroy(void *stream) {
+    VF_API_BEGIN
+    if (stream) VF_HIP(hipStreamDestroy((hipStream_t)stream));
+    VF_API_END
+}
+
 int vf_stream_sync(void *stream) {
     VF_API_BEGIN
     sync((hipStream_t)stream);
@@ -1017,8 +1147,9 @@ const char *vf_version(void) { return "voxelfem-mi355x 0.1 (gfx950)"; }
 
 int vf_set_kernel_variant(int variant) {
     VF_API_BEGIN
-    need(variant >= 0 && variant <= 3,
-         "kernel variant must be 0 (autotuned), 1 (register-gather), 2 (LDS-tiled) or 3 (Walsh K u, LDS-tiled smoother)");
+    need(variant >= 0 && variant <= 4,
+         "kernel variant must be 0 (autotuned), 1 (register-gather), 2 (LDS-tiled), 3 (Walsh K u, LDS-tiled smoother) "
+         "or 4 (Walsh K u, structured register-gather smoother)");
     g_kernel_variant = variant;
     VF_API_END
 }
@@ -1561,6 +1692,131 @@ int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
     VF_API_END
 }
 
+int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                            int64_t smoothing, int fullMultigrid, int precision, int64_t pipelines, int useGraphs,
+                            double *objective, double *grad, int gradWhere, double *perLayerCompliance,
+                            int64_t *perLayerIterations) {
+    VF_API_BEGIN
+    need(mg && objective && grad, "null argument");
+    need(layerIncrement >= 1, "layerIncrement must be positive");
+    vf_sim *sim = mg->sim;
+    const int64_t numLayers = sim->ne[1];
+    std::vector<int64_t> heights;
+    for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) heights.push_back(l);
+    const int nl = (int)heights.size();
+    int P = pipelines > 0 ? (int)pipelines : 8;
+    P = std::max(1, std::min(P, nl));
+    const size_t n = 3 * sim->nn(), ne = sim->nel();
+    // the base solver's kernel choices are reused by every pipeline (tuned once, on the full design)
+    sim->set_mask_height(sim->h[1] * (double)numLayers, true);
+    mg->sync_masks();
+    mg->ensure_stiffness();
+    mg->tune<double>();
+    if (precision == VF_PREC_FP32) mg->tune<float>();
+    sync(mg->st());
+
+    struct Pipe {
+        hipStream_t stream = nullptr;
+        std::unique_ptr<vf_sim> sim;
+        std::unique_ptr<vf_mg> mg;
+        DBuf<double> f, u;
+    };
+    std::vector<Pipe> pipes(P);
+    DBuf<double> grads((size_t)nl * ne);
+    std::vector<double> comp(nl, 0.0);
+    std::vector<int64_t> its(nl, 0);
+    auto cleanup = [&]() {
+        for (auto &p : pipes) {
+            if (p.stream) (void)hipStreamSynchronize(p.stream);
+            p.mg.reset();
+            p.sim.reset();
+            p.f.release();
+            p.u.release();
+            if (p.stream) (void)hipStreamDestroy(p.stream);
+            p.stream = nullptr;
+        }
+    };
+    try {
+        for (int p = 0; p < P; ++p) {
+            VF_HIP(hipStreamCreateWithFlags(&pipes[p].stream, hipStreamNonBlocking));
+            pipes[p].sim = sim->clone(pipes[p].stream);
+            pipes[p].mg = std::make_unique<vf_mg>(pipes[p].sim.get(), mg->L);
+            vf_mg &m = *pipes[p].mg;
+            m.symmetricGS = mg->symmetricGS;
+            m.useGraphs = useGraphs != 0;
+            std::copy(mg->kuChoice, mg->kuChoice + 2, m.kuChoice);
+            std::copy(mg->gsChoice, mg->gsChoice + 2, m.gsChoice);
+            pipes[p].f.resize(n);
+            pipes[p].u.resize(n);
+        }
+        // layer k (height heights[k], largest first) runs on pipeline k % P; each pipeline is driven by
+        // its own host thread, so the pipelines' kernels overlap on the device
+        std::vector<std::exception_ptr> errs(P);
+        std::vector<std::thread> threads;
+        int device = 0;
+        VF_HIP(hipGetDevice(&device));
+        for (int p = 0; p < P; ++p)
+            threads.emplace_back([&, p]() {
+                try {
+                    VF_HIP(hipSetDevice(device));
+                    Pipe &pp = pipes[p];
+                    vf_sim &S = *pp.sim;
+                    vf_mg &M = *pp.mg;
+                    hipStream_t st = pp.stream;
+                    for (int k = p; k < nl; k += P) {
+                        S.set_mask_height(S.h[1] * (double)heights[k], true);
+                        M.sync_masks();
+                        if (S.hasForce) d2d(pp.f.data(), S.force.data(), n, st);
+                        else launch_fill<double>(n, pp.f.data(), 0.0, st);
+                        if (S.grav[0] != 0 || S.grav[1] != 0 || S.grav[2] != 0)
+                            launch_self_weight(S.g, S.rho.data(), S.grav, S.h[0] * S.h[1] * S.h[2], pp.f.data(), st);
+                        launch_fill<double>(n, pp.u.data(), 0.0, st);
+                        its[k] = M.pcg(pp.u.data(), pp.f.data(), maxIter, tol, (int)mgIterations, (int)smoothing,
+                                       fullMultigrid != 0, precision, true, nullptr, nullptr, nullptr);
+                        double *sc = S.scalars();
+                        launch_dot_active(S.g, pp.f.data(), pp.u.data(), S.red_partials(reduction_partials(S.g)),
+                                          sc + 10, st);
+                        comp[k] = S.read_scalar(sc + 10);
+                        launch_compliance_gradient(S.g, S.K0.data(), S.rho.data(), pp.u.data(),
+                                                   grads.data() + (size_t)k * ne, S.law, S.E0, S.Emin, S.gamma, S.q,
+                                                   S.grav, S.h[0] * S.h[1] * S.h[2], 0, st);
+                    }
+                    sync(st);
+                } catch (...) {
+                    errs[p] = std::current_exception();
+                }
+            });
+        for (auto &t : threads) t.join();
+        for (auto &e : errs)
+            if (e) std::rethrow_exception(e);
+    } catch (...) {
+        cleanup();
+        throw;
+    }
+    cleanup();
+    // objective and gradient summed in layer order (the order of the sequential evaluator)
+    double total = 0;
+    for (int k = 0; k < nl; ++k) {
+        total += comp[k];
+        if (perLayerCompliance) perLayerCompliance[k] = comp[k];
+        if (perLayerIterations) perLayerIterations[k] = its[k];
+    }
+    *objective = 0.5 * total / (double)nl;
+    hipStream_t s = mg->st();
+    DBuf<double> g(ne);
+    launch_fill<double>(ne, g.data(), 0.0, s);
+    for (int k = 0; k < nl; ++k) launch_axpy_plain(ne, 1.0, grads.data() + (size_t)k * ne, g.data(), s);
+    std::vector<double> hg(ne);
+    d2h(hg.data(), g.data(), ne, s);
+    sync(s);
+    for (auto &v : hg) v /= (double)nl;
+    if (gradWhere == VF_DEVICE) { h2d(grad, hg.data(), ne, s); sync(s); }
+    else std::copy(hg.begin(), hg.end(), grad);
+    // the base simulator ends, like the sequential evaluator, at the last (lowest) layer's mask
+    sim->set_mask_height(sim->h[1] * (double)heights.back(), true);
+    VF_API_END
+}
+
 int vf_smoothing_filter(const int64_t dims[3], double rmin, int type, const double *in, double *out, int where,
                         void *stream) {
     VF_API_BEGIN
@@ -1665,7 +1921,14 @@ int vf_bench_apply_K(vf_sim *sim, int64_t reps, int precision, double *avg_ms) {
     VF_API_END
 }
 
-int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[5]) {
+int vf_mg_set_use_graphs(vf_mg *mg, int enable) {
+    VF_API_BEGIN
+    need(mg != nullptr, "null solver");
+    mg->useGraphs = enable != 0;
+    VF_API_END
+}
+
+int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]) {
     VF_API_BEGIN
     need(mg && choice && ms, "null argument");
     need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
@@ -1675,13 +1938,13 @@ int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[5]) {
     if (p == 0) {
         mg->tune<double>();
         choice[0] = mg->ku_family<double>();
-        choice[1] = mg->tiled_gs<double>() ? 1 : 0;
+        choice[1] = mg->gs_family<double>();
     } else {
         mg->tune<float>();
         choice[0] = mg->ku_family<float>();
-        choice[1] = mg->tiled_gs<float>() ? 1 : 0;
+        choice[1] = mg->gs_family<float>();
     }
-    for (int i = 0; i < 5; ++i) ms[i] = mg->tuneMs[p][i];
+    for (int i = 0; i < 6; ++i) ms[i] = mg->tuneMs[p][i];
     VF_API_END
 }
 

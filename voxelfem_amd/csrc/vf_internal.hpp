@@ -126,6 +126,10 @@ void launch_fine_apply_tiled(const Grid &g, const T *K0, const T *E, const T *u,
 template <typename T>
 void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
                              const uint8_t *dmask, int mode, hipStream_t s);
+// One colour of the fine block Gauss-Seidel sweep on the structured element matrix, register gather
+template <typename T>
+void launch_fine_gs_gather_f(const Grid &g, const T *F, const T *E, T *u, const T *b, const uint8_t *dmask, int color,
+                             int forward, hipStream_t s);
 // One x-parity of a fine block Gauss-Seidel sweep: planes of that parity read from uin, neighbour
 // planes from unb, result planes written to uout (uout != uin). A forward sweep is parity 0 with
 // unb = uin then parity 1 with unb = uout; a backward sweep runs parity 1 first.
@@ -203,13 +207,14 @@ void launch_assemble_blocks(const Grid &g, const double *Ke, double *blocks, hip
 // level-1 block rows assembled from the fine moduli and the eight coarsened element matrices
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *blocks,
                                hipStream_t s);
-void launch_assemble_dense(const Grid &g, const double *Ke, const int *red, int ld, double *A, hipStream_t s);
-// in-place blocked Gauss-Jordan inverse of an SPD matrix (n x n row-major, leading dim ld);
-// returns -A^-1 in A (the sweep-operator sign), scratch >= 3*ld*64 + 64*64 doubles
-void spd_inverse_inplace(int n, int ld, double *A, double *scratch, hipStream_t s);
+// coarsest level: band Cholesky of the reduced operator (half-bandwidth kb blocks of 64) and the explicit
+// X = L^-1 (dense lower triangular, leading dimension n rounded up to 64); flag != 0 if not positive definite
+size_t band_storage(int n, int kb);
+void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int n, int kb, double *B, double *Linv,
+                           double *X, int *flag, hipStream_t s);
 template <typename T>
-void launch_coarse_apply(int nred, const double *Ainv_neg, const int *redToFull, int nnodes, const T *b, T *x,
-                         double *tmp_in, double *tmp_out, hipStream_t s);
+void launch_coarse_chol_apply(int nred, const double *X, const int *redToFull, int nnodes, const T *b, T *x,
+                              double *tmp_in, double *tmp_out, hipStream_t s);
 
 // elementwise physics
 void launch_young(const Grid &g, const double *rho, double *E, float *E32, int law, double E0, double Emin,
@@ -223,6 +228,8 @@ void launch_energy_density(const Grid &g, const double *K0, const double *E, con
 void launch_self_weight(const Grid &g, const double *rho, const double *grav, double vol, double *f, hipStream_t s);
 
 // filters / optimiser
+// y += a x
+void launch_axpy_plain(size_t n, double a, const double *x, double *y, hipStream_t s);
 void launch_smoothing_filter(const int dims[3], double rmin, int type, const double *in, double *out, hipStream_t s);
 void launch_projection(double beta, size_t n, const double *in, double *out, hipStream_t s);
 void launch_projection_backprop(double beta, size_t n, const double *g, const double *vars, double *out,

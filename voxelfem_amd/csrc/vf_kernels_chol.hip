@@ -1,0 +1,345 @@
+// vf_kernels_chol.hip -- direct solve of the coarsest multigrid level on the device (FP64).
+//
+// The reference factors the reduced coarsest operator with CHOLMOD (TensorProductSimulator.hh:1198-1230)
+// after every stiffness update. Here the reduced operator (fixed and detached DoFs removed, natural node
+// order, so its half-bandwidth is about 3 (nny nnz + nnz + 1)) is stored as 64 x 64 blocks of its lower
+// band and factored by a blocked right-looking band Cholesky A = L L^T:
+//   for each block column k: L_kk = chol(A_kk) (one workgroup, LDS), L_ik = A_ik L_kk^-T for the kb blocks
+//   below, A_ij -= L_ik L_jk^T for the trailing triangle of the band.
+// The work is O(n bw^2) instead of the O(n^3) of a dense factorisation. For the solves, the explicit dense
+// lower-triangular X = L^-1 is formed (one workgroup per block column, sequential down the column), so a
+// coarse solve x = X^T (X b) is two bandwidth-bound triangular matrix-vector products instead of a chain of
+// small sequential triangular solves.
+#include "vf_internal.hpp"
+
+namespace vf {
+namespace {
+
+constexpr int NB = 64;        // block size
+constexpr int BB = NB * NB;   // doubles per block
+constexpr int TPB = 256;
+
+// block (I, I-q) of the band, row-major inside the block
+__device__ __forceinline__ double *bandblk(double *B, int kb, int I, int q) { return B + ((size_t)I * (kb + 1) + q) * BB; }
+__device__ __forceinline__ const double *bandblk(const double *B, int kb, int I, int q) {
+    return B + ((size_t)I * (kb + 1) + q) * BB;
+}
+
+// ---- assembly of the lower band from the cached coarsest element matrices (one work-item per
+//      (row DoF, neighbour slot)); rows/cols of fixed or detached DoFs are absent (red[] < 0).
+__global__ void __launch_bounds__(TPB) k_band_assemble(Grid g, const double *__restrict__ Ke, const int *__restrict__ red,
+                                                       int kb, double *__restrict__ B) {
+    const long t = (long)blockIdx.x * TPB + threadIdx.x;
+    if (t >= (long)g.nnodes * 27 * 3) return;
+    const int row = (int)(t % (3L * g.nnodes)), slot = (int)(t / (3L * g.nnodes));
+    const int rr = red[row];
+    if (rr < 0) return;
+    const int n = row / 3, a = row % 3;
+    const int nz = g.nn[2], ix = n / (g.nn[1] * nz), iy = (n / nz) % g.nn[1], iz = n % nz;
+    const int off[3] = {slot / 9 - 1, (slot / 3) % 3 - 1, slot % 3 - 1};
+    const int id[3] = {ix, iy, iz};
+    for (int d = 0; d < 3; ++d)
+        if ((unsigned)(id[d] + off[d]) >= (unsigned)g.nn[d]) return;
+    const int m = ((ix + off[0]) * g.nn[1] + iy + off[1]) * nz + iz + off[2];
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = max(max(id[d] - 1, id[d] + off[d] - 1), 0);
+        hi[d] = min(min(id[d], id[d] + off[d]), g.ne[d] - 1);
+    }
+    double Bv[3] = {0, 0, 0};
+    for (int ex = lo[0]; ex <= hi[0]; ++ex)
+        for (int ey = lo[1]; ey <= hi[1]; ++ey)
+            for (int ez = lo[2]; ez <= hi[2]; ++ez) {
+                const int ln = 4 * (ix - ex) + 2 * (iy - ey) + (iz - ez);
+                const int lm = 4 * (ix + off[0] - ex) + 2 * (iy + off[1] - ey) + (iz + off[2] - ez);
+                const double *K = Ke + (size_t)((ex * g.ne[1] + ey) * g.ne[2] + ez) * 576;
+                for (int c = 0; c < 3; ++c) Bv[c] += K[(3 * ln + a) * 24 + 3 * lm + c];
+            }
+    for (int c = 0; c < 3; ++c) {
+        const int cc = red[3 * m + c];
+        if (cc < 0 || cc > rr) continue;  // lower triangle only
+        const int I = rr / NB, J = cc / NB;
+        bandblk(B, kb, I, I - J)[(rr % NB) * NB + cc % NB] = Bv[c];
+    }
+}
+
+// identity on the padding rows of the last diagonal block
+__global__ void k_band_pad(int n, int nb, int kb, double *B) {
+    const int i = n + threadIdx.x;
+    if (i < nb * NB) bandblk(B, kb, nb - 1, 0)[(i % NB) * NB + i % NB] = 1.0;
+}
+
+// ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1, one workgroup. flag != 0 on a non-positive pivot.
+__global__ void __launch_bounds__(TPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
+                                                    int *__restrict__ flag) {
+    __shared__ double A[NB][NB + 1];
+    __shared__ double X[NB][NB + 1];
+    const int t = threadIdx.x;
+    double *Dk = bandblk(B, kb, k, 0);
+    for (int w = t; w < BB; w += TPB) A[w / NB][w % NB] = Dk[w];
+    __syncthreads();
+    for (int j = 0; j < NB; ++j) {
+        // column j: pivot, scale, rank-1 update of the trailing lower triangle
+        const double piv = A[j][j];
+        __syncthreads();
+        if (!(piv > 0)) {
+            if (t == 0) atomicOr(flag, 1);
+            return;  // uniform: every work-item read the same pivot
+        }
+        const double ljj = sqrt(piv);
+        const double inv = 1.0 / ljj;
+        if (t == 0) A[j][j] = ljj;
+        for (int i = j + 1 + t; i < NB; i += TPB) A[i][j] *= inv;
+        __syncthreads();
+        const int m = NB - j - 1;
+        for (int w = t; w < m * m; w += TPB) {
+            const int r = j + 1 + w / m, c = j + 1 + w % m;
+            if (c <= r) A[r][c] -= A[r][j] * A[c][j];
+        }
+        __syncthreads();
+    }
+    // X = L^-1 column by column (work-item c < NB solves L x = e_c by forward substitution)
+    if (t < NB) {
+        const int c = t;
+        for (int r = 0; r < NB; ++r) {
+            double v = r == c ? 1.0 : 0.0;
+            for (int m = c; m < r; ++m) v -= A[r][m] * X[m][c];
+            X[r][c] = r < c ? 0.0 : v / A[r][r];
+        }
+    }
+    __syncthreads();
+    double *Lk = Linv + (size_t)k * BB;
+    for (int w = t; w < BB; w += TPB) {
+        const int r = w / NB, c = w % NB;
+        Dk[w] = c <= r ? A[r][c] : 0.0;
+        Lk[w] = X[r][c];
+    }
+}
+
+// C (+)= alpha * P * Q^T for 64 x 64 row-major blocks, 256 work-items with 4 x 4 register tiles.
+// sP and sQ are LDS staging (P, Q copied in); accumulates into acc[4][4] for rows 4*(t/16).., cols 4*(t%16)..
+__device__ __forceinline__ void gemm_nt_64(const double (*sP)[NB + 1], const double (*sQ)[NB + 1], int t,
+                                           double acc[4][4]) {
+    const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+    for (int m = 0; m < NB; ++m) {
+        double p[4], q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            p[i] = sP[r0 + i][m];
+            q[i] = sQ[c0 + i][m];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] += p[i] * q[j];
+    }
+}
+
+// ---- L_ik = A_ik Linv_kk^T for i = k+1 .. min(k+kb, nb-1)   (grid: one workgroup per block)
+__global__ void __launch_bounds__(TPB) k_trsm_panel(int k, int nb, int kb, double *__restrict__ B,
+                                                    const double *__restrict__ Linv) {
+    __shared__ double sP[NB][NB + 1];
+    __shared__ double sQ[NB][NB + 1];
+    const int q = 1 + blockIdx.x, I = k + q;
+    if (I >= nb) return;
+    const int t = threadIdx.x;
+    double *Aik = bandblk(B, kb, I, q);
+    const double *Lk = Linv + (size_t)k * BB;
+    for (int w = t; w < BB; w += TPB) {
+        sP[w / NB][w % NB] = Aik[w];
+        sQ[w / NB][w % NB] = Lk[w];
+    }
+    __syncthreads();
+    double acc[4][4] = {};
+    gemm_nt_64(sP, sQ, t, acc);
+    __syncthreads();
+    const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Aik[(r0 + i) * NB + c0 + j] = acc[i][j];
+}
+
+// ---- A_IJ -= L_Ik L_Jk^T for k < J <= I <= min(k+kb, nb-1)  (grid: one workgroup per pair)
+__global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, double *__restrict__ B) {
+    __shared__ double sP[NB][NB + 1];
+    __shared__ double sQ[NB][NB + 1];
+    // decode pair index -> (a, b) with 1 <= b <= a <= m
+    int p = blockIdx.x, a = 1;
+    while (p >= a) { p -= a; ++a; }
+    const int b = p + 1;
+    const int I = k + a, J = k + b;
+    if (I >= nb) return;
+    const int t = threadIdx.x;
+    const double *Lik = bandblk(B, kb, I, a), *Ljk = bandblk(B, kb, J, b);
+    double *Aij = bandblk(B, kb, I, a - b);
+    for (int w = t; w < BB; w += TPB) {
+        sP[w / NB][w % NB] = Lik[w];
+        sQ[w / NB][w % NB] = Ljk[w];
+    }
+    __syncthreads();
+    double acc[4][4] = {};
+    gemm_nt_64(sP, sQ, t, acc);
+    const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) Aij[(r0 + i) * NB + c0 + j] -= acc[i][j];
+}
+
+// ---- X = L^-1 (dense lower triangular, row-major, ld = nb*64): one workgroup per block column J.
+//      X_JJ = Linv_JJ; X_IJ = -Linv_II * sum_{q=1..min(kb,I-J)} L_{I,I-q} X_{I-q,J}.
+__global__ void __launch_bounds__(TPB) k_band_inverse(int nb, int kb, const double *__restrict__ B,
+                                                      const double *__restrict__ Linv, double *__restrict__ X) {
+    __shared__ double sP[NB][NB + 1];
+    __shared__ double sQ[NB][NB + 1];
+    const int J = blockIdx.x, t = threadIdx.x;
+    const size_t ld = (size_t)nb * NB;
+    const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
+    auto Xblk = [&](int I) { return X + (size_t)I * NB * ld + (size_t)J * NB; };
+    {
+        const double *Lj = Linv + (size_t)J * BB;
+        double *Xjj = Xblk(J);
+        for (int w = t; w < BB; w += TPB) Xjj[(size_t)(w / NB) * ld + w % NB] = Lj[w];
+    }
+    __threadfence();
+    __syncthreads();
+    for (int I = J + 1; I < nb; ++I) {
+        // T = sum_q L_{I,I-q} X_{I-q,J}   (T^T computed as gemm_nt with P = L, Q = X^T)
+        double acc[4][4] = {};
+        const int qmax = min(kb, I - J);
+        for (int q = 1; q <= qmax; ++q) {
+            const double *Lb = bandblk(B, kb, I, q);
+            const double *Xb = Xblk(I - q);
+            __syncthreads();
+            for (int w = t; w < BB; w += TPB) {
+                const int r = w / NB, c = w % NB;
+                sP[r][c] = Lb[w];
+                sQ[c][r] = Xb[(size_t)r * ld + c];  // sQ[col][m] = X[m][col]
+            }
+            __syncthreads();
+            gemm_nt_64(sP, sQ, t, acc);
+        }
+        // X_IJ = -Linv_II T
+        __syncthreads();
+        const double *Li = Linv + (size_t)I * BB;
+        for (int w = t; w < BB; w += TPB) sP[w / NB][w % NB] = Li[w];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) sQ[c0 + j][r0 + i] = acc[i][j];  // sQ[col][m] = T[m][col]
+        __syncthreads();
+        double out[4][4] = {};
+        gemm_nt_64(sP, sQ, t, out);
+        double *Xi = Xblk(I);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) Xi[(size_t)(r0 + i) * ld + c0 + j] = -out[i][j];
+        // the block is read back by other work-items of this workgroup in the next kb steps
+        __threadfence();
+        __syncthreads();
+    }
+}
+
+// ---- y = X b over the lower triangle (one wavefront per row)
+__global__ void __launch_bounds__(TPB) k_trmv_lower(int n, int ld, const double *__restrict__ X,
+                                                    const double *__restrict__ b, double *__restrict__ y) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double *Xr = X + (size_t)row * ld;
+    double s = 0;
+    for (int j = lane; j <= row; j += 64) s += Xr[j] * b[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[row] = s;
+}
+
+// ---- x = X^T y over the lower triangle: workgroup of 256 handles 64 columns; rows split over 4
+//      wavefront groups, combined through LDS in a fixed order
+__global__ void __launch_bounds__(TPB) k_trmv_upper_t(int n, int ld, const double *__restrict__ X,
+                                                      const double *__restrict__ y, double *__restrict__ x) {
+    __shared__ double part[4][64];
+    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+    const int col = blockIdx.x * 64 + lane;
+    double s = 0;
+    if (col < n)
+        for (int i = col + grp; i < n; i += 4) s += X[(size_t)i * ld + col] * y[i];
+    part[grp][lane] = s;
+    __syncthreads();
+    if (grp == 0 && col < n) x[col] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_cgather(int nred, const int *__restrict__ redToFull, int nnodes,
+                                                 const T *__restrict__ b, double *__restrict__ out) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= nred) return;
+    const int f = redToFull[i];
+    out[i] = (double)b[(f % 3) * nnodes + f / 3];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_cscatter(int nred, const int *__restrict__ redToFull, int nnodes,
+                                                  const double *__restrict__ in, T *__restrict__ x) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= nred) return;
+    const int f = redToFull[i];
+    x[(f % 3) * nnodes + f / 3] = (T)in[i];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_cfill(size_t n, T *x, T v) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) x[i] = v;
+}
+
+inline unsigned nblk(long count) { return (unsigned)((count + TPB - 1) / TPB); }
+
+}  // namespace
+
+size_t band_storage(int n, int kb) {
+    const int nb = (n + NB - 1) / NB;
+    return (size_t)nb * (kb + 1) * BB;
+}
+
+void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int n, int kb, double *B, double *Linv,
+                           double *X, int *flag, hipStream_t s) {
+    const int nb = (n + NB - 1) / NB;
+    const size_t ld = (size_t)nb * NB;
+    VF_HIP(hipMemsetAsync(B, 0, band_storage(n, kb) * sizeof(double), s));
+    VF_HIP(hipMemsetAsync(X, 0, ld * ld * sizeof(double), s));
+    VF_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B);
+    if ((int)ld > n) hipLaunchKernelGGL(k_band_pad, dim3(1), dim3(NB), 0, s, n, nb, kb, B);
+    for (int k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(TPB), 0, s, k, kb, B, Linv, flag);
+        const int m = std::min(kb, nb - 1 - k);
+        if (m > 0) {
+            hipLaunchKernelGGL(k_trsm_panel, dim3(m), dim3(TPB), 0, s, k, nb, kb, B, Linv);
+            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B);
+        }
+    }
+    hipLaunchKernelGGL(k_band_inverse, dim3(nb), dim3(TPB), 0, s, nb, kb, B, Linv, X);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_coarse_chol_apply(int nred, const double *X, const int *redToFull, int nnodes, const T *b, T *x,
+                              double *tmp_in, double *tmp_out, hipStream_t s) {
+    const int ld = (nred + NB - 1) / NB * NB;
+    hipLaunchKernelGGL(k_cfill<T>, dim3(nblk(3L * nnodes)), dim3(TPB), 0, s, (size_t)3 * nnodes, x, T(0));
+    if (nred == 0) return;
+    hipLaunchKernelGGL(k_cgather<T>, dim3(nblk(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, b, tmp_in);
+    hipLaunchKernelGGL(k_trmv_lower, dim3((nred + 3) / 4), dim3(TPB), 0, s, nred, ld, X, tmp_in, tmp_out);
+    hipLaunchKernelGGL(k_trmv_upper_t, dim3((nred + 63) / 64), dim3(TPB), 0, s, nred, ld, X, tmp_out, tmp_in);
+    hipLaunchKernelGGL(k_cscatter<T>, dim3(nblk(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, tmp_in, x);
+    VF_HIP(hipGetLastError());
+}
+
+template void launch_coarse_chol_apply<double>(int, const double *, const int *, int, const double *, double *,
+                                               double *, double *, hipStream_t);
+template void launch_coarse_chol_apply<float>(int, const double *, const int *, int, const float *, float *,
+                                              double *, double *, hipStream_t);
+
+}  // namespace vf

@@ -1,8 +1,7 @@
 // vf_kernels_setup.hip -- per-density-update setup of the multigrid hierarchy on gfx950:
-// Galerkin coarsened element matrices (MultigridSolver.hh:704-819), assembly of the 27-point
-// block rows of the intermediate levels, assembly of the reduced coarsest operator and its
-// FP64 inverse (replacing the CHOLMOD factorisation of TensorProductSimulator.hh:1198-1230),
-// and the coarse solve as one dense FP64 matrix-vector product.
+// Galerkin coarsened element matrices (MultigridSolver.hh:704-819) and the assembly of the 27-point
+// block rows of the intermediate levels (level 1 straight from the fine moduli). The coarsest level's
+// factorisation lives in vf_kernels_chol.hip.
 #include "vf_internal.hpp"
 
 namespace vf {
@@ -183,213 +182,6 @@ __global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, co
     for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = B[k];
 }
 
-// ---- reduced dense coarsest operator: one work-item per (row DoF, neighbour slot)
-__global__ void __launch_bounds__(TPB) k_assemble_dense(Grid g, const double *__restrict__ Ke,
-                                                        const int *__restrict__ red, int ld, double *__restrict__ A) {
-    const long t = (long)blockIdx.x * TPB + threadIdx.x;
-    if (t >= (long)g.nnodes * 27 * 3) return;
-    const int row = (int)(t % (3L * g.nnodes)), slot = (int)(t / (3L * g.nnodes));
-    const int rr = red[row];
-    if (rr < 0) return;
-    const int n = row / 3, a = row % 3;
-    const int nz = g.nn[2], ix = n / (g.nn[1] * nz), iy = (n / nz) % g.nn[1], iz = n % nz;
-    const int off[3] = {slot / 9 - 1, (slot / 3) % 3 - 1, slot % 3 - 1};
-    const int id[3] = {ix, iy, iz};
-    for (int d = 0; d < 3; ++d)
-        if ((unsigned)(id[d] + off[d]) >= (unsigned)g.nn[d]) return;
-    const int m = ((ix + off[0]) * g.nn[1] + iy + off[1]) * nz + iz + off[2];
-    int lo[3], hi[3];
-    for (int d = 0; d < 3; ++d) {
-        lo[d] = max(max(id[d] - 1, id[d] + off[d] - 1), 0);
-        hi[d] = min(min(id[d], id[d] + off[d]), g.ne[d] - 1);
-    }
-    double B[3] = {0, 0, 0};
-    for (int ex = lo[0]; ex <= hi[0]; ++ex)
-        for (int ey = lo[1]; ey <= hi[1]; ++ey)
-            for (int ez = lo[2]; ez <= hi[2]; ++ez) {
-                const int ln = 4 * (ix - ex) + 2 * (iy - ey) + (iz - ez);
-                const int lm = 4 * (ix + off[0] - ex) + 2 * (iy + off[1] - ey) + (iz + off[2] - ez);
-                const double *K = Ke + (size_t)((ex * g.ne[1] + ey) * g.ne[2] + ez) * 576;
-                for (int c = 0; c < 3; ++c) B[c] += K[(3 * ln + a) * 24 + 3 * lm + c];
-            }
-    for (int c = 0; c < 3; ++c) {
-        const int cc = red[3 * m + c];
-        if (cc >= 0) A[(size_t)rr * ld + cc] = B[c];
-    }
-}
-
-__global__ void k_set_diag(int n0, int n1, int ld, double *A) {
-    const int i = n0 + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n1) A[(size_t)i * ld + i] = 1.0;
-}
-
-// ---- blocked Gauss-Jordan sweep (in place, returns -A^-1). BS x BS pivot blocks.
-constexpr int BS = 64;
-
-// Sweep the pivot block D = A[P,P] in LDS: D <- -D^-1 (scalar sweep operator), store +D^-1 in Dinv.
-__global__ void __launch_bounds__(256) k_pivot_inverse(int p, int ld, const double *__restrict__ A,
-                                                       double *__restrict__ Dinv) {
-    __shared__ double D[BS][BS + 1];
-    __shared__ double colk[BS], rowk[BS];
-    const int t = threadIdx.x;
-    const size_t base = (size_t)p * BS;
-    for (int w = t; w < BS * BS; w += 256) D[w / BS][w % BS] = A[(base + w / BS) * ld + base + w % BS];
-    __syncthreads();
-    for (int k = 0; k < BS; ++k) {
-        const double dinv = 1.0 / D[k][k];
-        if (t < BS) {
-            colk[t] = D[t][k];
-            rowk[t] = D[k][t] * dinv;
-        }
-        __syncthreads();
-        // each work-item owns a 4x4 sub-block: rows (t/16)*4.., columns (t%16)*4..
-        const int i0 = (t >> 4) * 4, j0 = (t & 15) * 4;
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int i = i0 + a;
-            const double ci = colk[i];
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int j = j0 + b;
-                double v;
-                if (i == k && j == k) v = -dinv;
-                else if (i == k) v = rowk[j];
-                else if (j == k) v = ci * dinv;
-                else v = D[i][j] - ci * rowk[j];
-                D[i][j] = v;
-            }
-        }
-        __syncthreads();
-    }
-    for (int w = t; w < BS * BS; w += 256) Dinv[w] = -D[w / BS][w % BS];
-}
-
-// W[:, J] = Dinv * A[P, J] and CD[I, :] = A[I, P] * Dinv, Cold[I,:] = A[I,P] for every 64-tile J / I.
-// blockIdx.y == 0: row panel (W); blockIdx.y == 1: column panel (Cold, CD).
-__global__ void __launch_bounds__(256) k_panels(int p, int ld, const double *__restrict__ A,
-                                                const double *__restrict__ Dinv, double *__restrict__ W,
-                                                double *__restrict__ Cold, double *__restrict__ CD) {
-    __shared__ double sD[BS][BS + 1];
-    __shared__ double sX[BS][BS + 1];
-    const int t = threadIdx.x, J = blockIdx.x;
-    const size_t P0 = (size_t)p * BS, J0 = (size_t)J * BS;
-    for (int w = t; w < BS * BS; w += 256) sD[w / BS][w % BS] = Dinv[w];
-    if (blockIdx.y == 0) {
-        for (int w = t; w < BS * BS; w += 256) sX[w / BS][w % BS] = A[(P0 + w / BS) * ld + J0 + w % BS];
-        __syncthreads();
-        // W[i][J0+j] = sum_k D[i][k] X[k][j]
-        for (int w = t; w < BS * BS; w += 256) {
-            const int i = w / BS, j = w % BS;
-            double s = 0;
-#pragma unroll 16
-            for (int k = 0; k < BS; ++k) s += sD[i][k] * sX[k][j];
-            W[(size_t)i * ld + J0 + j] = s;
-        }
-    } else {
-        for (int w = t; w < BS * BS; w += 256) {
-            const double v = A[(J0 + w / BS) * ld + P0 + w % BS];
-            sX[w / BS][w % BS] = v;
-            Cold[(J0 + w / BS) * BS + w % BS] = v;
-        }
-        __syncthreads();
-        // CD[J0+i][j] = sum_k X[i][k] D[k][j]
-        for (int w = t; w < BS * BS; w += 256) {
-            const int i = w / BS, j = w % BS;
-            double s = 0;
-#pragma unroll 16
-            for (int k = 0; k < BS; ++k) s += sX[i][k] * sD[k][j];
-            CD[(J0 + i) * BS + j] = s;
-        }
-    }
-}
-
-// Tile (I,J) of the sweep update: A_IJ -= Cold_I W_J off the pivot row/column; the pivot row and
-// column take W and CD, the pivot block takes -Dinv.
-__global__ void __launch_bounds__(256) k_sweep_update(int p, int ld, double *__restrict__ A,
-                                                      const double *__restrict__ Dinv, const double *__restrict__ W,
-                                                      const double *__restrict__ Cold, const double *__restrict__ CD) {
-    const int I = blockIdx.y, J = blockIdx.x, t = threadIdx.x;
-    const size_t I0 = (size_t)I * BS, J0 = (size_t)J * BS;
-    if (I == p && J == p) {
-        for (int w = t; w < BS * BS; w += 256) A[(I0 + w / BS) * ld + J0 + w % BS] = -Dinv[w];
-        return;
-    }
-    if (I == p) {
-        for (int w = t; w < BS * BS; w += 256) A[(I0 + w / BS) * ld + J0 + w % BS] = W[(size_t)(w / BS) * ld + J0 + w % BS];
-        return;
-    }
-    if (J == p) {
-        for (int w = t; w < BS * BS; w += 256) A[(I0 + w / BS) * ld + J0 + w % BS] = CD[(I0 + w / BS) * BS + w % BS];
-        return;
-    }
-    __shared__ double sC[BS][BS + 1];
-    __shared__ double sW[BS][BS + 1];
-    for (int w = t; w < BS * BS; w += 256) {
-        sC[w / BS][w % BS] = Cold[(I0 + w / BS) * BS + w % BS];
-        sW[w / BS][w % BS] = W[(size_t)(w / BS) * ld + J0 + w % BS];
-    }
-    __syncthreads();
-    // each work-item owns a 4x4 sub-tile
-    const int ti = (t / 16) * 4, tj = (t % 16) * 4;
-    double acc[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = 0;
-    for (int k = 0; k < BS; ++k) {
-        double c[4], w[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) c[a] = sC[ti + a][k];
-#pragma unroll
-        for (int b = 0; b < 4; ++b) w[b] = sW[k][tj + b];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] += c[a] * w[b];
-    }
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) A[(I0 + ti + a) * ld + J0 + tj + b] -= acc[a][b];
-}
-
-// ---- coarse solve: x_red = -(A_neg) b_red  with one wavefront per row
-template <typename T>
-__global__ void __launch_bounds__(TPB) k_coarse_gather(int nred, const int *__restrict__ redToFull, int nnodes,
-                                                       const T *__restrict__ b, double *__restrict__ out) {
-    const int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= nred) return;
-    const int f = redToFull[i];
-    out[i] = (double)b[(f % 3) * nnodes + f / 3];
-}
-
-__global__ void __launch_bounds__(TPB) k_gemv_neg(int n, int ld, const double *__restrict__ A,
-                                                  const double *__restrict__ x, double *__restrict__ y) {
-    const int lane = threadIdx.x & 63;
-    const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
-    if (row >= n) return;
-    const double *Ar = A + (size_t)row * ld;
-    double s = 0;
-    for (int j = lane; j < n; j += 64) s += Ar[j] * x[j];
-    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-    if (lane == 0) y[row] = -s;
-}
-
-template <typename T>
-__global__ void __launch_bounds__(TPB) k_coarse_scatter(int nred, const int *__restrict__ redToFull, int nnodes,
-                                                        const double *__restrict__ in, T *__restrict__ x) {
-    const int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= nred) return;
-    const int f = redToFull[i];
-    x[(f % 3) * nnodes + f / 3] = (T)in[i];
-}
-
-template <typename T>
-__global__ void __launch_bounds__(TPB) k_fill_plain(size_t n, T *x, T v) {
-    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
-    if (i < n) x[i] = v;
-}
-
 }  // namespace
 
 void launch_galerkin_l1_elems(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *Ke,
@@ -421,46 +213,5 @@ void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Cha
                        blocks);
     VF_HIP(hipGetLastError());
 }
-
-void launch_assemble_dense(const Grid &g, const double *Ke, const int *red, int ld, double *A, hipStream_t s) {
-    const size_t total = (size_t)ld * ld;
-    hipLaunchKernelGGL(k_fill_plain<double>, dim3((unsigned)((total + TPB - 1) / TPB)), dim3(TPB), 0, s, total, A, 0.0);
-    hipLaunchKernelGGL(k_assemble_dense, dim3(nblocks((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, ld, A);
-    VF_HIP(hipGetLastError());
-}
-
-// A: ld x ld with ld a multiple of 64; rows/cols [n, ld) are padding (identity is written here).
-void spd_inverse_inplace(int n, int ld, double *A, double *scratch, hipStream_t s) {
-    if (ld % BS) throw std::runtime_error("spd_inverse_inplace: leading dimension must be a multiple of 64");
-    if (ld > n) hipLaunchKernelGGL(k_set_diag, dim3((ld - n + 255) / 256), dim3(256), 0, s, n, ld, ld, A);
-    const int nb = ld / BS;
-    double *Dinv = scratch;
-    double *W = Dinv + BS * BS;
-    double *Cold = W + (size_t)BS * ld;
-    double *CD = Cold + (size_t)ld * BS;
-    for (int p = 0; p < nb; ++p) {
-        hipLaunchKernelGGL(k_pivot_inverse, dim3(1), dim3(256), 0, s, p, ld, A, Dinv);
-        hipLaunchKernelGGL(k_panels, dim3(nb, 2), dim3(256), 0, s, p, ld, A, Dinv, W, Cold, CD);
-        hipLaunchKernelGGL(k_sweep_update, dim3(nb, nb), dim3(256), 0, s, p, ld, A, Dinv, W, Cold, CD);
-    }
-    VF_HIP(hipGetLastError());
-}
-
-template <typename T>
-void launch_coarse_apply(int nred, const double *Ainv_neg, const int *redToFull, int nnodes, const T *b, T *x,
-                         double *tmp_in, double *tmp_out, hipStream_t s) {
-    const int ld = (nred + BS - 1) / BS * BS;
-    hipLaunchKernelGGL(k_fill_plain<T>, dim3(nblocks(3L * nnodes)), dim3(TPB), 0, s, (size_t)3 * nnodes, x, T(0));
-    if (nred == 0) return;
-    hipLaunchKernelGGL(k_coarse_gather<T>, dim3(nblocks(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, b, tmp_in);
-    hipLaunchKernelGGL(k_gemv_neg, dim3((nred + 3) / 4), dim3(TPB), 0, s, nred, ld, Ainv_neg, tmp_in, tmp_out);
-    hipLaunchKernelGGL(k_coarse_scatter<T>, dim3(nblocks(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, tmp_out, x);
-    VF_HIP(hipGetLastError());
-}
-
-template void launch_coarse_apply<double>(int, const double *, const int *, int, const double *, double *, double *,
-                                          double *, hipStream_t);
-template void launch_coarse_apply<float>(int, const double *, const int *, int, const float *, float *, double *,
-                                         double *, hipStream_t);
 
 }  // namespace vf

@@ -411,6 +411,147 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
     }
 }
 
+// ================================================================== GS, one colour per launch, register gather
+// Row of a node whose 27 neighbour values sit in registers (un[s][c], s = 9 dx + 3 dy + dz, 0..2 each):
+// the structured element matrix of lds_row, neighbours taken in groups of equal delta so each group's
+// nine F values are loaded once.
+template <typename T, bool WANT_M>
+__device__ __forceinline__ void reg_row(const T *__restrict__ F, const T un[27][3], const T Ee[8], T f[3], T M[9]) {
+    T a[8][3];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) a[e][0] = a[e][1] = a[e][2] = T(0);
+    T Fs[9];
+#pragma unroll
+    for (int t = 0; t < 27; ++t) {
+        const int s = kNeighbourOrder[t];
+        const int dx = s / 9, dy = (s / 3) % 3, dz = s % 3;
+        if (t == 0 || delta_of(kNeighbourOrder[t - 1]) != delta_of(s)) {
+            const T *Fg = F + 16 * delta_of(s);
+            asm volatile("" : "+s"(Fg));
+            const CT<T> *Fp = (const CT<T> *)Fg;
+#pragma unroll
+            for (int q = 0; q < 9; ++q) Fs[q] = Fp[q];
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
+            const int mx = dx - 1 + i, my = dy - 1 + j, mz = dz - 1 + k;
+            if (mx < 0 || mx > 1 || my < 0 || my > 1 || mz < 0 || mz > 1) continue;
+#pragma unroll
+            for (int d = 0; d < 3; ++d)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int sg = c == d ? 1 : side(e, c) * side(e, d);
+                    if (sg > 0) a[e][c] += Fs[3 * c + d] * un[s][d];
+                    else a[e][c] -= Fs[3 * c + d] * un[s][d];
+                }
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) asm volatile("" : "+v"(a[e][0]), "+v"(a[e][1]), "+v"(a[e][2]));
+        __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        f[0] += Ee[e] * a[e][0];
+        f[1] += Ee[e] * a[e][1];
+        f[2] += Ee[e] * a[e][2];
+    }
+    if (WANT_M) {
+        T es = 0, exy = 0, exz = 0, eyz = 0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            es += Ee[e];
+            exy += side(e, 0) * side(e, 1) > 0 ? Ee[e] : -Ee[e];
+            exz += side(e, 0) * side(e, 2) > 0 ? Ee[e] : -Ee[e];
+            eyz += side(e, 1) * side(e, 2) > 0 ? Ee[e] : -Ee[e];
+        }
+        const T *Fg = F;
+        asm volatile("" : "+s"(Fg));
+        const CT<T> *F0 = (const CT<T> *)Fg;
+        M[0] = F0[0] * es;
+        M[4] = F0[4] * es;
+        M[8] = F0[8] * es;
+        M[1] = F0[1] * exy;
+        M[3] = F0[3] * exy;
+        M[2] = F0[2] * exz;
+        M[6] = F0[6] * exz;
+        M[5] = F0[5] * eyz;
+        M[7] = F0[7] * eyz;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    if constexpr (sizeof(T) == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return __builtin_bit_cast(T, v);
+    } else {
+        const int v = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        return __builtin_bit_cast(T, v);
+    }
+}
+
+// One colour of the 8-colour block Gauss-Seidel sweep (MultigridSolver.hh:408-458), one node per work-item.
+// Interior nodes load their 81 neighbour values with buffer loads that share one per-lane offset (the
+// neighbour and component offsets are scalar), so the gather costs no vector address arithmetic;
+// nodes on the grid boundary take the bounds-checked path.
+template <typename T>
+__global__ void __launch_bounds__(256) k_gs_gather_f(Grid g, const T *__restrict__ F, const T *__restrict__ E,
+                                                     T *__restrict__ u, const T *__restrict__ b,
+                                                     const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
+                                                     int forward) {
+    const int mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int N = g.nnodes, nx = g.nn[0], ny = g.nn[1], nz = g.nn[2];
+    const int sx = ny * nz, sy = nz;
+    const int n = (ix * ny + iy) * nz + iz;
+    const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
+    if (dm == 7) return;
+    T un[27][3];
+    const bool interior = ix > 0 && ix < nx - 1 && iy > 0 && iy < ny - 1 && iz > 0 && iz < nz - 1;
+    if (interior) {
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void *)u, 0, (int)((size_t)3 * N * sizeof(T)), 0x00020000);
+        const int vo = (n - sx - sy - 1) * (int)sizeof(T);
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+            const int o = (s / 9) * sx + ((s / 3) % 3) * sy + s % 3;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) un[s][c] = buffer_load<T>(rs, vo, (c * N + o) * (int)sizeof(T));
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < 27; ++s) {
+            const int dx = s / 9 - 1, dy = (s / 3) % 3 - 1, dz = s % 3 - 1;
+            const bool ok = (unsigned)(ix + dx) < (unsigned)nx && (unsigned)(iy + dy) < (unsigned)ny &&
+                            (unsigned)(iz + dz) < (unsigned)nz;
+            const int m = n + dx * sx + dy * sy + dz;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) un[s][c] = ok ? u[(size_t)c * N + m] : T(0);
+        }
+    }
+    T Ee[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int ex = ix - ((e >> 2) & 1), ey = iy - ((e >> 1) & 1), ez = iz - (e & 1);
+        const bool ok = (unsigned)ex < (unsigned)g.ne[0] && (unsigned)ey < (unsigned)g.ne[1] &&
+                        (unsigned)ez < (unsigned)g.ne[2];
+        Ee[e] = ok ? E[((size_t)ex * g.ne[1] + ey) * g.ne[2] + ez] : T(0);
+    }
+    T f[3] = {0, 0, 0};
+    T M[9];
+    reg_row<T, true>(F, un, Ee, f, M);
+    const T bms[3] = {b[n] - f[0], b[(size_t)N + n] - f[1], b[(size_t)2 * N + n] - f[2]};
+    T ud[3];
+    gs_point_update(ud, bms, M, dm, forward);
+    u[n] += ud[0];
+    u[(size_t)N + n] += ud[1];
+    u[(size_t)2 * N + n] += ud[2];
+}
+
 inline int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 int cu_count() {
@@ -478,6 +619,24 @@ void launch_fine_gs_phase(const Grid &g, const T *F, const T *E, const T *uin, c
                        unb, uout, b, dmask, parity, forward, ntY, ntZ, pchunk);
     VF_HIP(hipGetLastError());
 }
+
+template <typename T>
+void launch_fine_gs_gather_f(const Grid &g, const T *F, const T *E, T *u, const T *b, const uint8_t *dmask, int color,
+                             int forward, hipStream_t s) {
+    const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
+    const long mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    const long count = (mx > 0 && my > 0 && mz > 0) ? mx * my * mz : 0;
+    if (!count) return;
+    if ((size_t)3 * g.nnodes * sizeof(T) >= ((size_t)1 << 31)) throw std::runtime_error("field too large for 32-bit buffer offsets");
+    hipLaunchKernelGGL(k_gs_gather_f<T>, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, s, g, F, E, u, b, dmask,
+                       cx, cy, cz, forward);
+    VF_HIP(hipGetLastError());
+}
+
+template void launch_fine_gs_gather_f<double>(const Grid &, const double *, const double *, double *, const double *,
+                                              const uint8_t *, int, int, hipStream_t);
+template void launch_fine_gs_gather_f<float>(const Grid &, const float *, const float *, float *, const float *,
+                                             const uint8_t *, int, int, hipStream_t);
 
 template void launch_fine_apply_tiled<double>(const Grid &, const double *, const double *, const double *, double *,
                                               const double *, const uint8_t *, int, hipStream_t);

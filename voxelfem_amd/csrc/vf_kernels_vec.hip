@@ -519,6 +519,20 @@ void launch_self_weight(const Grid &g, const double *rho, const double *grav, do
     VF_HIP(hipGetLastError());
 }
 
+namespace {
+__global__ void __launch_bounds__(TPB) k_axpy_plain(size_t n, double a, const double *__restrict__ x,
+                                                    double *__restrict__ y) {
+    const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
+    if (i < n) y[i] += a * x[i];
+}
+}  // namespace
+
+void launch_axpy_plain(size_t n, double a, const double *x, double *y, hipStream_t s) {
+    if (!n) return;
+    hipLaunchKernelGGL(k_axpy_plain, dim3((unsigned)((n + TPB - 1) / TPB)), dim3(TPB), 0, s, n, a, x, y);
+    VF_HIP(hipGetLastError());
+}
+
 void launch_smoothing_filter(const int dims[3], double rmin, int type, const double *in, double *out, hipStream_t s) {
     const int R = (int)ceil(rmin) - 1;
     double W = 0;
