@@ -155,7 +155,16 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": bytes64 / t64 / 1e9, "peak": PEAK_HBM_GBS, "unit": "GB/s"}
     roof["frac"] = roof["achieved"] / roof["peak"]
+    # measured HBM bytes per launch (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated), if recorded for this
+    # kernel and grid
     roof["traffic"] = None
+    tfile = os.path.join(ROOT, "profiles", "ku_walsh_384x192x192_traffic.json")
+    if tuned["fp64"][0] == "walsh" and os.path.exists(tfile):
+        rec = json.load(open(tfile))
+        if tuple(rec.get("grid", ())) == ne:
+            roof["traffic"] = rec["traffic_bytes"]
+    # the same launch against the HBM roof alone (the Walsh-basis kernel does ~5x fewer flops than F)
+    roof["hbm_frac"] = bytes64 / t64 / 1e9 / PEAK_HBM_GBS
     roof["kernel"] = "fine K.u, FP64, " + {"tiled": "k_ku_tiled<double,0> (LDS-tiled node rows)",
                                            "walsh": "k_ku_walsh<double,0> (element-centric, Walsh basis)",
                                            "gather": "k_fine_apply<double,0> (register-gather)"}[tuned["fp64"][0]]

@@ -47,6 +47,15 @@ inline Grid make_grid(const int64_t ne[3]) {
     return g;
 }
 
+// Workgroups are dispatched to the eight XCDs (each with its own L2) round-robin in the order of their
+// linear id. This maps a linear id to a logical one such that consecutive logical workgroups - which
+// the kernels give neighbouring tiles sharing halo data - run on the same XCD.
+__device__ __forceinline__ int xcd_remap(int linear, int total) {
+    const int q = total / 8, r = total % 8;
+    const int xcd = linear % 8, slot = linear / 8;
+    return xcd * q + (xcd < r ? xcd : r) + slot;
+}
+
 // Simple owning device buffer.
 template <typename T>
 struct DBuf {

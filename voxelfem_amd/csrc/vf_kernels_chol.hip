@@ -70,49 +70,53 @@ __global__ void k_band_pad(int n, int nb, int kb, double *B) {
 }
 
 // ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1, one workgroup. flag != 0 on a non-positive pivot.
+// Right-looking column Cholesky in LDS, two barriers per column: every work-item reads the pivot and
+// the 64 leading ones scale column j; then the 16 x 16 work-item grid updates the trailing triangle.
+// The inverse is formed row by row: X[r][:] = (e_r - sum_{m<r} L[r][m] X[m][:]) / L[r][r], each row's
+// sum split over four groups of 64 work-items and combined in a fixed order.
 __global__ void __launch_bounds__(TPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
                                                     int *__restrict__ flag) {
     __shared__ double A[NB][NB + 1];
     __shared__ double X[NB][NB + 1];
-    const int t = threadIdx.x;
+    __shared__ double part[4][NB];
+    __shared__ double diag[NB];
+    const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
     double *Dk = bandblk(B, kb, k, 0);
     for (int w = t; w < BB; w += TPB) A[w / NB][w % NB] = Dk[w];
     __syncthreads();
     for (int j = 0; j < NB; ++j) {
-        // column j: pivot, scale, rank-1 update of the trailing lower triangle
         const double piv = A[j][j];
-        __syncthreads();
         if (!(piv > 0)) {
             if (t == 0) atomicOr(flag, 1);
             return;  // uniform: every work-item read the same pivot
         }
         const double ljj = sqrt(piv);
-        const double inv = 1.0 / ljj;
-        if (t == 0) A[j][j] = ljj;
-        for (int i = j + 1 + t; i < NB; i += TPB) A[i][j] *= inv;
+        if (t > j && t < NB) A[t][j] *= 1.0 / ljj;
+        if (t == 0) diag[j] = ljj;
         __syncthreads();
-        const int m = NB - j - 1;
-        for (int w = t; w < m * m; w += TPB) {
-            const int r = j + 1 + w / m, c = j + 1 + w % m;
-            if (c <= r) A[r][c] -= A[r][j] * A[c][j];
-        }
+        for (int r = j + 1 + tr; r < NB; r += 16)
+            for (int c = j + 1 + tc; c <= r; c += 16) A[r][c] -= A[r][j] * A[c][j];
         __syncthreads();
     }
-    // X = L^-1 column by column (work-item c < NB solves L x = e_c by forward substitution)
-    if (t < NB) {
-        const int c = t;
-        for (int r = 0; r < NB; ++r) {
-            double v = r == c ? 1.0 : 0.0;
-            for (int m = c; m < r; ++m) v -= A[r][m] * X[m][c];
-            X[r][c] = r < c ? 0.0 : v / A[r][r];
-        }
-    }
+    if (t < NB) A[t][t] = diag[t];
     __syncthreads();
+    const int c = t & 63, grp = t >> 6;
+    for (int r = 0; r < NB; ++r) {
+        double sum = 0;
+        for (int m = grp; m < r; m += 4) sum += A[r][m] * X[m][c];
+        part[grp][c] = sum;
+        __syncthreads();
+        if (t < NB) {
+            const double tot = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+            X[r][c] = c > r ? 0.0 : ((r == c ? 1.0 : 0.0) - tot) / A[r][r];
+        }
+        __syncthreads();
+    }
     double *Lk = Linv + (size_t)k * BB;
     for (int w = t; w < BB; w += TPB) {
-        const int r = w / NB, c = w % NB;
-        Dk[w] = c <= r ? A[r][c] : 0.0;
-        Lk[w] = X[r][c];
+        const int r = w / NB, cc = w % NB;
+        Dk[w] = cc <= r ? A[r][cc] : 0.0;
+        Lk[w] = X[r][cc];
     }
 }
 

@@ -272,10 +272,12 @@ __global__ void __launch_bounds__(TPB) k_l1_gs(Grid gc, Grid gf, const T *__rest
 }
 
 // ------------------------------------------------------------------ assembled block rows
-// Colour-major block storage (see BlockSlot). The neighbours are visited three at a time with the
+// Colour-major block storage (see BlockSlot). The neighbours are visited GROUP at a time with the
 // accumulators pinned after each group, which bounds how many of the 243 block entries and 81 u values
 // the compiler keeps in flight (and so the register count) while still overlapping memory latency.
-template <typename T, bool WANT_M>
+// Large levels use groups of 3 (occupancy); small levels, whose launches are latency-bound, groups of 9
+// (three dependent memory rounds per row instead of nine).
+template <typename T, bool WANT_M, int GROUP = 3>
 __device__ __forceinline__ void block_row(const Grid &g, const T *__restrict__ blocks, const BlockSlot &bs, int ix,
                                           int iy, int iz, const T *__restrict__ u, T f[3], T M[9]) {
     const int N = g.nnodes, sx = g.nn[1] * g.nn[2], sy = g.nn[2];
@@ -302,11 +304,34 @@ __device__ __forceinline__ void block_row(const Grid &g, const T *__restrict__ b
             }
             f[a] += acc;
         }
-        if (s % 3 == 2) {
+        if (s % GROUP == GROUP - 1) {
             asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]));
             __builtin_amdgcn_sched_barrier(0);
         }
     }
+}
+
+template <typename T, int MODE, int GROUP>
+__device__ __forceinline__ void block_apply_row(const Grid &g, const T *__restrict__ blocks, const T *__restrict__ u,
+                                                T *__restrict__ y, const T *__restrict__ b,
+                                                const uint8_t *__restrict__ dmask) {
+    const int nz = g.nn[2], slab = g.ydet * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= g.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
+    const int n = (ix * g.nn[1] + iy) * nz + iz;
+    const BlockSlot bs = block_slot(g, ix, iy, iz);
+    T f[3] = {0, 0, 0};
+    T M[9];
+    block_row<T, false, GROUP>(g, blocks, bs, ix, iy, iz, u, f, M);
+    store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_block_apply_g9(Grid g, const T *__restrict__ blocks, const T *__restrict__ u,
+                                                        T *__restrict__ y, const T *__restrict__ b,
+                                                        const uint8_t *__restrict__ dmask) {
+    block_apply_row<T, MODE, 9>(g, blocks, u, y, b, dmask);
 }
 
 template <typename T, int MODE>
@@ -325,7 +350,7 @@ __global__ void __launch_bounds__(TPB) k_block_apply(Grid g, const T *__restrict
     store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
-template <typename T>
+template <typename T, int GROUP = 3>
 __global__ void __launch_bounds__(TPB) k_block_gs(Grid g, const T *__restrict__ blocks, T *__restrict__ u,
                                                   const T *__restrict__ b, const uint8_t *__restrict__ dmask, int cx,
                                                   int cy, int cz, int forward) {
@@ -340,7 +365,7 @@ __global__ void __launch_bounds__(TPB) k_block_gs(Grid g, const T *__restrict__ 
     const BlockSlot bs = block_slot(g, ix, iy, iz);
     T f[3] = {0, 0, 0};
     T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    block_row<T, true>(g, blocks, bs, ix, iy, iz, u, f, M);
+    block_row<T, true, GROUP>(g, blocks, bs, ix, iy, iz, u, f, M);
     const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
     gs_update(u, N, n, bms, M, dm, forward);
 }
@@ -495,11 +520,18 @@ void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef,
     VF_HIP(hipGetLastError());
 }
 
+// levels below this many nodes use the latency-oriented grouping of block_row
+constexpr int SMALL_LEVEL = 65536;
+
 template <typename T>
 void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
                         hipStream_t s) {
     const long count = (long)g.nn[0] * g.ydet * g.nn[2];
-    if (count > 0) { VF_DISPATCH_MODE(k_block_apply, T, mode, dim3(nblocks(count)), g, blocks, u, y, b, dmask); }
+    if (count > 0 && g.nnodes < SMALL_LEVEL) {
+        VF_DISPATCH_MODE(k_block_apply_g9, T, mode, dim3(nblocks(count)), g, blocks, u, y, b, dmask);
+    } else if (count > 0) {
+        VF_DISPATCH_MODE(k_block_apply, T, mode, dim3(nblocks(count)), g, blocks, u, y, b, dmask);
+    }
     if (mode == AM_SET && g.ydet < g.nn[1]) {
         const long cz = (long)g.nn[0] * (g.nn[1] - g.ydet) * g.nn[2];
         hipLaunchKernelGGL(k_zero_detached<T>, dim3(nblocks(cz)), dim3(TPB), 0, s, g, y);
@@ -548,8 +580,12 @@ void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uin
     const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
     const long count = color_count(g, cx, cy, cz);
     if (!count) return;
-    hipLaunchKernelGGL(k_block_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy, cz,
-                       forward);
+    if (g.nnodes < SMALL_LEVEL)
+        hipLaunchKernelGGL((k_block_gs<T, 9>), dim3(nblocks(count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy, cz,
+                           forward);
+    else
+        hipLaunchKernelGGL((k_block_gs<T, 3>), dim3(nblocks(count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy, cz,
+                           forward);
     VF_HIP(hipGetLastError());
 }
 

@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(256) k_gs_gather_f(Grid g, const T *__restrict
                                                      const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
                                                      int forward) {
     const int mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = xcd_remap(blockIdx.x, gridDim.x) * 256 + threadIdx.x;
     if (t >= mx * my * mz) return;
     const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
     const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;

@@ -102,9 +102,12 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
     __shared__ T su[2][3 * RN * RN];  // node planes, double-buffered
     __shared__ T sp[4][3 * NTW];      // per element column: its values at the four corners of the plane
     const int N = g.nnodes, nx = g.nn[0], ny = g.nn[1], nz = g.nn[2];
-    const int tz = blockIdx.x % ntZ, ty = blockIdx.x / ntZ;
+    const int tiles = gridDim.x;
+    const int logical = xcd_remap(blockIdx.x + blockIdx.y * tiles, tiles * gridDim.y);
+    const int tile = logical % tiles, chunk = logical / tiles;
+    const int tz = tile % ntZ, ty = tile / ntZ;
     const int sy = TN * ty - 1, sz = TN * tz - 1;  // first element (and staged node) of the tile
-    const int x0 = blockIdx.y * xchunk, x1 = min(x0 + xchunk, nx);
+    const int x0 = chunk * xchunk, x1 = min(x0 + xchunk, nx);
     const int t = threadIdx.x, ly = t / TE, lz = t - ly * TE;
     const int ey = sy + ly, ez = sz + lz;
     const bool elemOk = (unsigned)ey < (unsigned)g.ne[1] && (unsigned)ez < (unsigned)g.ne[2];
