@@ -103,6 +103,7 @@ def main():
     rho_value = 0.3
     tps = wl.configure(pv, ne, wl.CANTILEVER_BC, np.full(nel, rho_value))
     mg = tps.multigridSolver(levels)
+    mg.useGraphs = True  # steady-state PCG iterations replayed from a captured HIP graph
     sim_h, mg_h = tps._handle(), mg._handle()
     dev = torch.device("cuda:0")
     f = torch.zeros(3 * nn, dtype=torch.float64, device=dev)

@@ -45,6 +45,7 @@ def _solve(tps, levels, precision, tol=1e-8, nu=1, reps=2):
     from . import capi
 
     mg = tps.multigridSolver(levels)
+    mg.useGraphs = True
     nn = int(tps.numNodes())
     f = torch.zeros(3 * nn, dtype=torch.float64, device="cuda:0")
     capi.check(capi.lib().vf_sim_build_load(tps._handle(), f.data_ptr(), capi.VF_DEVICE))

@@ -563,7 +563,14 @@ struct vf_mg {
         DBuf<T> u(n), y(n), rhs(n);
         launch_fill<T>(n, u.data(), T(1e-3), s);
         launch_fill<T>(n, rhs.data(), T(1), s);
-        hipEvent_t a, b;
+        hipEvent_t a = nullptr, b = nullptr;
+        struct EventGuard {
+            hipEvent_t *e0, *e1;
+            ~EventGuard() {
+                if (*e0) (void)hipEventDestroy(*e0);
+                if (*e1) (void)hipEventDestroy(*e1);
+            }
+        } guard{&a, &b};  // the events live until every timing below is done
         VF_HIP(hipEventCreate(&a));
         VF_HIP(hipEventCreate(&b));
         auto best = [&](auto &&fn) {
@@ -600,13 +607,23 @@ struct vf_mg {
             for (int c = 0; c < 8; ++c)
                 launch_fine_gs_gather_f<T>(g, Fp<T>(), Ep<T>(), y.data(), rhs.data(), mask, c, 1, s);
         });
-        (void)hipEventDestroy(a);
-        (void)hipEventDestroy(b);
         int kbest = 0;
         for (int i = 1; i < 3; ++i)
             if (ku[i] < ku[kbest]) kbest = i;
         kuChoice[p] = kbest;
         gsChoice[p] = gsT <= gsG ? (gsT <= gsF ? 1 : 2) : (gsG <= gsF ? 0 : 2);
+        // level 1 residual: assembled block rows or the on-the-fly operator, whichever is faster here
+        l1OnTheFly[p] = false;
+        if (l1Blocks && L >= 2) {
+            const Grid &g1 = lv[1]->g;
+            const float tb = best([&] { launch_block_apply<T>(g1, blocksp<T>(1), u.data(), y.data(), rhs.data(), dm(1),
+                                                              AM_RESIDUAL, s); });
+            const float tf = best([&] { launch_l1_apply<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(),
+                                                           dm(1), AM_RESIDUAL, s); });
+            l1OnTheFly[p] = tf < tb;
+            tuneMs[p][6] = tb;
+            tuneMs[p][7] = tf;
+        }
         tuneMs[p][0] = ku[0];
         tuneMs[p][1] = ku[1];
         tuneMs[p][2] = sim->walsh ? ku[2] : 0.0f;
@@ -614,7 +631,8 @@ struct vf_mg {
         tuneMs[p][4] = gsT;
         tuneMs[p][5] = gsF;
     }
-    float tuneMs[2][6] = {{0, 0, 0, 0, 0, 0}, {0, 0, 0, 0, 0, 0}};
+    float tuneMs[2][8] = {};
+    bool l1OnTheFly[2] = {false, false};  // level-1 residual on the fly instead of from the block rows
 
     template <typename T>
     void apply(int l, const T *u, T *y, const T *b, int mode, bool useMask) {
@@ -623,7 +641,7 @@ struct vf_mg {
         if (l == 0 && fam == 2) launch_fine_apply_walsh<T>(lv[0]->g, Wp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 0 && fam == 1) launch_fine_apply_tiled<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
-        else if (l == 1 && !(l1Blocks && L >= 2))
+        else if (l == 1 && (!(l1Blocks && L >= 2) || l1OnTheFly[pidx<T>()]))
             launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l < L) launch_block_apply<T>(lv[l]->g, blocksp<T>(l), u, y, b, mask, mode, st());
         else {
@@ -1746,6 +1764,7 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
             m.useGraphs = useGraphs != 0;
             std::copy(mg->kuChoice, mg->kuChoice + 2, m.kuChoice);
             std::copy(mg->gsChoice, mg->gsChoice + 2, m.gsChoice);
+            std::copy(mg->l1OnTheFly, mg->l1OnTheFly + 2, m.l1OnTheFly);
             pipes[p].f.resize(n);
             pipes[p].u.resize(n);
         }

@@ -69,55 +69,59 @@ __global__ void k_band_pad(int n, int nb, int kb, double *B) {
     if (i < nb * NB) bandblk(B, kb, nb - 1, 0)[(i % NB) * NB + i % NB] = 1.0;
 }
 
-// ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1, one workgroup. flag != 0 on a non-positive pivot.
-// Right-looking column Cholesky in LDS, two barriers per column: every work-item reads the pivot and
-// the 64 leading ones scale column j; then the 16 x 16 work-item grid updates the trailing triangle.
-// The inverse is formed row by row: X[r][:] = (e_r - sum_{m<r} L[r][m] X[m][:]) / L[r][r], each row's
-// sum split over four groups of 64 work-items and combined in a fixed order.
-__global__ void __launch_bounds__(TPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
-                                                    int *__restrict__ flag) {
-    __shared__ double A[NB][NB + 1];
-    __shared__ double X[NB][NB + 1];
-    __shared__ double part[4][NB];
-    __shared__ double diag[NB];
-    const int t = threadIdx.x, tr = t >> 4, tc = t & 15;
+// ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1 by one wavefront (64 work-items), registers only.
+// Work-item r holds row r of the block (64 doubles). Column j: the pivot A[j][j] and the entries A[c][j]
+// the update needs are broadcast from their owners with readlane, so every FMA takes a uniform operand and
+// no barrier or LDS round trip sits on the 64-step dependency chain. The inverse X = L^-1 is then formed
+// column-parallel: work-item c holds column c of X and reads the (uniform) entries of L from the rows.
+// flag != 0 on a non-positive pivot.
+constexpr int PTPB = 64;
+__device__ __forceinline__ double bcast(double v, int lane) {
+    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffu), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+__global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
+                                                     int *__restrict__ flag) {
+    const int t = threadIdx.x;
     double *Dk = bandblk(B, kb, k, 0);
-    for (int w = t; w < BB; w += TPB) A[w / NB][w % NB] = Dk[w];
-    __syncthreads();
+    double row[NB];
+#pragma unroll
+    for (int c = 0; c < NB; ++c) row[c] = Dk[t * NB + c];
+    bool ok = true;
+#pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const double piv = A[j][j];
-        if (!(piv > 0)) {
-            if (t == 0) atomicOr(flag, 1);
-            return;  // uniform: every work-item read the same pivot
+        const double piv = bcast(row[j], j);
+        ok = ok && piv > 0;
+        const double ljj = sqrt(piv > 0 ? piv : 1.0);
+        if (t > j) row[j] *= 1.0 / ljj;
+        if (t == j) row[j] = ljj;
+#pragma unroll
+        for (int c = j + 1; c < NB; ++c) {
+            const double acj = bcast(row[j], c);  // A[c][j], scaled
+            if (t >= c) row[c] -= row[j] * acj;
         }
-        const double ljj = sqrt(piv);
-        if (t > j && t < NB) A[t][j] *= 1.0 / ljj;
-        if (t == 0) diag[j] = ljj;
-        __syncthreads();
-        for (int r = j + 1 + tr; r < NB; r += 16)
-            for (int c = j + 1 + tc; c <= r; c += 16) A[r][c] -= A[r][j] * A[c][j];
-        __syncthreads();
     }
-    if (t < NB) A[t][t] = diag[t];
-    __syncthreads();
-    const int c = t & 63, grp = t >> 6;
+    if (!ok) {
+        if (t == 0) atomicOr(flag, 1);
+        return;  // uniform: every work-item saw the same pivots
+    }
+    // X = L^-1: column c = t. X[r][c] = (e_rc - sum_{m<r} L[r][m] X[m][c]) / L[r][r]
+    double x[NB];
+#pragma unroll
     for (int r = 0; r < NB; ++r) {
         double sum = 0;
-        for (int m = grp; m < r; m += 4) sum += A[r][m] * X[m][c];
-        part[grp][c] = sum;
-        __syncthreads();
-        if (t < NB) {
-            const double tot = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
-            X[r][c] = c > r ? 0.0 : ((r == c ? 1.0 : 0.0) - tot) / A[r][r];
-        }
-        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < r; ++m) sum += bcast(row[m], r) * x[m];
+        const double lrr = bcast(row[r], r);
+        x[r] = t > r ? 0.0 : ((r == t ? 1.0 : 0.0) - sum) / lrr;
     }
     double *Lk = Linv + (size_t)k * BB;
-    for (int w = t; w < BB; w += TPB) {
-        const int r = w / NB, cc = w % NB;
-        Dk[w] = cc <= r ? A[r][cc] : 0.0;
-        Lk[w] = X[r][cc];
-    }
+#pragma unroll
+    for (int c = 0; c < NB; ++c) Dk[t * NB + c] = c <= t ? row[c] : 0.0;
+#pragma unroll
+    for (int r = 0; r < NB; ++r) Lk[r * NB + t] = x[r];
 }
 
 // C (+)= alpha * P * Q^T for 64 x 64 row-major blocks, 256 work-items with 4 x 4 register tiles.
@@ -317,7 +321,7 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
     hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B);
     if ((int)ld > n) hipLaunchKernelGGL(k_band_pad, dim3(1), dim3(NB), 0, s, n, nb, kb, B);
     for (int k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(TPB), 0, s, k, kb, B, Linv, flag);
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(PTPB), 0, s, k, kb, B, Linv, flag);
         const int m = std::min(kb, nb - 1 - k);
         if (m > 0) {
             hipLaunchKernelGGL(k_trsm_panel, dim3(m), dim3(TPB), 0, s, k, nb, kb, B, Linv);
