@@ -36,6 +36,18 @@ __device__ __forceinline__ int split_lo(int k, int n, int parts) { return (int)(
 template <typename T>
 using CT = __attribute__((address_space(4))) T;
 
+// raw buffer load: a byte offset at or past the buffer's size reads as zero
+template <typename T>
+__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    if constexpr (sizeof(T) == 8) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+        return __builtin_bit_cast(T, v);
+    } else {
+        const int v = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+        return __builtin_bit_cast(T, v);
+    }
+}
+
 // neighbour offsets s = 9 dx + 3 dy + dz (0..2 each, 1 = same coordinate) grouped by delta
 __device__ constexpr int kNeighbourOrder[27] = {13, 12, 14, 10, 16, 9, 11, 15, 17, 4, 22, 3, 5, 21,
                                                  23, 1, 7, 19, 25, 0, 2, 6, 8, 18, 20, 24, 26};
@@ -279,135 +291,451 @@ __device__ __forceinline__ void gs_point_update(T ud[3], const T bms[3], const T
     }
 }
 
-template <typename T, int TY, int TZ, int NT>
+// Colour position (0..3 inside an x-parity phase, in application order) of the in-plane neighbour
+// (dy, dz) (0..2, 1 = same coordinate) of a node at position p: its y parity differs when dy != 1 and
+// its z parity when dz != 1.
+template <bool FWD>
+__host__ __device__ constexpr int nbr_pos(int p, int dy, int dz) {
+    const int sub = (FWD ? p : 3 - p) ^ ((dy != 1) << 1) ^ (dz != 1);
+    return FWD ? sub : 3 - sub;
+}
+// in-plane neighbours (bit 3 dy + dz) whose colour is applied before position p (EARLIER) or not
+template <bool FWD, bool EARLIER>
+__host__ __device__ constexpr int nbr_mask(int p) {
+    int m = 0;
+    for (int dy = 0; dy < 3; ++dy)
+        for (int dz = 0; dz < 3; ++dz)
+            if ((nbr_pos<FWD>(p, dy, dz) < p) == EARLIER) m |= 1 << (3 * dy + dz);
+    return m;
+}
+
+// Sum over the elements of one node plane that contain the in-plane neighbour offset (dy, dz) of
+// X[j][k] (j, k = element offsets below the node), each term signed by the node's side along y (SY)
+// and z (SZ): element j contains offset dy when dy == 1, or dy == 0 and j == 1, or dy == 2 and j == 0;
+// the side is +1 for j == 1 and -1 for j == 0.
+template <typename T, bool SY, bool SZ>
+__device__ __forceinline__ T side_sum(const T X[2][2], int dy, int dz) {
+    T r = T(0);
+    bool first = true;
+#pragma unroll
+    for (int j = 1; j >= 0; --j) {
+        if (!(dy == 1 || (dy == 0 && j == 1) || (dy == 2 && j == 0))) continue;
+#pragma unroll
+        for (int k = 1; k >= 0; --k) {
+            if (!(dz == 1 || (dz == 0 && k == 1) || (dz == 2 && k == 0))) continue;
+            const T v = ((SY && j == 0) != (SZ && k == 0)) ? -X[j][k] : X[j][k];
+            r = first ? v : r + v;
+            first = false;
+        }
+    }
+    return r;
+}
+
+// Weighted neighbour sums of one x-plane of the separable-weight row (see w_row): for the in-plane
+// offsets of group g (bit 1: dy != 1, bit 0: dz != 1) selected by MASK (bit 3 dy + dz) add
+// W[c][d] u[d] to V[c][d]. Xp / Xs are the plane's element moduli combined along x without / with
+// the node's x side (for the diagonal and yz weights / for the xy and xz weights). p points at the
+// node in the plane (component stride CS, row stride RZ).
+template <typename T, int MASK>
+__device__ __forceinline__ void plane_sums(const T *p, int CS, int RZ, const T Xp[2][2], const T Xs[2][2], int g,
+                                           T V[9]) {
+#pragma unroll
+    for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int dz = 0; dz < 3; ++dz) {
+            if ((((dy != 1) << 1) | (dz != 1)) != g || !((MASK >> (3 * dy + dz)) & 1)) continue;
+            const T w = side_sum<T, false, false>(Xp, dy, dz);
+            const T wxy = side_sum<T, true, false>(Xs, dy, dz);
+            const T wxz = side_sum<T, false, true>(Xs, dy, dz);
+            const T wyz = side_sum<T, true, true>(Xp, dy, dz);
+            const int off = (dy - 1) * RZ + (dz - 1);
+            const T u0 = p[off], u1 = p[CS + off], u2 = p[2 * CS + off];
+            V[0] += w * u0;
+            V[4] += w * u1;
+            V[8] += w * u2;
+            V[1] += wxy * u1;
+            V[3] += wxy * u0;
+            V[2] += wxz * u2;
+            V[6] += wxz * u0;
+            V[5] += wyz * u2;
+            V[7] += wyz * u1;
+        }
+}
+
+template <typename T>
+__device__ __forceinline__ void contract_group(const T *__restrict__ F, int delta, const T V[9], T f[3]) {
+    const T *Fg = F + 16 * delta;
+    asm volatile("" : "+s"(Fg));
+    const CT<T> *Fp = (const CT<T> *)Fg;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] += Fp[3 * c] * V[3 * c] + Fp[3 * c + 1] * V[3 * c + 1] + Fp[3 * c + 2] * V[3 * c + 2];
+}
+
+// f += the in-plane part of a node row over the neighbours selected by MASK (X = E0 + E1, Xs = E1 - E0
+// over the two element layers), one delta group at a time.
+template <typename T, int MASK>
+__device__ __forceinline__ void in_plane_part(const T *__restrict__ F, const T *p, int CS, int RZ, const T X[2][2],
+                                              const T Xs[2][2], T f[3]) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        constexpr int kGroupMask[4] = {1 << 4, (1 << 3) | (1 << 5), (1 << 1) | (1 << 7), 1 | 4 | 64 | 256};
+        if (!(MASK & kGroupMask[g])) continue;
+        T V[9];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) V[q] = T(0);
+        plane_sums<T, MASK>(p, CS, RZ, X, Xs, g, V);
+        contract_group(F, g, V, f);
+        asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]));
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
+// The value of v, hidden from the optimiser: arithmetic derived from it is recomputed where it is used
+// instead of being hoisted out of the plane loop and kept live in registers.
+__device__ __forceinline__ int opaque(int v) {
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// One x-parity of the 8-colour block Gauss-Seidel sweep (MultigridSolver.hh:408-458). A workgroup owns
+// a (y, z) tile of every plane of that parity in its x-chunk and steps through them. Per plane:
+//  1. the loads of the next plane step (two u planes, two element planes) are issued into registers,
+//     so their latency overlaps steps 2-4;
+//  2. for every node the four colours will update (up to three per work-item) the row part that the
+//     colour steps do not change is evaluated in parallel and kept in registers as b - (K u)_fixed: the
+//     two neighbour planes (not touched by this phase), the node itself, and the in-plane neighbours
+//     whose colour comes later;
+//  3. the four in-plane colours are applied in order; a colour adds only its in-plane neighbours of
+//     earlier colours (0, 2, 6 and 8 of them), solves the node's 3x3 block and updates the LDS plane;
+//  4. the tile's own nodes are written out and the prefetched registers are stored into LDS.
+// Rows use the separable-weight form of w_row. Halo values outside the grid are zero; the tile loads
+// go through buffer loads whose offsets are precomputed once per work-item (out-of-grid entries get an
+// out-of-range offset, which the buffer unit returns as zero).
+template <typename T, int TY, int TZ, int NT, bool FWD>
 __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F, const T *__restrict__ E,
                                                  const T *__restrict__ uin, const T *__restrict__ unb,
                                                  T *__restrict__ uout, const T *__restrict__ b,
-                                                 const uint8_t *__restrict__ dmask, int parity, int forward,
-                                                 int ntY, int ntZ, int pchunk) {
+                                                 const uint8_t *__restrict__ dmask, int parity, int ntY, int ntZ,
+                                                 int pchunk) {
     constexpr int HY = 2, HZ = 4;                          // u halo: colour halo + one neighbour
     constexpr int RYM = TY + 2 * HY, RZM = TZ + 2 * HZ;
     constexpr int EYM = TY + 3, EZM = TZ + 7;              // element rows [y0-2, y1], cols [z0-4, z1+2]
-    constexpr int CS = RYM * RZM;
-    static_assert((TY / 2 + 1) * (TZ / 2 + 3) <= NT, "one node per work-item and colour position");
-    __shared__ T su[3][3 * CS];  // rolling planes: ix-1, ix (updated in place), ix+1
-    __shared__ T sE[2][EYM * EZM];
+    constexpr int CS = RYM * RZM, ES = EYM * EZM;
+    constexpr int NU = (3 * CS + NT - 1) / NT, NE = (ES + NT - 1) / NT;
+    constexpr int MAXN = (TY / 2 + 1) * (TZ / 2 + 3) + (TY / 2 + 1) * (TZ / 2 + 2) + (TY / 2) * (TZ / 2 + 1) +
+                         (TY / 2) * (TZ / 2);
+    constexpr int OOB = 0x7fffffff;  // byte offset past every buffer (their sizes are below 2^31 - 2^30)
+    static_assert((TY / 2 + 1) * (TZ / 2 + 3) <= NT, "at most one node per work-item and colour position");
+    static_assert(MAXN <= 3 * NT, "at most three nodes per work-item and plane");
+    static_assert(TY % 2 == 0 && TZ % 2 == 0, "even tile extents");
+    static_assert(RYM < 256 && RZM < 256, "slot packing");
+    __shared__ T su[3][3 * CS];  // planes ix-1, ix+1 (rotating) and ix (updated in place)
+    __shared__ T sE[2][ES];      // element planes ix-1 (i = 1) and ix (i = 0)
     const int N = g.nnodes, nx = g.nn[0], ny = g.nn[1], nz = g.nn[2];
     const int tz = blockIdx.x % ntZ, ty = blockIdx.x / ntZ;
     const int y0 = split_lo(ty, ny, ntY), y1 = split_lo(ty + 1, ny, ntY);
     const int z0 = split_lo(tz, nz, ntZ), z1 = split_lo(tz + 1, nz, ntZ);
     const int ry0 = y0 - HY, rz0 = z0 - HZ;
-    const int RY = y1 - y0 + 2 * HY, RZ = z1 - z0 + 2 * HZ;
-    const int ey0 = y0 - 2, ez0 = z0 - 4, EY = y1 - y0 + 3, EZ = z1 - z0 + 7;
+    const int ey0 = y0 - 2, ez0 = z0 - 4;
     const int t = threadIdx.x;
     const int nplanes = (nx - parity + 1) / 2;
     const int k0 = blockIdx.y * pchunk, k1 = min(k0 + pchunk, nplanes);
+    if (k0 >= k1) return;  // uniform over the workgroup
 
-    // loads iterate over the full-size tile (compile-time divisors) and skip what a partial tile lacks
-    auto load_u = [&](const T *src, int ix, T *dst) {
-        for (int w = t; w < 3 * CS; w += NT) {
-            const int c = w / CS, r = w - c * CS, ly = r / RZM, lz = r - ly * RZM;
-            if (ly >= RY || lz >= RZ) continue;
-            const int gy = ry0 + ly, gz = rz0 + lz;
-            T v = T(0);
-            if ((unsigned)ix < (unsigned)nx && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz)
-                v = src[(size_t)c * N + ((size_t)ix * ny + gy) * nz + gz];
-            dst[c * CS + ly * RZM + lz] = v;
+    // Node lists of the four colour positions (the tile grown by the halo the later colours read:
+    // (1,3), (1,2), (0,1), (0,0) in (y,z)), clipped to the active rows; concatenated in position order.
+    // Work-item t owns list entries t, t + NT, t + 2 NT (slots 0..2) for the whole launch.
+    int base[5], ya[4], za[4], ncol[4];
+    base[0] = 0;
+#pragma unroll
+    for (int pos = 0; pos < 4; ++pos) {
+        const int sub = FWD ? pos : 3 - pos;
+        const int cy = (sub >> 1) & 1, cz = sub & 1;
+        const int hy = pos < 2 ? 1 : 0, hz = 3 - pos;
+        int a = max(y0 - hy, 0), c = max(z0 - hz, 0);
+        const int yb = min(y1 + hy, g.ydet), zb = min(z1 + hz, nz);
+        a += (a & 1) != cy;
+        c += (c & 1) != cz;
+        const int nr = yb > a ? (yb - a + 1) / 2 : 0, nc = zb > c ? (zb - c + 1) / 2 : 0;
+        ya[pos] = a;
+        za[pos] = c;
+        ncol[pos] = max(nc, 1);
+        base[pos + 1] = base[pos] + nr * nc;
+    }
+    // slot r: (colour position + 1) << 16 | ly << 8 | lz in the LDS plane (0: no node); soff[r]: the
+    // node's index inside its plane
+    int slot[3], soff[3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        const int q = r * NT + t;
+        int pos = -1;
+#pragma unroll
+        for (int p = 0; p < 4; ++p)
+            if (q >= base[p] && q < base[p + 1]) pos = p;
+        slot[r] = 0;
+        soff[r] = 0;
+        if (pos >= 0) {
+            int j = q, a = 0, c = 0, nc = 1;
+#pragma unroll
+            for (int p = 0; p < 4; ++p)
+                if (pos == p) {
+                    j = q - base[p];
+                    a = ya[p];
+                    c = za[p];
+                    nc = ncol[p];
+                }
+            const int wr = j / nc;
+            const int iy = a + 2 * wr, iz = c + 2 * (j - wr * nc);
+            slot[r] = (pos + 1) << 16 | (iy - ry0) << 8 | (iz - rz0);
+            soff[r] = iy * nz + iz;
+        }
+    }
+    auto s_pos = [](int sl) { return (sl >> 16) - 1; };
+    auto s_ly = [](int sl) { return (sl >> 8) & 255; };
+    auto s_lz = [](int sl) { return sl & 255; };
+
+    // per-work-item byte offsets of its tile entries inside one plane (OOB outside the grid)
+    int uo[NU], eo[NE];
+#pragma unroll
+    for (int i = 0; i < NU; ++i) {
+        const int w = t + i * NT;
+        const int c = w / CS, r = w - c * CS, ly = r / RZM, lz = r - ly * RZM;
+        const int gy = ry0 + ly, gz = rz0 + lz;
+        const bool ok = w < 3 * CS && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
+        uo[i] = ok ? (c * N + gy * nz + gz) * (int)sizeof(T) : OOB;
+    }
+#pragma unroll
+    for (int i = 0; i < NE; ++i) {
+        const int w = t + i * NT;
+        const int ly = w / EZM, lz = w - ly * EZM;
+        const int gy = ey0 + ly, gz = ez0 + lz;
+        const bool ok = w < ES && (unsigned)gy < (unsigned)g.ne[1] && (unsigned)gz < (unsigned)g.ne[2];
+        eo[i] = ok ? (gy * g.ne[2] + gz) * (int)sizeof(T) : OOB;
+    }
+    const __amdgpu_buffer_rsrc_t rin =
+        __builtin_amdgcn_make_buffer_rsrc((void *)uin, 0, (int)((size_t)3 * N * sizeof(T)), 0x00020000);
+    const __amdgpu_buffer_rsrc_t rnb =
+        __builtin_amdgcn_make_buffer_rsrc((void *)unb, 0, (int)((size_t)3 * N * sizeof(T)), 0x00020000);
+    const int NEL = g.ne[0] * g.ne[1] * g.ne[2];
+    const __amdgpu_buffer_rsrc_t rE =
+        __builtin_amdgcn_make_buffer_rsrc((void *)E, 0, (int)((size_t)NEL * sizeof(T)), 0x00020000);
+    const int planeB = ny * nz * (int)sizeof(T), eplaneB = g.ne[1] * g.ne[2] * (int)sizeof(T);
+
+    auto fetch_u = [&](__amdgpu_buffer_rsrc_t rs, int ix, T (&v)[NU]) {
+        if ((unsigned)ix < (unsigned)nx) {
+            const int pb = ix * planeB;
+#pragma unroll
+            for (int i = 0; i < NU; ++i) v[i] = buffer_load<T>(rs, (int)((unsigned)uo[i] + (unsigned)pb), 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NU; ++i) v[i] = T(0);
         }
     };
-    auto load_E = [&](int ex, T *dst) {
-        for (int w = t; w < EYM * EZM; w += NT) {
-            const int ly = w / EZM, lz = w - ly * EZM;
-            if (ly >= EY || lz >= EZ) continue;
-            const int ey = ey0 + ly, ez = ez0 + lz;
-            T v = T(0);
-            if ((unsigned)ex < (unsigned)g.ne[0] && (unsigned)ey < (unsigned)g.ne[1] && (unsigned)ez < (unsigned)g.ne[2])
-                v = E[((size_t)ex * g.ne[1] + ey) * g.ne[2] + ez];
-            dst[ly * EZM + lz] = v;
+    auto store_u = [&](const T (&v)[NU], T *dst) {
+        const int tl = opaque(t);
+#pragma unroll
+        for (int i = 0; i < NU; ++i) {
+            const int w = tl + i * NT;
+            if ((i + 1) * NT <= 3 * CS || w < 3 * CS) dst[w] = v[i];
+        }
+    };
+    auto fetch_E = [&](int ex, T (&v)[NE]) {
+        if ((unsigned)ex < (unsigned)g.ne[0]) {
+            const int pb = ex * eplaneB;
+#pragma unroll
+            for (int i = 0; i < NE; ++i) v[i] = buffer_load<T>(rE, (int)((unsigned)eo[i] + (unsigned)pb), 0);
+        } else {
+#pragma unroll
+            for (int i = 0; i < NE; ++i) v[i] = T(0);
+        }
+    };
+    auto store_E = [&](const T (&v)[NE], T *dst) {
+        const int tl = opaque(t);
+#pragma unroll
+        for (int i = 0; i < NE; ++i) {
+            const int w = tl + i * NT;
+            if ((i + 1) * NT <= ES || w < ES) dst[w] = v[i];
         }
     };
 
-    // slots: neighbour planes ix-1 / ix+1 rotate through slots A/B (plane ix+1 of one step is plane
-    // ix-1 of the next), slot C holds the plane being updated
     int sA = 0, sB = 1;
     const int sC = 2;
-    if (k0 < k1) load_u(unb, parity + 2 * k0 - 1, su[sA]);
+    {
+        const int ix = parity + 2 * k0;
+        T v[NU];
+        T ev[NE];
+        fetch_u(rnb, ix - 1, v);
+        store_u(v, su[sA]);
+        fetch_u(rin, ix, v);
+        store_u(v, su[sC]);
+        fetch_u(rnb, ix + 1, v);
+        store_u(v, su[sB]);
+        fetch_E(ix - 1, ev);
+        store_E(ev, sE[0]);
+        fetch_E(ix, ev);
+        store_E(ev, sE[1]);
+    }
+    __syncthreads();
+
     for (int kp = k0; kp < k1; ++kp) {
         const int ix = parity + 2 * kp;
-        // Each colour position updates at most one node per work-item (the host checks the node count
-        // against NT). The right-hand side and Dirichlet mask of the four nodes are requested before the
-        // tile loads so that their latency overlaps them.
-        int pli[4], piy[4], piz[4];
-        uint8_t pdm[4];
-        T pb[4][3];
+        const bool more = kp + 1 < k1;
+        // 1. next plane step's inputs into registers
+        T pc[NU], pb[NU], pe0[NE], pe1[NE];
+        if (more) {
+            fetch_u(rin, ix + 2, pc);
+            fetch_u(rnb, ix + 3, pb);
+            fetch_E(ix + 1, pe0);
+            fetch_E(ix + 2, pe1);
+        }
+        // 2. b - (K u)_fixed for each owned node; its right-hand side and mask loads overlap the first
+        // node's arithmetic
+        T og[3][3];
+        uint8_t dcur[3];
+        const int nplane = ix * ny * nz;
 #pragma unroll
-        for (int pos = 0; pos < 4; ++pos) {
-            const int sub = forward ? pos : 3 - pos;
-            const int cy = (sub >> 1) & 1, cz = sub & 1;
-            const int hy = pos < 2 ? 1 : 0, hz = 3 - pos;
-            // nodes of colour (cy, cz) in [y0-hy, y1+hy) x [z0-hz, z1+hz), clipped to the active rows
-            int ya = max(y0 - hy, 0), za = max(z0 - hz, 0);
-            const int yb = min(y1 + hy, g.ydet), zb = min(z1 + hz, nz);
-            ya += (ya & 1) != cy;
-            za += (za & 1) != cz;
-            const int nr = yb > ya ? (yb - ya + 1) / 2 : 0, nc = zb > za ? (zb - za + 1) / 2 : 0;
-            pdm[pos] = 7;
-            pli[pos] = 0;
-            piy[pos] = piz[pos] = 0;
-            pb[pos][0] = pb[pos][1] = pb[pos][2] = T(0);
-            if (t < nr * nc) {
-                const int wr = t / nc;
-                const int iy = ya + 2 * wr, iz = za + 2 * (t - wr * nc);
-                const int n = ((ix * ny) + iy) * nz + iz;
-                piy[pos] = iy;
-                piz[pos] = iz;
-                pli[pos] = (iy - ry0) * RZM + (iz - rz0);
-                pdm[pos] = dmask ? dmask[n] : uint8_t(0);
-                pb[pos][0] = b[n];
-                pb[pos][1] = b[(size_t)N + n];
-                pb[pos][2] = b[(size_t)2 * N + n];
+        for (int r = 0; r < 3; ++r) {
+            dcur[r] = 7;
+            og[r][0] = og[r][1] = og[r][2] = T(0);
+            if (slot[r]) {
+                const int n = nplane + soff[r];
+                dcur[r] = dmask ? dmask[n] : uint8_t(0);
+                og[r][0] = b[n];
+                og[r][1] = b[(size_t)N + n];
+                og[r][2] = b[(size_t)2 * N + n];
             }
         }
-        load_u(uin, ix, su[sC]);
-        load_u(unb, ix + 1, su[sB]);
-        load_E(ix - 1, sE[0]);
-        load_E(ix, sE[1]);
-        __syncthreads();
         const T *pm = su[sA], *pp = su[sB];
-        T *pc = su[sC];
+        T *pcur = su[sC];
+        auto fixed_part = [&](auto R) {
+            constexpr int r = decltype(R)::value;
+            if (slot[r] && dcur[r] != 7) {
+                const int ly = s_ly(slot[r]), lz = s_lz(slot[r]);
+                const int li = ly * RZM + lz;
+                const int ei = (ly + HY - 2) * EZM + (lz + HZ - 4);
+                T e1[2][2], e0[2][2], e0n[2][2];
 #pragma unroll
-        for (int pos = 0; pos < 4; ++pos) {
-            if (pdm[pos] != 7) {
-                const int iy = piy[pos], iz = piz[pos], li = pli[pos];
-                T Ee[8];
+                for (int j = 0; j < 2; ++j)
 #pragma unroll
-                for (int e = 0; e < 8; ++e) {
-                    const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
-                    Ee[e] = sE[1 - i][(iy - j - ey0) * EZM + (iz - k - ez0)];
-                }
+                    for (int k = 0; k < 2; ++k) {
+                        e1[j][k] = sE[0][ei - j * EZM - k];  // x - 1 plane: elements i = 1, side +1
+                        e0[j][k] = sE[1][ei - j * EZM - k];  // x + 1 plane: elements i = 0, side -1
+                        e0n[j][k] = -e0[j][k];
+                    }
                 T f[3] = {0, 0, 0};
-                T M[9];
-                lds_row<T, true>(F, pm + li, pc + li, pp + li, CS, RZM, Ee, f, M);
-                const T bms[3] = {pb[pos][0] - f[0], pb[pos][1] - f[1], pb[pos][2] - f[2]};
-                T ud[3];
-                gs_point_update(ud, bms, M, pdm[pos], forward);
-                pc[li] += ud[0];
-                pc[CS + li] += ud[1];
-                pc[2 * CS + li] += ud[2];
+#pragma unroll
+                for (int gg = 0; gg < 4; ++gg) {
+                    T V[9];
+#pragma unroll
+                    for (int q = 0; q < 9; ++q) V[q] = T(0);
+                    plane_sums<T, 511>(pm + li, CS, RZM, e1, e1, gg, V);
+                    plane_sums<T, 511>(pp + li, CS, RZM, e0, e0n, gg, V);
+                    contract_group(F, 4 | gg, V, f);
+                    // one group at a time: keeps the neighbour values of later groups out of registers
+                    asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]));
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                T X[2][2], Xs[2][2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        X[j][k] = e0[j][k] + e1[j][k];
+                        Xs[j][k] = e1[j][k] - e0[j][k];
+                    }
+                switch (s_pos(slot[r])) {
+                    case 0: in_plane_part<T, nbr_mask<FWD, false>(0)>(F, pcur + li, CS, RZM, X, Xs, f); break;
+                    case 1: in_plane_part<T, nbr_mask<FWD, false>(1)>(F, pcur + li, CS, RZM, X, Xs, f); break;
+                    case 2: in_plane_part<T, nbr_mask<FWD, false>(2)>(F, pcur + li, CS, RZM, X, Xs, f); break;
+                    default: in_plane_part<T, nbr_mask<FWD, false>(3)>(F, pcur + li, CS, RZM, X, Xs, f); break;
+                }
+                og[r][0] -= f[0];
+                og[r][1] -= f[1];
+                og[r][2] -= f[2];
             }
+            asm volatile("" : "+v"(og[r][0]), "+v"(og[r][1]), "+v"(og[r][2]));
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        fixed_part(std::integral_constant<int, 0>());
+        fixed_part(std::integral_constant<int, 1>());
+        fixed_part(std::integral_constant<int, 2>());
+        // 3. the four in-plane colours
+        // each slot is tested separately (at most one holds a node of this colour) so that every
+        // register array keeps constant indices
+        auto colour_slot = [&](auto P, auto R) {
+            constexpr int pos = decltype(P)::value, r = decltype(R)::value;
+            if (s_pos(slot[r]) == pos && dcur[r] != 7) {
+                const uint8_t dm = dcur[r];
+                const int ly = s_ly(slot[r]), lz = s_lz(slot[r]);
+                const T o[3] = {og[r][0], og[r][1], og[r][2]};
+                const int li = ly * RZM + lz;
+                const int ei = (ly + HY - 2) * EZM + (lz + HZ - 4);
+                T X[2][2], Xs[2][2];
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int k = 0; k < 2; ++k) {
+                        const T e1 = sE[0][ei - j * EZM - k], e0 = sE[1][ei - j * EZM - k];
+                        X[j][k] = e0 + e1;
+                        Xs[j][k] = e1 - e0;
+                    }
+                T f[3] = {0, 0, 0};
+                constexpr int mask = nbr_mask<FWD, true>(pos);
+                if constexpr (mask != 0) in_plane_part<T, mask>(F, pcur + li, CS, RZM, X, Xs, f);
+                T M[9];
+                {
+                    const T w = side_sum<T, false, false>(X, 1, 1), wxy = side_sum<T, true, false>(Xs, 1, 1);
+                    const T wxz = side_sum<T, false, true>(Xs, 1, 1), wyz = side_sum<T, true, true>(X, 1, 1);
+                    const T *Fg = F;
+                    asm volatile("" : "+s"(Fg));
+                    const CT<T> *F0 = (const CT<T> *)Fg;
+                    M[0] = F0[0] * w;
+                    M[4] = F0[4] * w;
+                    M[8] = F0[8] * w;
+                    M[1] = F0[1] * wxy;
+                    M[3] = F0[3] * wxy;
+                    M[2] = F0[2] * wxz;
+                    M[6] = F0[6] * wxz;
+                    M[5] = F0[5] * wyz;
+                    M[7] = F0[7] * wyz;
+                }
+                const T bms[3] = {o[0] - f[0], o[1] - f[1], o[2] - f[2]};
+                T ud[3];
+                gs_point_update(ud, bms, M, dm, FWD ? 1 : 0);
+                pcur[li] += ud[0];
+                pcur[CS + li] += ud[1];
+                pcur[2 * CS + li] += ud[2];
+            }
+        };
+        auto colour = [&](auto P) {
+            colour_slot(P, std::integral_constant<int, 0>());
+            colour_slot(P, std::integral_constant<int, 1>());
+            colour_slot(P, std::integral_constant<int, 2>());
             __syncthreads();
-        }
-        // the tile of plane ix, updated or not
+        };
+        colour(std::integral_constant<int, 0>());
+        colour(std::integral_constant<int, 1>());
+        colour(std::integral_constant<int, 2>());
+        colour(std::integral_constant<int, 3>());
+        // 4. the tile of plane ix, updated or not
         const int OY = y1 - y0, OZ = z1 - z0;
         for (int w = t; w < 3 * TY * TZ; w += NT) {
             const int c = w / (TY * TZ), r = w - c * TY * TZ, oy = r / TZ, oz = r - oy * TZ;
             if (oy >= OY || oz >= OZ) continue;
-            uout[(size_t)c * N + ((size_t)ix * ny + y0 + oy) * nz + z0 + oz] = pc[c * CS + (oy + HY) * RZM + oz + HZ];
+            uout[(size_t)c * N + ((size_t)ix * ny + y0 + oy) * nz + z0 + oz] = pcur[c * CS + (oy + HY) * RZM + oz + HZ];
         }
-        const int tmp = sA;  // plane ix+1 becomes the next step's plane ix'-1
-        sA = sB;
-        sB = tmp;
-        __syncthreads();
+        if (more) {
+            __syncthreads();
+            // plane ix+1 becomes the next step's plane ix'-1; the freed slot takes plane ix'+1
+            const int tmp = sA;
+            sA = sB;
+            sB = tmp;
+            store_u(pc, su[sC]);
+            store_u(pb, su[sB]);
+            store_E(pe0, sE[0]);
+            store_E(pe1, sE[1]);
+            __syncthreads();
+        }
     }
 }
 
@@ -480,16 +808,119 @@ __device__ __forceinline__ void reg_row(const T *__restrict__ F, const T un[27][
     }
 }
 
-template <typename T>
-__device__ __forceinline__ T buffer_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    if constexpr (sizeof(T) == 8) {
-        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
-        return __builtin_bit_cast(T, v);
-    } else {
-        const int v = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
-        return __builtin_bit_cast(T, v);
+// Row of a node in separable-weight form (same inputs and outputs as reg_row). With e = 4i+2j+k the
+// element at node - (i,j,k) and s a neighbour at offset o, the structured element matrix gives
+//   f[c] = sum_s sum_d F[delta(s)][c][d] W_s[c][d] u_s[d],
+//   W_s[c][c] = sum_{e contains s} E_e,   W_s[c][d] = sum_{e contains s} E_e s_c(e) s_d(e)  (c != d).
+// Element e contains s iff, per axis a, o_a = 0, or o_a = -1 and i_a = 1, or o_a = +1 and i_a = 0, so the
+// four weight sets are separable 2 -> 3 transforms of the 2x2x2 moduli, per axis either "plain"
+// (E1, E0 + E1, E0) or "signed" (E1, E1 - E0, -E0) for o_a = -1, 0, +1. Neighbours are taken plane by
+// plane (x offset -1, +1, 0) and, inside a plane, in groups of equal delta; each group's weighted sums
+// V[c][d] are contracted with its nine F values. About 470 operations per row instead of 8 x 72 products.
+template <typename T, bool WANT_M>
+__device__ __forceinline__ void w_row(const T *__restrict__ F, const T un[27][3], const T Ee[8], T f[3], T M[9]) {
+    // z transforms per (i, j), index oz + 1
+    T zp[2][2][3], zs[2][2][3];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const T e0 = Ee[4 * i + 2 * j], e1 = Ee[4 * i + 2 * j + 1];
+            zp[i][j][0] = e1;
+            zp[i][j][1] = e0 + e1;
+            zp[i][j][2] = e0;
+            zs[i][j][0] = e1;
+            zs[i][j][1] = e1 - e0;
+            zs[i][j][2] = -e0;
+        }
+    // y transforms, index [i][oy + 1][oz + 1]: P plain(zp) for the diagonal, A signed(zp) for xy,
+    // B plain(zs) for xz, C signed(zs) for yz
+    T P[2][3][3], A[2][3][3], B[2][3][3], C[2][3][3];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int oz = 0; oz < 3; ++oz) {
+            P[i][0][oz] = zp[i][1][oz];
+            P[i][1][oz] = zp[i][0][oz] + zp[i][1][oz];
+            P[i][2][oz] = zp[i][0][oz];
+            A[i][0][oz] = zp[i][1][oz];
+            A[i][1][oz] = zp[i][1][oz] - zp[i][0][oz];
+            A[i][2][oz] = -zp[i][0][oz];
+            B[i][0][oz] = zs[i][1][oz];
+            B[i][1][oz] = zs[i][0][oz] + zs[i][1][oz];
+            B[i][2][oz] = zs[i][0][oz];
+            C[i][0][oz] = zs[i][1][oz];
+            C[i][1][oz] = zs[i][1][oz] - zs[i][0][oz];
+            C[i][2][oz] = -zs[i][0][oz];
+        }
+    constexpr int kPlane[3] = {0, 2, 1};
+#pragma unroll
+    for (int pi = 0; pi < 3; ++pi) {
+        const int dx = kPlane[pi];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            T V[9];
+#pragma unroll
+            for (int q = 0; q < 9; ++q) V[q] = T(0);
+#pragma unroll
+            for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+                for (int dz = 0; dz < 3; ++dz) {
+                    if ((((dy != 1) << 1) | (dz != 1)) != g) continue;
+                    T p, qxy, qxz, qyz;
+                    if (dx == 0) {  // x offset -1: elements i = 1, s_x = +1
+                        p = P[1][dy][dz];
+                        qxy = A[1][dy][dz];
+                        qxz = B[1][dy][dz];
+                        qyz = C[1][dy][dz];
+                    } else if (dx == 2) {  // x offset +1: elements i = 0, s_x = -1
+                        p = P[0][dy][dz];
+                        qxy = -A[0][dy][dz];
+                        qxz = -B[0][dy][dz];
+                        qyz = C[0][dy][dz];
+                    } else {
+                        p = P[0][dy][dz] + P[1][dy][dz];
+                        qxy = A[1][dy][dz] - A[0][dy][dz];
+                        qxz = B[1][dy][dz] - B[0][dy][dz];
+                        qyz = C[0][dy][dz] + C[1][dy][dz];
+                    }
+                    const int s = 9 * dx + 3 * dy + dz;
+                    const T u0 = un[s][0], u1 = un[s][1], u2 = un[s][2];
+                    V[0] += p * u0;
+                    V[4] += p * u1;
+                    V[8] += p * u2;
+                    V[1] += qxy * u1;
+                    V[3] += qxy * u0;
+                    V[2] += qxz * u2;
+                    V[6] += qxz * u0;
+                    V[5] += qyz * u2;
+                    V[7] += qyz * u1;
+                    if (WANT_M && s == 13) {
+                        // the node's own 3x3 block: F[0] with the self weights
+                        const T *Fg = F;
+                        asm volatile("" : "+s"(Fg));
+                        const CT<T> *F0 = (const CT<T> *)Fg;
+                        M[0] = F0[0] * p;
+                        M[4] = F0[4] * p;
+                        M[8] = F0[8] * p;
+                        M[1] = F0[1] * qxy;
+                        M[3] = F0[3] * qxy;
+                        M[2] = F0[2] * qxz;
+                        M[6] = F0[6] * qxz;
+                        M[5] = F0[5] * qyz;
+                        M[7] = F0[7] * qyz;
+                    }
+                }
+            const int delta = ((dx != 1) << 2) | g;
+            const T *Fg = F + 16 * delta;
+            asm volatile("" : "+s"(Fg));
+            const CT<T> *Fp = (const CT<T> *)Fg;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) f[c] += Fp[3 * c] * V[3 * c] + Fp[3 * c + 1] * V[3 * c + 1] + Fp[3 * c + 2] * V[3 * c + 2];
+        }
     }
 }
+
 
 // One colour of the 8-colour block Gauss-Seidel sweep (MultigridSolver.hh:408-458), one node per work-item.
 // Interior nodes load their 81 neighbour values with buffer loads that share one per-lane offset (the
@@ -543,7 +974,7 @@ __global__ void __launch_bounds__(256) k_gs_gather_f(Grid g, const T *__restrict
     }
     T f[3] = {0, 0, 0};
     T M[9];
-    reg_row<T, true>(F, un, Ee, f, M);
+    w_row<T, true>(F, un, Ee, f, M);
     const T bms[3] = {b[n] - f[0], b[(size_t)N + n] - f[1], b[(size_t)2 * N + n] - f[2]};
     T ud[3];
     gs_point_update(ud, bms, M, dm, forward);
@@ -569,8 +1000,8 @@ int cu_count() {
 
 // fine K u tile: 16 x 16 outputs, 256 work-items
 constexpr int KU_TY = 16, KU_TZ = 16, KU_NT = 256;
-// GS tile: 16 x 32 outputs, 192 work-items (the four colour positions cover 171/162/136/128 nodes)
-constexpr int GS_TY = 16, GS_TZ = 32, GS_NT = 192;
+// GS tile: 16 x 32 outputs, 256 work-items (the four colour positions cover up to 171/162/136/128 nodes)
+constexpr int GS_TY = 16, GS_TZ = 32, GS_NT = 256;
 
 }  // namespace
 
@@ -610,13 +1041,21 @@ void launch_fine_gs_phase(const Grid &g, const T *F, const T *E, const T *uin, c
     if (nplanes <= 0) return;
     const int ntY = ceil_div(g.nn[1], GS_TY), ntZ = ceil_div(g.nn[2], GS_TZ);
     const int tiles = ntY * ntZ;
-    const int want = 6 * cu_count();
+    // as many workgroups as fit at once (LDS: two per CU in FP64, four in FP32); each chunk of planes
+    // pays one unpipelined load of its first plane step
+    const int want = (sizeof(T) == 8 ? 2 : 4) * cu_count();
     int chunks = std::max(1, std::min(nplanes, ceil_div(want, tiles)));
     const int pchunk = ceil_div(nplanes, chunks);
     chunks = ceil_div(nplanes, pchunk);
     if (chunks > 65535) throw std::runtime_error("grid too large for the two-phase smoother");
-    hipLaunchKernelGGL((k_gs_phase<T, GS_TY, GS_TZ, GS_NT>), dim3(tiles, chunks), dim3(GS_NT), 0, s, g, F, E, uin,
-                       unb, uout, b, dmask, parity, forward, ntY, ntZ, pchunk);
+    if ((size_t)3 * g.nnodes * sizeof(T) >= ((size_t)1 << 30) || (size_t)g.nelems * sizeof(T) >= ((size_t)1 << 30))
+        throw std::runtime_error("field too large for the two-phase smoother's 32-bit buffer offsets");
+    if (forward)
+        hipLaunchKernelGGL((k_gs_phase<T, GS_TY, GS_TZ, GS_NT, true>), dim3(tiles, chunks), dim3(GS_NT), 0, s, g, F, E,
+                           uin, unb, uout, b, dmask, parity, ntY, ntZ, pchunk);
+    else
+        hipLaunchKernelGGL((k_gs_phase<T, GS_TY, GS_TZ, GS_NT, false>), dim3(tiles, chunks), dim3(GS_NT), 0, s, g, F,
+                           E, uin, unb, uout, b, dmask, parity, ntY, ntZ, pchunk);
     VF_HIP(hipGetLastError());
 }
 

@@ -1,10 +1,12 @@
 """Driver for counter-based profiling of the fine-level K u (run it under rocprofv3 --pmc):
 
-    python -m voxelfem_amd.profile_ku [--grid 384,192,192] [--variant 3] [--reps 3]
+    python -m voxelfem_amd.profile_ku [--grid 384,192,192] [--variant 3] [--reps 3] [--which 0] [--prec fp64]
 
-It builds the cantilever workload's hierarchy, launches the chosen K u family `reps` times through
-vf_bench_mg_kernel (variant: 1 register-gather, 2 LDS-tiled, 3 Walsh basis), then runs a 2 GiB device fill
-and a 2 GiB device copy whose byte counts are known, to calibrate FETCH_SIZE / WRITE_SIZE.
+It builds the cantilever workload's hierarchy, launches the chosen fine-level kernel family `reps` times
+through vf_bench_mg_kernel (variant: 1 register-gather, 2 LDS-tiled, 3 Walsh-basis K u with the tiled
+smoother, 4 Walsh-basis K u with the structured gather smoother; which: 0 K u, 2 one smoothing sweep), then
+runs a 2 GiB device fill and a 2 GiB device copy whose byte counts are known, to calibrate FETCH_SIZE /
+WRITE_SIZE.
 """
 import argparse
 
@@ -16,6 +18,8 @@ def main(argv=None):
     ap.add_argument("--grid", default="384,192,192")
     ap.add_argument("--variant", type=int, default=3)
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--which", type=int, default=0, choices=[0, 2])
+    ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
     args = ap.parse_args(argv)
     import torch
 
@@ -29,9 +33,11 @@ def main(argv=None):
     h = mg._handle()
     capi.check(capi.lib().vf_mg_update_stiffness(h))
     capi.set_kernel_variant(args.variant)
-    ms = capi.bench_mg_kernel(h, 0, 0, args.reps, capi.VF_PREC_FP64)
+    prec = capi.VF_PREC_FP32 if args.prec == "fp32" else capi.VF_PREC_FP64
+    ms = capi.bench_mg_kernel(h, args.which, 0, args.reps, prec)
     capi.set_kernel_variant(0)
-    print(f"K u FP64 variant {args.variant}: {ms * 1e3:.1f} us per launch", flush=True)
+    what = "K u" if args.which == 0 else "sweep"
+    print(f"{what} {args.prec} variant {args.variant}: {ms * 1e3:.1f} us per call", flush=True)
     a = torch.ones(2**28, dtype=torch.float64, device="cuda:0")
     b = torch.zeros_like(a)
     b.copy_(a)
