@@ -383,9 +383,16 @@ struct vf_mg {
     bool symmetricGS = true;
     bool useGraphs = false;  // replay the steady-state PCG iteration from a captured HIP graph
     bool l1Blocks = false;  // level 1 held as assembled block rows (set by update_stiffness)
+    // block rows of the current stiffness in FP64 / FP32, assembled on first use by a cycle of that precision
+    bool blocksReady[2] = {false, false};
     // coarsest operator
     DBuf<double> Ainv, invScratch, ctmp_in, ctmp_out, band;  // Ainv holds X = L^-1 (see factor_coarsest)
     DBuf<int> cholFlag;
+    // the coarsest factorization runs on its own stream, overlapping the block assembly and the first
+    // cycle's descent; the main stream waits for it at the first coarse solve (finish_factor)
+    hipStream_t fstream = nullptr;
+    hipEvent_t evGalerkin = nullptr, evFactor = nullptr;
+    bool factorPending = false;
     int bandKb = 1;
     DBuf<int> dRedToFull, dRed;
     int nred = 0, ldA = 0;
@@ -413,6 +420,17 @@ struct vf_mg {
             }
         }
     }
+
+    ~vf_mg() {
+        if (fstream) {
+            (void)hipStreamSynchronize(fstream);
+            (void)hipStreamDestroy(fstream);
+        }
+        if (evGalerkin) (void)hipEventDestroy(evGalerkin);
+        if (evFactor) (void)hipEventDestroy(evFactor);
+    }
+    vf_mg(const vf_mg &) = delete;
+    vf_mg &operator=(const vf_mg &) = delete;
 
     vf_mg(vf_sim *s, int levels) : sim(s), L(levels) {
         if (levels < 0) throw std::runtime_error("numCoarseningLevels must be >= 0");
@@ -555,6 +573,7 @@ struct vf_mg {
     // best of two runs after a warm-up) and keep the fastest per operation and precision.
     template <typename T>
     void tune() {
+        ensure_blocks<T>();
         const int p = pidx<T>();
         if (g_kernel_variant != 0 || !sim->structured || (kuChoice[p] >= 0 && gsChoice[p] >= 0)) return;
         const Grid &g = lv[0]->g;
@@ -688,6 +707,7 @@ struct vf_mg {
     template <typename T>
     void coarse_solve(const T *b, T *x) {
         ensure_factor();
+        finish_factor();
         launch_coarse_chol_apply<T>(nred, Ainv.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
                                     ctmp_out.data(), st());
     }
@@ -832,14 +852,7 @@ struct vf_mg {
             // level 1 as assembled block rows (HBM holds them easily; the reference applies this level
             // on the fly to save memory), unless the register-gather family is forced
             l1Blocks = g_kernel_variant != 1;
-            if (l1Blocks) {
-                LevelData &c1 = *lv[1];
-                const size_t nb = (size_t)c1.g.nnodes * 243;
-                c1.blocks.resize(nb);
-                c1.blocks32.resize(nb);
-                launch_assemble_l1_blocks(c1.g, lv[0]->g, dChat.data(), sim->E.data(), c1.blocks.data(), s);
-                launch_convert<float, double>(nb, c1.blocks.data(), c1.blocks32.data(), s);
-            } else {
+            if (!l1Blocks) {
                 lv[1]->blocks.release();
                 lv[1]->blocks32.release();
             }
@@ -851,18 +864,46 @@ struct vf_mg {
                 c.Ke.resize((size_t)c.g.nelems * 576);
                 launch_galerkin_recurse(c.g, lv[l - 1]->g, dphi.data(), lv[l - 1]->Ke.data(), c.Ke.data(), s);
             }
-            for (int l = 2; l < L; ++l) {
-                LevelData &c = *lv[l];
-                const size_t nb = (size_t)c.g.nnodes * 243;
-                c.blocks.resize(nb);
-                c.blocks32.resize(nb);
-                launch_assemble_blocks(c.g, c.Ke.data(), c.blocks.data(), s);
-                launch_convert<float, double>(nb, c.blocks.data(), c.blocks32.data(), s);
-            }
         }
+        blocksReady[0] = blocksReady[1] = false;
         factor_coarsest();
         stiffnessReady = true;
         stiffnessVersion = sim->version;
+    }
+    template <typename T>
+    DBuf<T> &block_buf(int l);
+    // block rows of levels 1 (when assembled) .. L-1 in the precision T of the cycle about to use them,
+    // summed in FP64 from the current element matrices
+    template <typename T>
+    void ensure_blocks() {
+        ensure_stiffness();
+        const int p = pidx<T>();
+        if (blocksReady[p]) return;
+        hipStream_t s = st();
+        if (L >= 2 && l1Blocks) {
+            LevelData &c1 = *lv[1];
+            DBuf<T> &buf = block_buf<T>(1);
+            buf.resize((size_t)c1.g.nnodes * 243);
+            launch_assemble_l1_blocks<T>(c1.g, lv[0]->g, dChat.data(), sim->E.data(), buf.data(), s);
+        }
+        for (int l = 2; l < L; ++l) {
+            LevelData &c = *lv[l];
+            DBuf<T> &buf = block_buf<T>(l);
+            buf.resize((size_t)c.g.nnodes * 243);
+            launch_assemble_blocks<T>(c.g, c.Ke.data(), buf.data(), s);
+        }
+        blocksReady[p] = true;
+    }
+    // the main stream waits for the coarsest factorization launched by the last update; a non-positive
+    // pivot is reported here
+    void finish_factor() {
+        if (!factorPending) return;
+        factorPending = false;
+        VF_HIP(hipStreamWaitEvent(st(), evFactor, 0));
+        int flag = 0;
+        d2h(&flag, cholFlag.data(), 1, fstream);
+        sync(fstream);
+        if (flag) throw std::runtime_error("coarsest-level operator is not positive definite");
     }
     void ensure_factor() {
         if (!stiffnessReady) update_stiffness();
@@ -870,6 +911,7 @@ struct vf_mg {
     // reduced coarsest operator: Dirichlet components and detached nodes removed
     // (findFixedVars, TensorProductSimulator.hh:1181-1195), then its FP64 inverse.
     void factor_coarsest() {
+        finish_factor();  // buffers below may be reallocated
         LevelData &c = *lv[L];
         hipStream_t s = st();
         const Grid &g = c.g;
@@ -927,12 +969,18 @@ struct vf_mg {
         ctmp_out.resize(ldA);
         cholFlag.resize(1);
         if (nred == 0) return;  // nothing to solve: the coarse correction is zero
+        if (!fstream) {
+            VF_HIP(hipStreamCreateWithFlags(&fstream, hipStreamNonBlocking));
+            VF_HIP(hipEventCreateWithFlags(&evGalerkin, hipEventDisableTiming));
+            VF_HIP(hipEventCreateWithFlags(&evFactor, hipEventDisableTiming));
+        }
+        // the coarsest element matrices and the index maps above are complete on the main stream
+        VF_HIP(hipEventRecord(evGalerkin, s));
+        VF_HIP(hipStreamWaitEvent(fstream, evGalerkin, 0));
         band_cholesky_inverse(g, c.Ke.data(), dRed.data(), std::max(nred, 1), bandKb, band.data(), invScratch.data(),
-                              Ainv.data(), cholFlag.data(), s);
-        int flag = 0;
-        d2h(&flag, cholFlag.data(), 1, s);
-        sync(s);
-        if (flag) throw std::runtime_error("coarsest-level operator is not positive definite");
+                              Ainv.data(), cholFlag.data(), fstream);
+        VF_HIP(hipEventRecord(evFactor, fstream));
+        factorPending = true;
     }
 
     // ---------------------------------------------------------------- PCG (MultigridSolver.hh:1046-1152)
@@ -1060,8 +1108,16 @@ template <> const double *vf_mg::Ep<double>() const { return sim->E.data(); }
 template <> const float *vf_mg::Ep<float>() const { return sim->E32.data(); }
 template <> const double *vf_mg::Chatp<double>() const { return dChat.data(); }
 template <> const float *vf_mg::Chatp<float>() const { return dChat32.data(); }
-template <> const double *vf_mg::blocksp<double>(int l) const { return lv[l]->blocks.data(); }
-template <> const float *vf_mg::blocksp<float>(int l) const { return lv[l]->blocks32.data(); }
+template <> const double *vf_mg::blocksp<double>(int l) const {
+    if (!blocksReady[0]) throw std::runtime_error("internal: FP64 block rows used before assembly");
+    return lv[l]->blocks.data();
+}
+template <> const float *vf_mg::blocksp<float>(int l) const {
+    if (!blocksReady[1]) throw std::runtime_error("internal: FP32 block rows used before assembly");
+    return lv[l]->blocks32.data();
+}
+template <> DBuf<double> &vf_mg::block_buf<double>(int l) { return lv[l]->blocks; }
+template <> DBuf<float> &vf_mg::block_buf<float>(int l) { return lv[l]->blocks32; }
 
 // =================================================================================================
 // C ABI
@@ -1495,7 +1551,7 @@ int vf_mg_apply_K(vf_mg *mg, int64_t l, const double *u, double *y, int where) {
     need(mg && u && y, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
     mg->sync_masks();
-    if (l >= 2) mg->ensure_stiffness();
+    if (l >= 1) mg->ensure_blocks<double>();
     const size_t n = level_n(mg, l);
     Staged st{mg->sim, where};
     const double *du = st.in(u, n, mg->sim->stage_in(n));
@@ -1511,7 +1567,7 @@ int vf_mg_residual(vf_mg *mg, int64_t l, const double *u, const double *b, doubl
     need(mg && u && b && r, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
     mg->sync_masks();
-    if (l >= 2) mg->ensure_stiffness();
+    if (l >= 1) mg->ensure_blocks<double>();
     const size_t n = level_n(mg, l);
     Staged st{mg->sim, where};
     const double *du = st.in(u, n, mg->sim->stage_in(n));
@@ -1528,7 +1584,7 @@ int vf_mg_smooth(vf_mg *mg, int64_t l, double *u, const double *b, int forward, 
     need(mg && u && b, "null argument");
     need(l >= 0 && l < mg->L, "smoothing needs a non-coarsest level");
     mg->sync_masks();
-    if (l >= 2) mg->ensure_stiffness();
+    if (l >= 1) mg->ensure_blocks<double>();
     const size_t n = level_n(mg, l);
     Staged st{mg->sim, where};
     double *du = where == VF_DEVICE ? u : mg->sim->stage_out(n);

@@ -212,9 +212,12 @@ void launch_galerkin_l2_elems(const Grid &g2, const Grid &gf, const double *M2, 
                               hipStream_t s);
 void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, const double *Kf, double *Kc,
                              hipStream_t s);
-void launch_assemble_blocks(const Grid &g, const double *Ke, double *blocks, hipStream_t s);
+// block rows (FP64 sums stored as OutT = double or float)
+template <typename OutT>
+void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s);
 // level-1 block rows assembled from the fine moduli and the eight coarsened element matrices
-void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *blocks,
+template <typename OutT>
+void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
                                hipStream_t s);
 // coarsest level: band Cholesky of the reduced operator (half-bandwidth kb blocks of 64) and the explicit
 // X = L^-1 (dense lower triangular, leading dimension n rounded up to 64); flag != 0 if not positive definite

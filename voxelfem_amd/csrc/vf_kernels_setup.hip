@@ -109,9 +109,11 @@ __global__ void __launch_bounds__(576) k_galerkin_recurse(Grid gc, Grid gf, cons
 }
 
 // ---- block rows of an intermediate level from its element matrices.
-// block(n, slot)[a][c] = sum over elements e containing n and m = n + off(slot) of Ke_e[3 l_n + a][3 l_m + c].
+// block(n, slot)[a][c] = sum over elements e containing n and m = n + off(slot) of Ke_e[3 l_n + a][3 l_m + c],
+// accumulated in FP64 and stored in the precision of the cycle that reads it.
+template <typename OutT>
 __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *__restrict__ Ke,
-                                                         double *__restrict__ blocks) {
+                                                         OutT *__restrict__ blocks) {
     const long t = (long)blockIdx.x * TPB + threadIdx.x;
     if (t >= (long)g.nnodes * 27) return;
     const int n = (int)(t % g.nnodes), slot = (int)(t / g.nnodes);
@@ -138,16 +140,18 @@ __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *_
                 }
     }
     const BlockSlot bs = block_slot(g, ix, iy, iz);
-    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = B[k];
+    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = (OutT)B[k];
 }
 
 // ---- level-1 block rows straight from the fine moduli: the block of coarse node n and neighbour slot
 // is sum over the coarse elements holding both nodes (unmasked: ey < ymask) of sum_fi E_child(fi) *
 // Chat_fi[(3 ln + a), (3 lm + c)], i.e. the assembled form of the level-1 operator that
-// MultigridSolver.hh:483-491 applies on the fly. Colour-major output layout (BlockSlot).
+// MultigridSolver.hh:483-491 applies on the fly. Colour-major output layout (BlockSlot); accumulated in
+// FP64, stored in the precision of the cycle that reads it.
+template <typename OutT>
 __global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, const double *__restrict__ Chat,
                                                             const double *__restrict__ Ef,
-                                                            double *__restrict__ blocks) {
+                                                            OutT *__restrict__ blocks) {
     const long t = (long)blockIdx.x * TPB + threadIdx.x;
     if (t >= (long)gc.nnodes * 27) return;
     const int n = (int)(t % gc.nnodes), slot = (int)(t / gc.nnodes);
@@ -179,7 +183,7 @@ __global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, co
                 }
     }
     const BlockSlot bs = block_slot(gc, ix, iy, iz);
-    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = B[k];
+    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = (OutT)B[k];
 }
 
 }  // namespace
@@ -202,16 +206,25 @@ void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, 
     VF_HIP(hipGetLastError());
 }
 
-void launch_assemble_blocks(const Grid &g, const double *Ke, double *blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble_blocks, dim3(nblocks((long)g.nnodes * 27)), dim3(TPB), 0, s, g, Ke, blocks);
+template <typename OutT>
+void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_assemble_blocks<OutT>, dim3(nblocks((long)g.nnodes * 27)), dim3(TPB), 0, s, g, Ke, blocks);
     VF_HIP(hipGetLastError());
 }
 
-void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *blocks,
+template <typename OutT>
+void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble_l1_blocks, dim3(nblocks((long)gc.nnodes * 27)), dim3(TPB), 0, s, gc, gf, Chat, Ef,
-                       blocks);
+    hipLaunchKernelGGL(k_assemble_l1_blocks<OutT>, dim3(nblocks((long)gc.nnodes * 27)), dim3(TPB), 0, s, gc, gf, Chat,
+                       Ef, blocks);
     VF_HIP(hipGetLastError());
 }
+
+template void launch_assemble_blocks<double>(const Grid &, const double *, double *, hipStream_t);
+template void launch_assemble_blocks<float>(const Grid &, const double *, float *, hipStream_t);
+template void launch_assemble_l1_blocks<double>(const Grid &, const Grid &, const double *, const double *, double *,
+                                                hipStream_t);
+template void launch_assemble_l1_blocks<float>(const Grid &, const Grid &, const double *, const double *, float *,
+                                               hipStream_t);
 
 }  // namespace vf
