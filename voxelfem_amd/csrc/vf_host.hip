@@ -796,7 +796,9 @@ struct vf_mg {
     }
 
     // applyPreconditionerInv (MultigridSolver.hh:577-580) on device vectors (SoA, fine level)
-    void precondition(const double *r, double *s, int mgIter, int nu, bool fullMG, int prec) {
+    // resident (FP32 cycles only): the right-hand side is already in lv[0]->b32 and the result stays in
+    // lv[0]->x32 (r and s are not touched)
+    void precondition(const double *r, double *s, int mgIter, int nu, bool fullMG, int prec, bool resident = false) {
         sync_masks();
         const size_t n0 = 3 * (size_t)lv[0]->g.nnodes;
         if (nu == 0 || mgIter == 0) { d2d(s, r, n0, st()); return; }
@@ -815,13 +817,13 @@ struct vf_mg {
         } else {
             ensure_buffers<float>(0);
             tune<float>();
-            launch_convert<float, double>(n0, r, lv[0]->b32.data(), st());
+            if (!resident) launch_convert<float, double>(n0, r, lv[0]->b32.data(), st());
             launch_fill<float>(n0, lv[0]->x32.data(), 0.0f, st());
             for (int i = 0; i < mgIter; ++i) {
                 if (fullMG && i == 0) fmg<float>(0, nu);
                 else vcycle<float>(0, nu);
             }
-            launch_convert<double, float>(n0, lv[0]->x32.data(), s, st());
+            if (!resident) launch_convert<double, float>(n0, lv[0]->x32.data(), s, st());
         }
         b0v = x0v = nullptr;
     }
@@ -1033,8 +1035,39 @@ struct vf_mg {
         int64_t it = 0;
         bool updated = false, first = true;
         const bool flexible = prec == VF_PREC_FP32;
+        // Without a host callback the iterate is only needed at the end, so the fused form runs: the
+        // preconditioned vector is read in the cycle's precision, x += alpha d is applied inside the next
+        // direction update (and once after the loop), and the residual update also writes the FP32
+        // right-hand side of the next cycle.
+        const bool fused = !cb && mgIter > 0 && nu > 0 && L > 0;
+        float *r32 = nullptr;
+        if (fused && prec == VF_PREC_FP32) {
+            ensure_buffers<float>(0);
+            r32 = lv[0]->b32.data();
+            launch_convert<float, double>(n0, r, r32, s);
+        }
         // one PCG iteration, device work only (every launch is on s, no host synchronisation)
         auto body = [&](bool firstIt) {
+            if (fused) {
+                precondition(r, sv, mgIter, nu, fullMG, prec, true);
+                const double *adp = (flexible && !firstIt) ? Ad : nullptr;
+                launch_pcg_scalars(sc, 0, s);  // rz_old <- rz
+                if (prec == VF_PREC_FP32) {
+                    float *s32 = lv[0]->x32.data();
+                    zero_dirichlet<float>(0, s32);
+                    launch_pcg_dots_s<float>(g, r, s32, adp, part, sc, s);
+                    launch_pcg_direction_x<float>(g, s32, d, x, sc, flexible ? 1 : 0, firstIt ? 1 : 0, s);
+                } else {
+                    zero_dirichlet<double>(0, sv);
+                    launch_pcg_dots_s<double>(g, r, sv, adp, part, sc, s);
+                    launch_pcg_direction_x<double>(g, sv, d, x, sc, flexible ? 1 : 0, firstIt ? 1 : 0, s);
+                }
+                apply<double>(0, d, Ad, nullptr, AM_SET, true);
+                launch_dot_active(g, d, Ad, part, sc + 3, s);
+                launch_pcg_scalars(sc, 1, s);  // alpha of this iteration
+                launch_pcg_residual(g, r, Ad, sc, part, sc + 4, r32, s);
+                return;
+            }
             if (mgIter > 0) precondition(r, sv, mgIter, nu, fullMG, prec);
             else d2d(sv, r, n0, s);
             zero_dirichlet<double>(0, sv);
@@ -1079,6 +1112,7 @@ struct vf_mg {
             callback(it);
             first = false;
         }
+        if (fused && it > 0) launch_pcg_finish_x(g, x, d, sc, s);
         if (stats) {
             stats->iterations = it;
             stats->residual_norm = std::sqrt(rr);

@@ -201,6 +201,18 @@ int reduction_partials_full(size_t n);
 // PCG vector kernels (MultigridSolver.hh:1119-1143); scalars live in device memory `sc`
 // (layout documented in vf_kernels_vec.hip).
 void launch_pcg_direction(const Grid &g, const double *s, double *d, const double *sc, int mode, hipStream_t st);
+// fused PCG steps (no host callback): sc[0] = <r,s> (and sc[1] = <Ad,s> when adp is given) with s in the
+// cycle's precision; first ? d = s : (x += sc[5] d, d = s + beta d); r -= alpha Ad with <r,r> and an
+// optional float copy of r; x += sc[5] d after the last iteration
+template <typename TS>
+void launch_pcg_dots_s(const Grid &g, const double *r, const TS *s, const double *adp, double *partials, double *out,
+                       hipStream_t st);
+template <typename TS>
+void launch_pcg_direction_x(const Grid &g, const TS *s, double *d, double *x, const double *sc, int mode, int first,
+                            hipStream_t st);
+void launch_pcg_residual(const Grid &g, double *r, const double *Ad, const double *sc, double *partials, double *rr_out,
+                         float *r32, hipStream_t st);
+void launch_pcg_finish_x(const Grid &g, double *x, const double *d, const double *sc, hipStream_t st);
 void launch_pcg_update(const Grid &g, double *x, double *r, const double *d, const double *Ad, const double *sc,
                        double *partials, double *rr_out, hipStream_t st);
 void launch_pcg_scalars(double *sc, int op, hipStream_t st);

@@ -192,6 +192,93 @@ __global__ void __launch_bounds__(TPB) k_pcg_update(Active a, double *__restrict
     if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
 }
 
+// Fused forms used when the iterate is not needed between iterations (no host callback): the
+// preconditioned vector s is read in the precision the cycle produced it (TS), x += alpha d is deferred
+// into the next direction update, and the residual update also writes the single-precision right-hand
+// side of the next cycle. The arithmetic of each entry is the same as in the unfused kernels.
+template <typename TS>
+__global__ void __launch_bounds__(TPB) k_pcg_dots_s(Active a, const double *__restrict__ r, const TS *__restrict__ s,
+                                                    const double *__restrict__ adp, double *__restrict__ partials,
+                                                    int stride) {
+    __shared__ double sh[TPB / 64];
+    const size_t base = active_base(a);
+    double rs = 0, as = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            const double sv = (double)s[base + i];
+            rs += r[base + i] * sv;
+            if (adp) as += adp[base + i] * sv;
+        }
+    }
+    rs = block_sum(rs, sh);
+    if (adp) {
+        __syncthreads();
+        as = block_sum(as, sh);
+    }
+    if (threadIdx.x == 0) {
+        partials[blockIdx.y * gridDim.x + blockIdx.x] = rs;
+        if (adp) partials[stride + blockIdx.y * gridDim.x + blockIdx.x] = as;
+    }
+}
+
+// first: d = s. Otherwise x += alpha_prev d, then d = s + beta d (beta as in k_pcg_direction).
+template <typename TS>
+__global__ void __launch_bounds__(TPB) k_pcg_direction_x(Active a, const TS *__restrict__ s, double *__restrict__ d,
+                                                         double *__restrict__ x, const double *__restrict__ sc,
+                                                         int mode, int first) {
+    const size_t base = active_base(a);
+    if (first) {
+        for (int k = 0; k < ITEMS; ++k) {
+            const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+            if (i < a.rowLen) d[base + i] = (double)s[base + i];
+        }
+        return;
+    }
+    const double alpha = sc[5];
+    const double beta = mode == 0 ? sc[0] / sc[2] : -sc[5] * sc[1] / sc[2];
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            const double dd = d[base + i];
+            x[base + i] += alpha * dd;
+            d[base + i] = (double)s[base + i] + beta * dd;
+        }
+    }
+}
+
+// r -= alpha Ad with <r,r> partials; r32 (optional) receives float(r)
+__global__ void __launch_bounds__(TPB) k_pcg_residual(Active a, double *__restrict__ r, const double *__restrict__ Ad,
+                                                      const double *__restrict__ sc, double *__restrict__ partials,
+                                                      float *__restrict__ r32) {
+    __shared__ double sh[TPB / 64];
+    const double alpha = sc[0] / sc[3];
+    const size_t base = active_base(a);
+    double s = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            const double rn = r[base + i] - alpha * Ad[base + i];
+            r[base + i] = rn;
+            if (r32) r32[base + i] = (float)rn;
+            s += rn * rn;
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// x += alpha d with alpha = sc[5] (the deferred update of the last iteration)
+__global__ void __launch_bounds__(TPB) k_pcg_finish_x(Active a, double *__restrict__ x, const double *__restrict__ d,
+                                                      const double *__restrict__ sc) {
+    const double alpha = sc[5];
+    const size_t base = active_base(a);
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) x[base + i] += alpha * d[base + i];
+    }
+}
+
 __global__ void k_pcg_scalars(double *sc, int op) {
     if (threadIdx.x != 0) return;
     if (op == 0) sc[2] = sc[0];                 // previous <r,s> <- <r,s>
@@ -458,6 +545,49 @@ void launch_pcg_direction(const Grid &g, const double *s, double *d, const doubl
     hipLaunchKernelGGL(k_pcg_direction, active_grid(a), dim3(TPB), 0, st, a, s, d, sc, mode);
     VF_HIP(hipGetLastError());
 }
+
+template <typename TS>
+void launch_pcg_dots_s(const Grid &g, const double *r, const TS *s, const double *adp, double *partials,
+                       double *out, hipStream_t st) {
+    const Active a = active_of(g, 0);
+    if (a.rowLen <= 0) { launch_fill<double>(adp ? 2 : 1, out, 0.0, st); return; }
+    const dim3 gr = active_grid(a);
+    const int cnt = (int)(gr.x * gr.y);
+    hipLaunchKernelGGL(k_pcg_dots_s<TS>, gr, dim3(TPB), 0, st, a, r, s, adp, partials, cnt);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, st, partials, cnt, cnt, adp ? 2 : 1, out);
+    VF_HIP(hipGetLastError());
+}
+template <typename TS>
+void launch_pcg_direction_x(const Grid &g, const TS *s, double *d, double *x, const double *sc, int mode, int first,
+                            hipStream_t st) {
+    const Active a = active_of(g, 0);
+    if (a.rowLen <= 0) return;
+    hipLaunchKernelGGL(k_pcg_direction_x<TS>, active_grid(a), dim3(TPB), 0, st, a, s, d, x, sc, mode, first);
+    VF_HIP(hipGetLastError());
+}
+void launch_pcg_residual(const Grid &g, double *r, const double *Ad, const double *sc, double *partials,
+                         double *rr_out, float *r32, hipStream_t st) {
+    const Active a = active_of(g, 0);
+    if (a.rowLen <= 0) { launch_fill<double>(1, rr_out, 0.0, st); return; }
+    const dim3 gr = active_grid(a);
+    hipLaunchKernelGGL(k_pcg_residual, gr, dim3(TPB), 0, st, a, r, Ad, sc, partials, r32);
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, st, partials, (int)(gr.x * gr.y), 0, 1, rr_out);
+    VF_HIP(hipGetLastError());
+}
+void launch_pcg_finish_x(const Grid &g, double *x, const double *d, const double *sc, hipStream_t st) {
+    const Active a = active_of(g, 0);
+    if (a.rowLen <= 0) return;
+    hipLaunchKernelGGL(k_pcg_finish_x, active_grid(a), dim3(TPB), 0, st, a, x, d, sc);
+    VF_HIP(hipGetLastError());
+}
+template void launch_pcg_dots_s<double>(const Grid &, const double *, const double *, const double *, double *,
+                                        double *, hipStream_t);
+template void launch_pcg_dots_s<float>(const Grid &, const double *, const float *, const double *, double *,
+                                       double *, hipStream_t);
+template void launch_pcg_direction_x<double>(const Grid &, const double *, double *, double *, const double *, int,
+                                             int, hipStream_t);
+template void launch_pcg_direction_x<float>(const Grid &, const float *, double *, double *, const double *, int, int,
+                                            hipStream_t);
 
 void launch_pcg_update(const Grid &g, double *x, double *r, const double *d, const double *Ad, const double *sc,
                        double *partials, double *rr_out, hipStream_t st) {
