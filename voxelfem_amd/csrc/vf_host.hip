@@ -34,7 +34,12 @@ void sync(hipStream_t s) { VF_HIP(hipStreamSynchronize(s)); }
 
 template <typename T>
 void h2d(T *dst, const T *src, size_t n, hipStream_t s) {
-    if (n) VF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+    // The sources are ordinary (pageable) host buffers, often temporaries; the copy is complete when this
+    // returns, so they may be reused or freed right away.
+    if (n) {
+        VF_HIP(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+        VF_HIP(hipStreamSynchronize(s));
+    }
 }
 template <typename T>
 void d2h(T *dst, const T *src, size_t n, hipStream_t s) {
@@ -386,7 +391,7 @@ struct vf_mg {
     // block rows of the current stiffness in FP64 / FP32, assembled on first use by a cycle of that precision
     bool blocksReady[2] = {false, false};
     // coarsest operator
-    DBuf<double> Ainv, invScratch, ctmp_in, ctmp_out, band;  // Ainv holds X = L^-1 (see factor_coarsest)
+    DBuf<double> Ainv, AinvT, invScratch, ctmp_in, ctmp_out, band;  // X = L^-1 and X^T (see prepare_coarsest, launch_factor)
     DBuf<int> cholFlag;
     // the coarsest factorization runs on its own stream, overlapping the block assembly and the first
     // cycle's descent; the main stream waits for it at the first coarse solve (finish_factor)
@@ -653,6 +658,9 @@ struct vf_mg {
     float tuneMs[2][8] = {};
     bool l1OnTheFly[2] = {false, false};  // level-1 residual on the fly instead of from the block rows
 
+    // Ad = K d on the fine level (Dirichlet rows zero) and *dAd = <d, Ad> over the active nodes, fused into
+    // the Walsh-basis product when that family is in use
+    void apply_dot(const double *d, double *Ad, double *partials, double *dAd);
     template <typename T>
     void apply(int l, const T *u, T *y, const T *b, int mode, bool useMask) {
         const uint8_t *mask = useMask ? dm(l) : nullptr;
@@ -708,7 +716,7 @@ struct vf_mg {
     void coarse_solve(const T *b, T *x) {
         ensure_factor();
         finish_factor();
-        launch_coarse_chol_apply<T>(nred, Ainv.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
+        launch_coarse_chol_apply<T>(nred, Ainv.data(), AinvT.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
                                     ctmp_out.data(), st());
     }
 
@@ -836,6 +844,7 @@ struct vf_mg {
     void update_stiffness() {
         sync_masks();
         hipStream_t s = st();
+        prepare_coarsest();  // host index maps and their upload, before the device work is queued
         if (L == 0) {
             LevelData &c = *lv[0];
             c.Ke.resize((size_t)c.g.nelems * 576);
@@ -868,7 +877,7 @@ struct vf_mg {
             }
         }
         blocksReady[0] = blocksReady[1] = false;
-        factor_coarsest();
+        launch_factor();
         stiffnessReady = true;
         stiffnessVersion = sim->version;
     }
@@ -912,7 +921,7 @@ struct vf_mg {
     }
     // reduced coarsest operator: Dirichlet components and detached nodes removed
     // (findFixedVars, TensorProductSimulator.hh:1181-1195), then its FP64 inverse.
-    void factor_coarsest() {
+    void prepare_coarsest() {
         finish_factor();  // buffers below may be reallocated
         LevelData &c = *lv[L];
         hipStream_t s = st();
@@ -965,12 +974,17 @@ struct vf_mg {
         h2d(dRed.data(), red.data(), red.size(), s);
         h2d(dRedToFull.data(), redToFull.data(), redToFull.size(), s);
         Ainv.resize((size_t)ldA * ldA);
+        AinvT.resize((size_t)ldA * ldA);
         band.resize(band_storage(std::max(nred, 1), bandKb));
         invScratch.resize((size_t)(ldA / 64) * 64 * 64);
         ctmp_in.resize(ldA);
         ctmp_out.resize(ldA);
         cholFlag.resize(1);
+    }
+    void launch_factor() {
         if (nred == 0) return;  // nothing to solve: the coarse correction is zero
+        const Grid &g = lv[L]->g;
+        hipStream_t s = st();
         if (!fstream) {
             VF_HIP(hipStreamCreateWithFlags(&fstream, hipStreamNonBlocking));
             VF_HIP(hipEventCreateWithFlags(&evGalerkin, hipEventDisableTiming));
@@ -979,8 +993,8 @@ struct vf_mg {
         // the coarsest element matrices and the index maps above are complete on the main stream
         VF_HIP(hipEventRecord(evGalerkin, s));
         VF_HIP(hipStreamWaitEvent(fstream, evGalerkin, 0));
-        band_cholesky_inverse(g, c.Ke.data(), dRed.data(), std::max(nred, 1), bandKb, band.data(), invScratch.data(),
-                              Ainv.data(), cholFlag.data(), fstream);
+        band_cholesky_inverse(g, lv[L]->Ke.data(), dRed.data(), std::max(nred, 1), bandKb, band.data(),
+                              invScratch.data(), Ainv.data(), AinvT.data(), cholFlag.data(), fstream);
         VF_HIP(hipEventRecord(evFactor, fstream));
         factorPending = true;
     }
@@ -1062,8 +1076,7 @@ struct vf_mg {
                     launch_pcg_dots_s<double>(g, r, sv, adp, part, sc, s);
                     launch_pcg_direction_x<double>(g, sv, d, x, sc, flexible ? 1 : 0, firstIt ? 1 : 0, s);
                 }
-                apply<double>(0, d, Ad, nullptr, AM_SET, true);
-                launch_dot_active(g, d, Ad, part, sc + 3, s);
+                apply_dot(d, Ad, part, sc + 3);
                 launch_pcg_scalars(sc, 1, s);  // alpha of this iteration
                 launch_pcg_residual(g, r, Ad, sc, part, sc + 4, r32, s);
                 return;
@@ -1108,7 +1121,13 @@ struct vf_mg {
                 body(first);
             }
             rr = sim->read_scalar(sc + 4);
-            if (std::isnan(rr)) throw NanError("NaN encountered at iteration" + std::to_string(it));
+            if (std::isnan(rr)) {
+                double v[6];
+                d2h(v, sc, 6, s);
+                sync(s);
+                throw NanError("NaN encountered at iteration" + std::to_string(it) + " (<r,s> " + std::to_string(v[0]) +
+                               ", <d,Ad> " + std::to_string(v[3]) + ", <r,r> " + std::to_string(v[4]) + ")");
+            }
             callback(it);
             first = false;
         }
@@ -1152,6 +1171,15 @@ template <> const float *vf_mg::blocksp<float>(int l) const {
 }
 template <> DBuf<double> &vf_mg::block_buf<double>(int l) { return lv[l]->blocks; }
 template <> DBuf<float> &vf_mg::block_buf<float>(int l) { return lv[l]->blocks32; }
+void vf_mg::apply_dot(const double *d, double *Ad, double *partials, double *dAd) {
+    if (ku_family<double>() == 2) {
+        launch_fine_apply_walsh<double>(lv[0]->g, Wp<double>(), Ep<double>(), d, Ad, nullptr, dm(0), AM_SET, st(),
+                                        partials, dAd);
+    } else {
+        apply<double>(0, d, Ad, nullptr, AM_SET, true);
+        launch_dot_active(lv[0]->g, d, Ad, partials, dAd, st());
+    }
+}
 
 // =================================================================================================
 // C ABI

@@ -132,9 +132,14 @@ void launch_fine_apply_tiled(const Grid &g, const T *K0, const T *E, const T *u,
                              const uint8_t *dmask, int mode, hipStream_t s);
 // Element-centric fine K u in the Walsh basis (AM_SET or AM_RESIDUAL); C = the 8 x 16 table of the
 // nonzero Walsh-basis element stiffness entries divided by 64 (see vf_kernels_walsh.hip).
+// With dotOut (SET mode only), *dotOut = <u, y> is also formed from per-workgroup partials (dotPartials
+// needs one entry per workgroup, at most 65536).
 template <typename T>
 void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
-                             const uint8_t *dmask, int mode, hipStream_t s);
+                             const uint8_t *dmask, int mode, hipStream_t s, double *dotPartials = nullptr,
+                             double *dotOut = nullptr);
+// fixed-order sum of count partials into *out
+void launch_final_sum(const double *partials, int count, double *out, hipStream_t s);
 // One colour of the fine block Gauss-Seidel sweep on the structured element matrix, register gather
 template <typename T>
 void launch_fine_gs_gather_f(const Grid &g, const T *F, const T *E, T *u, const T *b, const uint8_t *dmask, int color,
@@ -232,13 +237,15 @@ template <typename OutT>
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
                                hipStream_t s);
 // coarsest level: band Cholesky of the reduced operator (half-bandwidth kb blocks of 64) and the explicit
-// X = L^-1 (dense lower triangular, leading dimension n rounded up to 64); flag != 0 if not positive definite
+// X = L^-1 (dense lower triangular, leading dimension n rounded up to 64) and its transpose XT (filled on
+// and above the diagonal); flag != 0 if not positive definite
 size_t band_storage(int n, int kb);
 void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int n, int kb, double *B, double *Linv,
-                           double *X, int *flag, hipStream_t s);
+                           double *X, double *XT, int *flag, hipStream_t s);
+// x = X^T X b on the reduced unknowns (zero elsewhere)
 template <typename T>
-void launch_coarse_chol_apply(int nred, const double *X, const int *redToFull, int nnodes, const T *b, T *x,
-                              double *tmp_in, double *tmp_out, hipStream_t s);
+void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes, const T *b,
+                              T *x, double *tmp_in, double *tmp_out, hipStream_t s);
 
 // elementwise physics
 void launch_young(const Grid &g, const double *rho, double *E, float *E32, int law, double E0, double Emin,

@@ -263,19 +263,35 @@ __global__ void __launch_bounds__(TPB) k_trmv_lower(int n, int ld, const double 
     if (lane == 0) y[row] = s;
 }
 
-// ---- x = X^T y over the lower triangle: workgroup of 256 handles 64 columns; rows split over 4
-//      wavefront groups, combined through LDS in a fixed order
-__global__ void __launch_bounds__(TPB) k_trmv_upper_t(int n, int ld, const double *__restrict__ X,
-                                                      const double *__restrict__ y, double *__restrict__ x) {
-    __shared__ double part[4][64];
-    const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
-    const int col = blockIdx.x * 64 + lane;
-    double s = 0;
-    if (col < n)
-        for (int i = col + grp; i < n; i += 4) s += X[(size_t)i * ld + col] * y[i];
-    part[grp][lane] = s;
+// ---- XT = X^T for the 64 x 64 tiles on and below the diagonal of X (so XT is filled on and above
+//      its diagonal; the upper triangle of X is zero)
+__global__ void __launch_bounds__(TPB) k_transpose_lower(int ld, const double *__restrict__ X, double *__restrict__ XT) {
+    const int bi = blockIdx.y, bj = blockIdx.x;  // tile row / column of X
+    if (bj > bi) return;
+    __shared__ double tl[NB][NB + 1];
+    const int t = threadIdx.x;
+    for (int w = t; w < NB * NB; w += TPB) {
+        const int r = w / NB, c = w % NB;
+        tl[r][c] = X[(size_t)(bi * NB + r) * ld + bj * NB + c];
+    }
     __syncthreads();
-    if (grp == 0 && col < n) x[col] = ((part[0][lane] + part[1][lane]) + part[2][lane]) + part[3][lane];
+    for (int w = t; w < NB * NB; w += TPB) {
+        const int r = w / NB, c = w % NB;
+        XT[(size_t)(bj * NB + r) * ld + bi * NB + c] = tl[c][r];
+    }
+}
+
+// ---- x = XT y over the upper triangle of XT (one wavefront per row, coalesced row reads)
+__global__ void __launch_bounds__(TPB) k_trmv_upper(int n, int ld, const double *__restrict__ XT,
+                                                    const double *__restrict__ y, double *__restrict__ x) {
+    const int lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    if (row >= n) return;
+    const double *Xr = XT + (size_t)row * ld;
+    double s = 0;
+    for (int j = row + lane; j < n; j += 64) s += Xr[j] * y[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) x[row] = s;
 }
 
 template <typename T>
@@ -312,7 +328,7 @@ size_t band_storage(int n, int kb) {
 }
 
 void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int n, int kb, double *B, double *Linv,
-                           double *X, int *flag, hipStream_t s) {
+                           double *X, double *XT, int *flag, hipStream_t s) {
     const int nb = (n + NB - 1) / NB;
     const size_t ld = (size_t)nb * NB;
     VF_HIP(hipMemsetAsync(B, 0, band_storage(n, kb) * sizeof(double), s));
@@ -329,25 +345,26 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
         }
     }
     hipLaunchKernelGGL(k_band_inverse, dim3(nb), dim3(TPB), 0, s, nb, kb, B, Linv, X);
+    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT);
     VF_HIP(hipGetLastError());
 }
 
 template <typename T>
-void launch_coarse_chol_apply(int nred, const double *X, const int *redToFull, int nnodes, const T *b, T *x,
-                              double *tmp_in, double *tmp_out, hipStream_t s) {
+void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes,
+                              const T *b, T *x, double *tmp_in, double *tmp_out, hipStream_t s) {
     const int ld = (nred + NB - 1) / NB * NB;
     hipLaunchKernelGGL(k_cfill<T>, dim3(nblk(3L * nnodes)), dim3(TPB), 0, s, (size_t)3 * nnodes, x, T(0));
     if (nred == 0) return;
     hipLaunchKernelGGL(k_cgather<T>, dim3(nblk(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, b, tmp_in);
     hipLaunchKernelGGL(k_trmv_lower, dim3((nred + 3) / 4), dim3(TPB), 0, s, nred, ld, X, tmp_in, tmp_out);
-    hipLaunchKernelGGL(k_trmv_upper_t, dim3((nred + 63) / 64), dim3(TPB), 0, s, nred, ld, X, tmp_out, tmp_in);
+    hipLaunchKernelGGL(k_trmv_upper, dim3((nred + 3) / 4), dim3(TPB), 0, s, nred, ld, XT, tmp_out, tmp_in);
     hipLaunchKernelGGL(k_cscatter<T>, dim3(nblk(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, tmp_in, x);
     VF_HIP(hipGetLastError());
 }
 
-template void launch_coarse_chol_apply<double>(int, const double *, const int *, int, const double *, double *,
-                                               double *, double *, hipStream_t);
-template void launch_coarse_chol_apply<float>(int, const double *, const int *, int, const float *, float *,
-                                              double *, double *, hipStream_t);
+template void launch_coarse_chol_apply<double>(int, const double *, const double *, const int *, int, const double *,
+                                               double *, double *, double *, hipStream_t);
+template void launch_coarse_chol_apply<float>(int, const double *, const double *, const int *, int, const float *,
+                                              float *, double *, double *, hipStream_t);
 
 }  // namespace vf

@@ -531,6 +531,11 @@ void launch_dot2_active(const Grid &g, const double *a, const double *b, const d
     VF_HIP(hipGetLastError());
 }
 
+void launch_final_sum(const double *partials, int count, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_final_sum, dim3(1), dim3(FTPB), 0, s, partials, count, 0, 1, out);
+    VF_HIP(hipGetLastError());
+}
+
 void launch_dot_full(size_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s) {
     if (!n) { launch_fill<double>(1, out, 0.0, s); return; }
     const int cnt = reduction_partials_full(n);

@@ -95,12 +95,15 @@ __device__ __forceinline__ void walsh_apply(const T *__restrict__ C, T Ee, T v[8
         for (int d = 0; d < 3; ++d) v[m][d] = zh[m][d];
 }
 
+// dotp (SET mode, optional): per-workgroup partial sums of <u, y> over the owned nodes, in a fixed order.
 template <typename T, int MODE>
 __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ C, const T *__restrict__ E,
                                                   const T *__restrict__ u, T *__restrict__ y, const T *__restrict__ b,
-                                                  const uint8_t *__restrict__ dmask, int ntZ, int xchunk) {
+                                                  const uint8_t *__restrict__ dmask, int ntZ, int xchunk,
+                                                  double *__restrict__ dotp) {
     __shared__ T su[2][3 * RN * RN];  // node planes, double-buffered
     __shared__ T sp[4][3 * NTW];      // per element column: its values at the four corners of the plane
+    __shared__ double sred[NTW / 64];
     const int N = g.nnodes, nx = g.nn[0], ny = g.nn[1], nz = g.nn[2];
     const int tiles = gridDim.x;
     const int logical = xcd_remap(blockIdx.x + blockIdx.y * tiles, tiles * gridDim.y);
@@ -156,6 +159,7 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
 
     T uf[4][3];       // u at the element's face on the current node plane
     T carry[4][3];    // contributions of the previous element to the current node plane
+    double dacc = 0;  // <u, y> over the owned nodes (SET mode with dotp)
 #pragma unroll
     for (int q = 0; q < 4; ++q) carry[q][0] = carry[q][1] = carry[q][2] = T(0);
     fetch_plane(x0 - 1);
@@ -187,6 +191,8 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
         __syncthreads();
         T ub[4][3];
         read_face(buf, ub);
+        // u at the owned node (Y, Z) of plane ex: corner (1,1) of the element's face i = 0
+        const T un0 = uf[3][0], un1 = uf[3][1], un2 = uf[3][2];
         T v[8][3];
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -219,7 +225,18 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
                 T out = MODE == AM_RESIDUAL ? bv[c] - f : f;
                 if (dm & (1 << c)) out = T(0);
                 y[(size_t)c * N + n] = out;
+                if (MODE == AM_SET && dotp) dacc += (double)out * (double)(c == 0 ? un0 : (c == 1 ? un1 : un2));
             }
+        }
+    }
+    if (MODE == AM_SET && dotp) {
+        for (int o = 32; o > 0; o >>= 1) dacc += __shfl_xor(dacc, o, 64);
+        if ((t & 63) == 0) sred[t >> 6] = dacc;
+        __syncthreads();
+        if (t == 0) {
+            double r = 0;
+            for (int i = 0; i < NTW / 64; ++i) r += sred[i];
+            dotp[blockIdx.y * gridDim.x + blockIdx.x] = r;
         }
     }
 }
@@ -255,8 +272,10 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 template <typename T>
 void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
-                             const uint8_t *dmask, int mode, hipStream_t s) {
+                             const uint8_t *dmask, int mode, hipStream_t s, double *dotPartials, double *dotOut) {
     if (mode != AM_SET && mode != AM_RESIDUAL) throw std::runtime_error("Walsh K u supports the SET and RESIDUAL modes");
+    if (dotOut && (mode != AM_SET || !dotPartials)) throw std::runtime_error("the fused <u, K u> needs SET mode and partials");
+    if (dotOut && g.ydet <= 0) launch_fill<double>(1, dotOut, 0.0, s);
     if (g.ydet > 0) {
         // tiles cover node columns [TN*t, TN*t + TN); rows beyond ydet are skipped by the owners
         const int ntY = cdiv(g.ydet, TN), ntZ = cdiv(g.nn[2], TN);
@@ -268,9 +287,12 @@ void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, 
         if (chunks > 65535) throw std::runtime_error("grid too large for the Walsh K u");
         const dim3 grid(tiles, chunks);
         if (mode == AM_RESIDUAL)
-            hipLaunchKernelGGL((k_ku_walsh<T, AM_RESIDUAL>), grid, dim3(NTW), 0, s, g, C, E, u, y, b, dmask, ntZ, xchunk);
+            hipLaunchKernelGGL((k_ku_walsh<T, AM_RESIDUAL>), grid, dim3(NTW), 0, s, g, C, E, u, y, b, dmask, ntZ, xchunk,
+                               nullptr);
         else
-            hipLaunchKernelGGL((k_ku_walsh<T, AM_SET>), grid, dim3(NTW), 0, s, g, C, E, u, y, b, dmask, ntZ, xchunk);
+            hipLaunchKernelGGL((k_ku_walsh<T, AM_SET>), grid, dim3(NTW), 0, s, g, C, E, u, y, b, dmask, ntZ, xchunk,
+                               dotOut ? dotPartials : nullptr);
+        if (dotOut) launch_final_sum(dotPartials, (int)(grid.x * grid.y), dotOut, s);
     }
     if (mode == AM_SET && g.ydet < g.nn[1]) {
         const long cz = (long)g.nn[0] * (g.nn[1] - g.ydet) * g.nn[2];
@@ -280,8 +302,8 @@ void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, 
 }
 
 template void launch_fine_apply_walsh<double>(const Grid &, const double *, const double *, const double *, double *,
-                                              const double *, const uint8_t *, int, hipStream_t);
+                                              const double *, const uint8_t *, int, hipStream_t, double *, double *);
 template void launch_fine_apply_walsh<float>(const Grid &, const float *, const float *, const float *, float *,
-                                             const float *, const uint8_t *, int, hipStream_t);
+                                             const float *, const uint8_t *, int, hipStream_t, double *, double *);
 
 }  // namespace vf
