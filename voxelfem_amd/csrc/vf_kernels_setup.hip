@@ -146,44 +146,56 @@ __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *_
 // ---- level-1 block rows straight from the fine moduli: the block of coarse node n and neighbour slot
 // is sum over the coarse elements holding both nodes (unmasked: ey < ymask) of sum_fi E_child(fi) *
 // Chat_fi[(3 ln + a), (3 lm + c)], i.e. the assembled form of the level-1 operator that
-// MultigridSolver.hh:483-491 applies on the fly. Colour-major output layout (BlockSlot); accumulated in
-// FP64, stored in the precision of the cycle that reads it.
+// MultigridSolver.hh:483-491 applies on the fly. One launch covers every (colour class, slot): a
+// work-item owns one node of one class, so the colour-major output (BlockSlot) is written contiguously;
+// the slot and the element offset fix ln and lm for the whole workgroup, so the Chat entries are uniform
+// (scalar) loads and only the child moduli are per work-item. Sums in FP64, stored in OutT.
 template <typename OutT>
 __global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, const double *__restrict__ Chat,
                                                             const double *__restrict__ Ef,
                                                             OutT *__restrict__ blocks) {
-    const long t = (long)blockIdx.x * TPB + threadIdx.x;
-    if (t >= (long)gc.nnodes * 27) return;
-    const int n = (int)(t % gc.nnodes), slot = (int)(t / gc.nnodes);
-    const int nz = gc.nn[2], ix = n / (gc.nn[1] * nz), iy = (n / nz) % gc.nn[1], iz = n % nz;
-    const int off[3] = {slot / 9 - 1, (slot / 3) % 3 - 1, slot % 3 - 1};
-    const int id[3] = {ix, iy, iz};
+    const int slot = blockIdx.y, colour = blockIdx.z;
+    const int cx = (colour >> 2) & 1, cy = (colour >> 1) & 1, cz = colour & 1;
+    const int mx = (gc.nn[0] - cx + 1) / 2, my = (gc.nn[1] - cy + 1) / 2, mz = (gc.nn[2] - cz + 1) / 2;
+    const int cnt = mx * my * mz;
+    const int j = blockIdx.x * TPB + threadIdx.x;
+    if (j >= cnt) return;
+    const int jx = j / (my * mz), r = j - jx * my * mz, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int ox = slot / 9 - 1, oy = (slot / 3) % 3 - 1, oz = slot % 3 - 1;
+    const int eyEnd = min(gc.ne[1], gc.ymask);
+    const int fyz = gf.ne[1] * gf.ne[2];
     double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    bool inside = true;
-    for (int d = 0; d < 3; ++d) inside &= (unsigned)(id[d] + off[d]) < (unsigned)gc.nn[d];
-    if (inside) {
-        int lo[3], hi[3];
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = max(max(id[d] - 1, id[d] + off[d] - 1), 0);
-            hi[d] = min(min(id[d], id[d] + off[d]), gc.ne[d] - 1);
+    for (int e = 0; e < 8; ++e) {
+        // element at node - (i, jj, k); it holds m = n + o iff every local coordinate i + o stays in {0, 1}
+        const int i = (e >> 2) & 1, jj = (e >> 1) & 1, k = e & 1;
+        if (i + ox < 0 || i + ox > 1 || jj + oy < 0 || jj + oy > 1 || k + oz < 0 || k + oz > 1) continue;
+        const int ex = ix - i, ey = iy - jj, ez = iz - k;
+        if ((unsigned)ex >= (unsigned)gc.ne[0] || (unsigned)ey >= (unsigned)eyEnd || (unsigned)ez >= (unsigned)gc.ne[2])
+            continue;
+        const int ln = e, lm = 4 * (i + ox) + 2 * (jj + oy) + (k + oz);
+        const size_t fb = ((size_t)(2 * ex) * gf.ne[1] + 2 * ey) * gf.ne[2] + 2 * ez;
+        double Ec[8];
+#pragma unroll
+        for (int fi = 0; fi < 8; ++fi) Ec[fi] = Ef[fb + ((fi >> 2) & 1) * fyz + ((fi >> 1) & 1) * gf.ne[2] + (fi & 1)];
+#pragma unroll
+        for (int fi = 0; fi < 8; ++fi) {
+            const double *C = Chat + fi * 576 + (3 * ln) * 24 + 3 * lm;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) B[3 * a + c] += Ec[fi] * C[a * 24 + c];
         }
-        hi[1] = min(hi[1], gc.ymask - 1);
-        for (int ex = lo[0]; ex <= hi[0]; ++ex)
-            for (int ey = lo[1]; ey <= hi[1]; ++ey)
-                for (int ez = lo[2]; ez <= hi[2]; ++ez) {
-                    const int ln = 4 * (ix - ex) + 2 * (iy - ey) + (iz - ez);
-                    const int lm = 4 * (ix + off[0] - ex) + 2 * (iy + off[1] - ey) + (iz + off[2] - ez);
-                    for (int fi = 0; fi < 8; ++fi) {
-                        const int fx = 2 * ex + ((fi >> 2) & 1), fy = 2 * ey + ((fi >> 1) & 1), fz = 2 * ez + (fi & 1);
-                        const double Ee = Ef[(fx * gf.ne[1] + fy) * gf.ne[2] + fz];
-                        const double *C = Chat + fi * 576 + (3 * ln) * 24 + 3 * lm;
-                        for (int a = 0; a < 3; ++a)
-                            for (int c = 0; c < 3; ++c) B[3 * a + c] += Ee * C[a * 24 + c];
-                    }
-                }
     }
-    const BlockSlot bs = block_slot(gc, ix, iy, iz);
-    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = (OutT)B[k];
+    size_t before = 0;
+    for (int q = 0; q < colour; ++q) {
+        const long qx = (gc.nn[0] - ((q >> 2) & 1) + 1) / 2, qy = (gc.nn[1] - ((q >> 1) & 1) + 1) / 2,
+                   qz = (gc.nn[2] - (q & 1) + 1) / 2;
+        before += (size_t)(qx * qy * qz);
+    }
+    OutT *out = blocks + before * 243 + (size_t)slot * 9 * cnt + j;
+#pragma unroll
+    for (int q = 0; q < 9; ++q) out[(size_t)q * cnt] = (OutT)B[q];
 }
 
 }  // namespace
@@ -215,8 +227,9 @@ void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipSt
 template <typename OutT>
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
                                hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble_l1_blocks<OutT>, dim3(nblocks((long)gc.nnodes * 27)), dim3(TPB), 0, s, gc, gf, Chat,
-                       Ef, blocks);
+    const long maxClass = (long)((gc.nn[0] + 1) / 2) * ((gc.nn[1] + 1) / 2) * ((gc.nn[2] + 1) / 2);
+    hipLaunchKernelGGL(k_assemble_l1_blocks<OutT>, dim3(nblocks(maxClass), 27, 8), dim3(TPB), 0, s, gc, gf, Chat, Ef,
+                       blocks);
     VF_HIP(hipGetLastError());
 }
 
