@@ -399,8 +399,9 @@ __device__ __forceinline__ int opaque(int v) {
 
 // One x-parity of the 8-colour block Gauss-Seidel sweep (MultigridSolver.hh:408-458). A workgroup owns
 // a (y, z) tile of every plane of that parity in its x-chunk and steps through them. Per plane:
-//  1. the loads of the next plane step (two u planes, two element planes) are issued into registers,
-//     so their latency overlaps steps 2-4;
+//  1. the step's two new u planes and two element planes are loaded into LDS at the end of the previous
+//     step (FP32: four workgroups share a CU and cover each other's loads) or requested into registers at
+//     its start (FP64: two workgroups per CU);
 //  2. for every node the four colours will update (up to three per work-item) the row part that the
 //     colour steps do not change is evaluated in parallel and kept in registers as b - (K u)_fixed: the
 //     two neighbour planes (not touched by this phase), the node itself, and the in-plane neighbours
@@ -580,9 +581,11 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
     for (int kp = k0; kp < k1; ++kp) {
         const int ix = parity + 2 * kp;
         const bool more = kp + 1 < k1;
-        // 1. next plane step's inputs into registers
+        // FP64: two workgroups per CU (LDS), too few to cover each other's loads, so the next step's planes
+        // are requested here into registers and stored at the end of the step
+        constexpr bool kPrefetch = sizeof(T) == 8;
         T pc[NU], pb[NU], pe0[NE], pe1[NE];
-        if (more) {
+        if (kPrefetch && more) {
             fetch_u(rin, ix + 2, pc);
             fetch_u(rnb, ix + 3, pb);
             fetch_E(ix + 1, pe0);
@@ -725,6 +728,14 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
             uout[(size_t)c * N + ((size_t)ix * ny + y0 + oy) * nz + z0 + oz] = pcur[c * CS + (oy + HY) * RZM + oz + HZ];
         }
         if (more) {
+            // FP32: the next plane step's inputs are loaded now; the other workgroups on the CU keep its
+            // SIMDs busy meanwhile
+            if (!kPrefetch) {
+                fetch_u(rin, ix + 2, pc);
+                fetch_u(rnb, ix + 3, pb);
+                fetch_E(ix + 1, pe0);
+                fetch_E(ix + 2, pe1);
+            }
             __syncthreads();
             // plane ix+1 becomes the next step's plane ix'-1; the freed slot takes plane ix'+1
             const int tmp = sA;
