@@ -350,6 +350,65 @@ __global__ void __launch_bounds__(TPB) k_block_apply(Grid g, const T *__restrict
     store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
+// One colour of the block Gauss-Seidel sweep on a small level, four work-items per node: work-item
+// g < 3 of a node gathers the nine block rows of neighbour plane x + g - 1, the partial rows are summed
+// across the four with two lane exchanges, and work-item 1 (which holds the diagonal block) updates the
+// node. The per-node dependency chain is a third of the one-work-item form, which bounds these
+// latency-limited launches.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_block_gs_x4(Grid g, const T *__restrict__ blocks, T *__restrict__ u,
+                                                     const T *__restrict__ b, const uint8_t *__restrict__ dmask,
+                                                     int cx, int cy, int cz, int forward) {
+    const int mx = (g.nn[0] - cx + 1) / 2, my = (g.ydet - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    const int node = t >> 2, part = t & 3;
+    const bool valid = node < mx * my * mz;
+    const int jx = valid ? node / (my * mz) : 0, r = valid ? node - jx * my * mz : 0, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int N = g.nnodes, n = (ix * g.nn[1] + iy) * g.nn[2] + iz;
+    const uint8_t dm = valid ? (dmask ? dmask[n] : uint8_t(0)) : uint8_t(7);
+    T f[3] = {0, 0, 0};
+    T M[9];
+    if (dm != 7 && part < 3) {
+        const BlockSlot bs = block_slot(g, ix, iy, iz);
+        const T *__restrict__ bp = blocks + bs.base + bs.j;
+        const size_t st = (size_t)bs.cnt;
+        const int sx = g.nn[1] * g.nn[2], sy = g.nn[2];
+        const int dx = part - 1;
+        const bool okx = (unsigned)(ix + dx) < (unsigned)g.nn[0];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int s = 9 * part + q, dy = q / 3 - 1, dz = q % 3 - 1;
+            const bool ok = okx && (unsigned)(iy + dy) < (unsigned)g.nn[1] && (unsigned)(iz + dz) < (unsigned)g.nn[2];
+            const int m = n + dx * sx + dy * sy + dz;
+            T un[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) un[c] = ok ? u[(size_t)c * N + m] : T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                T acc = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T B = bp[(size_t)(s * 9 + 3 * a + c) * st];
+                    acc += B * un[c];
+                    if (q == 4) M[3 * a + c] = B;  // slot 13 (the node itself) when part == 1
+                }
+                f[a] += acc;
+            }
+        }
+    }
+    // every lane takes part in the exchanges (idle ones contribute zeros)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f[c] += __shfl_xor(f[c], 1, 64);
+        f[c] += __shfl_xor(f[c], 2, 64);
+    }
+    if (dm != 7 && part == 1) {
+        const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
+        gs_update(u, N, n, bms, M, dm, forward);
+    }
+}
+
 template <typename T, int GROUP = 3>
 __global__ void __launch_bounds__(TPB) k_block_gs(Grid g, const T *__restrict__ blocks, T *__restrict__ u,
                                                   const T *__restrict__ b, const uint8_t *__restrict__ dmask, int cx,
@@ -522,6 +581,8 @@ void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef,
 
 // levels below this many nodes use the latency-oriented grouping of block_row
 constexpr int SMALL_LEVEL = 65536;
+// levels below this many nodes smooth with four work-items per node (latency-bound launches)
+constexpr int GS_X4_LEVEL = 1 << 19;
 
 template <typename T>
 void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
@@ -580,7 +641,10 @@ void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uin
     const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
     const long count = color_count(g, cx, cy, cz);
     if (!count) return;
-    if (g.nnodes < SMALL_LEVEL)
+    if (g.nnodes < GS_X4_LEVEL)
+        hipLaunchKernelGGL(k_block_gs_x4<T>, dim3(nblocks(4 * count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy,
+                           cz, forward);
+    else if (g.nnodes < SMALL_LEVEL)
         hipLaunchKernelGGL((k_block_gs<T, 9>), dim3(nblocks(count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy, cz,
                            forward);
     else
