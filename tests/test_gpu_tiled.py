@@ -190,3 +190,32 @@ def test_lbl_batched_matches_sequential_and_oracle(vf, prec):
     tol = 1e-9 if prec == "fp64" else 1e-8
     assert ev.objective() == pytest.approx(obj, rel=tol)
     assert rel(ev.gradient(), grad) < 10 * tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_lbl_batched_on_reused_nan_memory(vf, prec):
+    """Regression: the layer solves read detached rows of the PCG direction vector (times zero moduli), so
+    those rows must be initialised even when freshly allocated device memory holds NaN bit patterns. The
+    device memory the pipelines will allocate is first filled with NaN and handed back to the runtime."""
+    import torch
+
+    pv = vf.pyVoxelFEM
+    ne = (16, 12, 8)
+    rho = wl.noise_density(ne, 7)
+    full = wl.configure(pv, ne, None, rho)
+    lbl_sim = full.getIntermediateFabricationShape(1.0, False, pv.InterpolationLaw.SIMP)
+    mg = lbl_sim.multigridSolver(2)
+    ev = pv.LayerByLayerEvaluator(lbl_sim)
+    ev.selectInitMethod("zero")
+    ev.precision = prec
+    ev.batched = False
+    ev.run(mg, True, 1, 500, 1e-10)
+    obj_seq = ev.objective()
+    junk = torch.full((1 << 26,), float("nan"), dtype=torch.float64, device="cuda:0")  # 512 MiB
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    ev.batched, ev.pipelines, ev.useGraphs = True, 3, True
+    ev.run(mg, True, 1, 500, 1e-10)
+    assert np.isfinite(ev.objective())
+    assert ev.objective() == pytest.approx(obj_seq, rel=1e-13)

@@ -1054,6 +1054,9 @@ struct vf_mg {
         // direction update (and once after the loop), and the residual update also writes the FP32
         // right-hand side of the next cycle.
         const bool fused = !cb && mgIter > 0 && nu > 0 && L > 0;
+        // the direction updates cover the active rows only; the element products also read the detached
+        // rows (with zero moduli), which must therefore hold finite values
+        if (fused) launch_fill<double>(n0, d, 0.0, s);
         float *r32 = nullptr;
         if (fused && prec == VF_PREC_FP32) {
             ensure_buffers<float>(0);

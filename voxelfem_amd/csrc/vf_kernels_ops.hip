@@ -327,6 +327,51 @@ __device__ __forceinline__ void block_apply_row(const Grid &g, const T *__restri
     store_out(y, b, g.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
+// Block-row product on a small level with four work-items per node (see k_block_gs_x4); work-item 0
+// writes the row.
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_block_apply_x4(Grid g, const T *__restrict__ blocks, const T *__restrict__ u,
+                                                        T *__restrict__ y, const T *__restrict__ b,
+                                                        const uint8_t *__restrict__ dmask) {
+    const int nz = g.nn[2], slab = g.ydet * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    const int node = t >> 2, part = t & 3;
+    const bool valid = node < g.nn[0] * slab;
+    const int ix = valid ? node / slab : 0, r = valid ? node - ix * slab : 0, iy = r / nz, iz = r - iy * nz;
+    const int N = g.nnodes, n = (ix * g.nn[1] + iy) * nz + iz;
+    T f[3] = {0, 0, 0};
+    if (valid && part < 3) {
+        const BlockSlot bs = block_slot(g, ix, iy, iz);
+        const T *__restrict__ bp = blocks + bs.base + bs.j;
+        const size_t st = (size_t)bs.cnt;
+        const int sx = g.nn[1] * nz;
+        const int dx = part - 1;
+        const bool okx = (unsigned)(ix + dx) < (unsigned)g.nn[0];
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            const int s = 9 * part + q, dy = q / 3 - 1, dz = q % 3 - 1;
+            const bool ok = okx && (unsigned)(iy + dy) < (unsigned)g.nn[1] && (unsigned)(iz + dz) < (unsigned)nz;
+            const int m = n + dx * sx + dy * nz + dz;
+            T un[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) un[c] = ok ? u[(size_t)c * N + m] : T(0);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                T acc = 0;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) acc += bp[(size_t)(s * 9 + 3 * a + c) * st] * un[c];
+                f[a] += acc;
+            }
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        f[c] += __shfl_xor(f[c], 1, 64);
+        f[c] += __shfl_xor(f[c], 2, 64);
+    }
+    if (valid && part == 0) store_out(y, b, N, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
 template <typename T, int MODE>
 __global__ void __launch_bounds__(TPB) k_block_apply_g9(Grid g, const T *__restrict__ blocks, const T *__restrict__ u,
                                                         T *__restrict__ y, const T *__restrict__ b,
@@ -588,7 +633,9 @@ template <typename T>
 void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
                         hipStream_t s) {
     const long count = (long)g.nn[0] * g.ydet * g.nn[2];
-    if (count > 0 && g.nnodes < SMALL_LEVEL) {
+    if (count > 0 && g.nnodes < GS_X4_LEVEL) {
+        VF_DISPATCH_MODE(k_block_apply_x4, T, mode, dim3(nblocks(4 * count)), g, blocks, u, y, b, dmask);
+    } else if (count > 0 && g.nnodes < SMALL_LEVEL) {
         VF_DISPATCH_MODE(k_block_apply_g9, T, mode, dim3(nblocks(count)), g, blocks, u, y, b, dmask);
     } else if (count > 0) {
         VF_DISPATCH_MODE(k_block_apply, T, mode, dim3(nblocks(count)), g, blocks, u, y, b, dmask);
