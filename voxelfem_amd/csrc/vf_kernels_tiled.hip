@@ -335,10 +335,11 @@ __device__ __forceinline__ T side_sum(const T X[2][2], int dy, int dz) {
 // offsets of group g (bit 1: dy != 1, bit 0: dz != 1) selected by MASK (bit 3 dy + dz) add
 // W[c][d] u[d] to V[c][d]. Xp / Xs are the plane's element moduli combined along x without / with
 // the node's x side (for the diagonal and yz weights / for the xy and xz weights). p points at the
-// node in the plane (component stride CS, row stride RZ).
+// node in the plane (component stride CS, row stride RZ); zm / zp are the offsets of its z - 1 / z + 1
+// neighbours (the plane rows are stored by z parity, so they depend on the node's parity).
 template <typename T, int MASK>
-__device__ __forceinline__ void plane_sums(const T *p, int CS, int RZ, const T Xp[2][2], const T Xs[2][2], int g,
-                                           T V[9]) {
+__device__ __forceinline__ void plane_sums(const T *p, int CS, int RZ, int zm, int zp, const T Xp[2][2],
+                                           const T Xs[2][2], int g, T V[9]) {
 #pragma unroll
     for (int dy = 0; dy < 3; ++dy)
 #pragma unroll
@@ -348,7 +349,7 @@ __device__ __forceinline__ void plane_sums(const T *p, int CS, int RZ, const T X
             const T wxy = side_sum<T, true, false>(Xs, dy, dz);
             const T wxz = side_sum<T, false, true>(Xs, dy, dz);
             const T wyz = side_sum<T, true, true>(Xp, dy, dz);
-            const int off = (dy - 1) * RZ + (dz - 1);
+            const int off = (dy - 1) * RZ + (dz == 0 ? zm : (dz == 2 ? zp : 0));
             const T u0 = p[off], u1 = p[CS + off], u2 = p[2 * CS + off];
             V[0] += w * u0;
             V[4] += w * u1;
@@ -374,8 +375,8 @@ __device__ __forceinline__ void contract_group(const T *__restrict__ F, int delt
 // f += the in-plane part of a node row over the neighbours selected by MASK (X = E0 + E1, Xs = E1 - E0
 // over the two element layers), one delta group at a time.
 template <typename T, int MASK>
-__device__ __forceinline__ void in_plane_part(const T *__restrict__ F, const T *p, int CS, int RZ, const T X[2][2],
-                                              const T Xs[2][2], T f[3]) {
+__device__ __forceinline__ void in_plane_part(const T *__restrict__ F, const T *p, int CS, int RZ, int zm, int zp,
+                                              const T X[2][2], const T Xs[2][2], T f[3]) {
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
         constexpr int kGroupMask[4] = {1 << 4, (1 << 3) | (1 << 5), (1 << 1) | (1 << 7), 1 | 4 | 64 | 256};
@@ -383,7 +384,7 @@ __device__ __forceinline__ void in_plane_part(const T *__restrict__ F, const T *
         T V[9];
 #pragma unroll
         for (int q = 0; q < 9; ++q) V[q] = T(0);
-        plane_sums<T, MASK>(p, CS, RZ, X, Xs, g, V);
+        plane_sums<T, MASK>(p, CS, RZ, zm, zp, X, Xs, g, V);
         contract_group(F, g, V, f);
         asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]));
         __builtin_amdgcn_sched_barrier(0);
@@ -413,15 +414,21 @@ __device__ __forceinline__ int opaque(int v) {
 // go through buffer loads whose offsets are precomputed once per work-item (out-of-grid entries get an
 // out-of-range offset, which the buffer unit returns as zero).
 template <typename T, int TY, int TZ, int NT, bool FWD>
-__global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F, const T *__restrict__ E,
+__global__ void __launch_bounds__(NT, sizeof(T) == 4 ? 4 : 1) k_gs_phase(Grid g, const T *__restrict__ F, const T *__restrict__ E,
                                                  const T *__restrict__ uin, const T *__restrict__ unb,
                                                  T *__restrict__ uout, const T *__restrict__ b,
                                                  const uint8_t *__restrict__ dmask, int parity, int ntY, int ntZ,
                                                  int pchunk) {
     constexpr int HY = 2, HZ = 4;                          // u halo: colour halo + one neighbour
     constexpr int RYM = TY + 2 * HY, RZM = TZ + 2 * HZ;
-    constexpr int EYM = TY + 3, EZM = TZ + 7;              // element rows [y0-2, y1], cols [z0-4, z1+2]
+    constexpr int EYM = TY + 3, EZM = TZ + 8;              // element rows [y0-2, y1], cols [z0-4, z1+2] (+1)
     constexpr int CS = RYM * RZM, ES = EYM * EZM;
+    // FP64: LDS rows hold the entries of even local z first, then the odd ones, so the nodes of one colour
+    // (every other z) sit at consecutive addresses and 8-byte reads do not conflict in the LDS banks.
+    // FP32 keeps natural rows (its reads conflict two-way at most; the split form costs it registers).
+    constexpr bool SPLIT = sizeof(T) == 8;
+    constexpr int HZ2 = RZM / 2, HEZ = EZM / 2;
+    static_assert(RZM % 2 == 0 && EZM % 2 == 0, "parity-split rows");
     constexpr int NU = (3 * CS + NT - 1) / NT, NE = (ES + NT - 1) / NT;
     constexpr int MAXN = (TY / 2 + 1) * (TZ / 2 + 3) + (TY / 2 + 1) * (TZ / 2 + 2) + (TY / 2) * (TZ / 2 + 1) +
                          (TY / 2) * (TZ / 2);
@@ -494,13 +501,19 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
     auto s_pos = [](int sl) { return (sl >> 16) - 1; };
     auto s_ly = [](int sl) { return (sl >> 8) & 255; };
     auto s_lz = [](int sl) { return sl & 255; };
+    // split-row LDS indices of node (ly, lz) and element (ly, lz), and the z-neighbour offsets of a node
+    auto uidx = [](int ly, int lz) { return ly * RZM + (SPLIT ? (lz & 1) * HZ2 + (lz >> 1) : lz); };
+    auto eidx = [](int ly, int lz) { return ly * EZM + (SPLIT ? (lz & 1) * HEZ + (lz >> 1) : lz); };
+    auto zoff_m = [](int lz) { return SPLIT ? ((lz & 1) ? -HZ2 : HZ2 - 1) : -1; };
+    auto zoff_p = [](int lz) { return SPLIT ? ((lz & 1) ? 1 - HZ2 : HZ2) : 1; };
 
     // per-work-item byte offsets of its tile entries inside one plane (OOB outside the grid)
     int uo[NU], eo[NE];
 #pragma unroll
     for (int i = 0; i < NU; ++i) {
         const int w = t + i * NT;
-        const int c = w / CS, r = w - c * CS, ly = r / RZM, lz = r - ly * RZM;
+        const int c = w / CS, r = w - c * CS, ly = r / RZM, rr = r - ly * RZM, par = rr / HZ2;
+        const int lz = SPLIT ? 2 * (rr - par * HZ2) + par : rr;
         const int gy = ry0 + ly, gz = rz0 + lz;
         const bool ok = w < 3 * CS && (unsigned)gy < (unsigned)ny && (unsigned)gz < (unsigned)nz;
         uo[i] = ok ? (c * N + gy * nz + gz) * (int)sizeof(T) : OOB;
@@ -508,7 +521,8 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
 #pragma unroll
     for (int i = 0; i < NE; ++i) {
         const int w = t + i * NT;
-        const int ly = w / EZM, lz = w - ly * EZM;
+        const int ly = w / EZM, rr = w - ly * EZM, par = rr / HEZ;
+        const int lz = SPLIT ? 2 * (rr - par * HEZ) + par : rr;
         const int gy = ey0 + ly, gz = ez0 + lz;
         const bool ok = w < ES && (unsigned)gy < (unsigned)g.ne[1] && (unsigned)gz < (unsigned)g.ne[2];
         eo[i] = ok ? (gy * g.ne[2] + gz) * (int)sizeof(T) : OOB;
@@ -614,15 +628,17 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
             constexpr int r = decltype(R)::value;
             if (slot[r] && dcur[r] != 7) {
                 const int ly = s_ly(slot[r]), lz = s_lz(slot[r]);
-                const int li = ly * RZM + lz;
-                const int ei = (ly + HY - 2) * EZM + (lz + HZ - 4);
+                const int li = uidx(ly, lz), zm = zoff_m(lz), zp = zoff_p(lz);
                 T e1[2][2], e0[2][2], e0n[2][2];
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
-                        e1[j][k] = sE[0][ei - j * EZM - k];  // x - 1 plane: elements i = 1, side +1
-                        e0[j][k] = sE[1][ei - j * EZM - k];  // x + 1 plane: elements i = 0, side -1
+                        // element local (ly - j, lz - k): the y and z halos of the node and element tiles
+                        // differ by exactly the element offset (HY = 2, HZ = 4)
+                        const int ei = eidx(ly - j, lz - k);
+                        e1[j][k] = sE[0][ei];  // x - 1 plane: elements i = 1, side +1
+                        e0[j][k] = sE[1][ei];  // x + 1 plane: elements i = 0, side -1
                         e0n[j][k] = -e0[j][k];
                     }
                 T f[3] = {0, 0, 0};
@@ -631,8 +647,8 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
                     T V[9];
 #pragma unroll
                     for (int q = 0; q < 9; ++q) V[q] = T(0);
-                    plane_sums<T, 511>(pm + li, CS, RZM, e1, e1, gg, V);
-                    plane_sums<T, 511>(pp + li, CS, RZM, e0, e0n, gg, V);
+                    plane_sums<T, 511>(pm + li, CS, RZM, zm, zp, e1, e1, gg, V);
+                    plane_sums<T, 511>(pp + li, CS, RZM, zm, zp, e0, e0n, gg, V);
                     contract_group(F, 4 | gg, V, f);
                     // one group at a time: keeps the neighbour values of later groups out of registers
                     asm volatile("" : "+v"(f[0]), "+v"(f[1]), "+v"(f[2]));
@@ -647,10 +663,10 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
                         Xs[j][k] = e1[j][k] - e0[j][k];
                     }
                 switch (s_pos(slot[r])) {
-                    case 0: in_plane_part<T, nbr_mask<FWD, false>(0)>(F, pcur + li, CS, RZM, X, Xs, f); break;
-                    case 1: in_plane_part<T, nbr_mask<FWD, false>(1)>(F, pcur + li, CS, RZM, X, Xs, f); break;
-                    case 2: in_plane_part<T, nbr_mask<FWD, false>(2)>(F, pcur + li, CS, RZM, X, Xs, f); break;
-                    default: in_plane_part<T, nbr_mask<FWD, false>(3)>(F, pcur + li, CS, RZM, X, Xs, f); break;
+                    case 0: in_plane_part<T, nbr_mask<FWD, false>(0)>(F, pcur + li, CS, RZM, zm, zp, X, Xs, f); break;
+                    case 1: in_plane_part<T, nbr_mask<FWD, false>(1)>(F, pcur + li, CS, RZM, zm, zp, X, Xs, f); break;
+                    case 2: in_plane_part<T, nbr_mask<FWD, false>(2)>(F, pcur + li, CS, RZM, zm, zp, X, Xs, f); break;
+                    default: in_plane_part<T, nbr_mask<FWD, false>(3)>(F, pcur + li, CS, RZM, zm, zp, X, Xs, f); break;
                 }
                 og[r][0] -= f[0];
                 og[r][1] -= f[1];
@@ -671,20 +687,20 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
                 const uint8_t dm = dcur[r];
                 const int ly = s_ly(slot[r]), lz = s_lz(slot[r]);
                 const T o[3] = {og[r][0], og[r][1], og[r][2]};
-                const int li = ly * RZM + lz;
-                const int ei = (ly + HY - 2) * EZM + (lz + HZ - 4);
+                const int li = uidx(ly, lz), zm = zoff_m(lz), zp = zoff_p(lz);
                 T X[2][2], Xs[2][2];
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
 #pragma unroll
                     for (int k = 0; k < 2; ++k) {
-                        const T e1 = sE[0][ei - j * EZM - k], e0 = sE[1][ei - j * EZM - k];
+                        const int ei = eidx(ly - j, lz - k);
+                        const T e1 = sE[0][ei], e0 = sE[1][ei];
                         X[j][k] = e0 + e1;
                         Xs[j][k] = e1 - e0;
                     }
                 T f[3] = {0, 0, 0};
                 constexpr int mask = nbr_mask<FWD, true>(pos);
-                if constexpr (mask != 0) in_plane_part<T, mask>(F, pcur + li, CS, RZM, X, Xs, f);
+                if constexpr (mask != 0) in_plane_part<T, mask>(F, pcur + li, CS, RZM, zm, zp, X, Xs, f);
                 T M[9];
                 {
                     const T w = side_sum<T, false, false>(X, 1, 1), wxy = side_sum<T, true, false>(Xs, 1, 1);
@@ -725,7 +741,7 @@ __global__ void __launch_bounds__(NT) k_gs_phase(Grid g, const T *__restrict__ F
         for (int w = t; w < 3 * TY * TZ; w += NT) {
             const int c = w / (TY * TZ), r = w - c * TY * TZ, oy = r / TZ, oz = r - oy * TZ;
             if (oy >= OY || oz >= OZ) continue;
-            uout[(size_t)c * N + ((size_t)ix * ny + y0 + oy) * nz + z0 + oz] = pcur[c * CS + (oy + HY) * RZM + oz + HZ];
+            uout[(size_t)c * N + ((size_t)ix * ny + y0 + oy) * nz + z0 + oz] = pcur[c * CS + uidx(oy + HY, oz + HZ)];
         }
         if (more) {
             // FP32: the next plane step's inputs are loaded now; the other workgroups on the CU keep its
