@@ -626,8 +626,9 @@ void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef,
 
 // levels below this many nodes use the latency-oriented grouping of block_row
 constexpr int SMALL_LEVEL = 65536;
-// levels below this many nodes smooth with four work-items per node (latency-bound launches)
-constexpr int GS_X4_LEVEL = 1 << 19;
+// levels below this many nodes smooth with four work-items per node (latency-bound launches); up to
+// GS_G9_LEVEL nodes, one work-item per node with nine block rows requested at a time
+constexpr int GS_X4_LEVEL = 1 << 19, GS_G9_LEVEL = 1 << 22;
 
 template <typename T>
 void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
@@ -691,7 +692,7 @@ void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uin
     if (g.nnodes < GS_X4_LEVEL)
         hipLaunchKernelGGL(k_block_gs_x4<T>, dim3(nblocks(4 * count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy,
                            cz, forward);
-    else if (g.nnodes < SMALL_LEVEL)
+    else if (g.nnodes < GS_G9_LEVEL)
         hipLaunchKernelGGL((k_block_gs<T, 9>), dim3(nblocks(count)), dim3(TPB), 0, s, g, blocks, u, b, dmask, cx, cy, cz,
                            forward);
     else
