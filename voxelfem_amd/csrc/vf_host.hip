@@ -1103,7 +1103,10 @@ struct vf_mg {
         GraphGuard graph;
         while (it < maxIter && rr > tol * tol * bb) {
             ++it;
-            if (mgIter > 0 && !updated) { update_stiffness(); updated = true; }
+            // the reference rebuilds the hierarchy at the first iteration of every solve
+            // (MultigridSolver.hh:1104-1107); here it is rebuilt only if the moduli changed since the last
+            // build (every change of densities, law, material or mask layers bumps the simulator's version)
+            if (mgIter > 0 && !updated) { ensure_stiffness(); updated = true; }
             if (useGraphs && !first && s != nullptr) {
                 if (!graph.exec) {
                     hipGraph_t gr = nullptr;
