@@ -145,6 +145,8 @@ def main():
 
     # fine-level K.u kernel, timed with HIP events on the simulator's stream
     tuned = {"fp64": capi.tuning(mg_h, capi.VF_PREC_FP64), "fp32": capi.tuning(mg_h, capi.VF_PREC_FP32)}
+    tuned1 = {"fp64": capi.tuning_level1(mg_h, capi.VF_PREC_FP64),
+              "fp32": capi.tuning_level1(mg_h, capi.VF_PREC_FP32)}
     ku_ms64 = capi.bench_mg_kernel(mg_h, 0, 0, args.ku_reps, capi.VF_PREC_FP64)
     ku_ms32 = capi.bench_mg_kernel(mg_h, 0, 0, args.ku_reps, capi.VF_PREC_FP32)
     bytes64 = 48.0 * nn + 8.0 * nel
@@ -203,6 +205,8 @@ def main():
         "fine_sweep": sweep,
         "fine_kernels": {p: {"K.u": t[0], "sweep": t[1], "tuner_ms": [round(v, 4) for v in t[2]]}
                          for p, t in tuned.items()},
+        "level1_operator": {p: {"residual": t[0], "sweep": t[1], "tuner_ms": [round(v, 4) for v in t[2]]}
+                            for p, t in tuned1.items()},
         "pcg_iterations": {"fp64": results["fp64"]["iterations"], "mixed": results["mixed"]["iterations"]},
         "final_rel_residual": {"fp64": results["fp64"]["rel_residual"], "mixed": results["mixed"]["rel_residual"]},
         "compliance": {"fp64": results["fp64"]["compliance"], "mixed": results["mixed"]["compliance"],

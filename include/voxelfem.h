@@ -211,6 +211,12 @@ int vf_set_kernel_variant(int variant);
  * ms[] = the autotuner's timings (gather, tiled, Walsh K u; gather, tiled, structured-gather sweep; 0
  * where it did not run). */
 int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]);
+/* Level-1 operator representation the solver uses in `precision` (running the autotuner if it has not
+ * run): choice[0] for the residual, choice[1] for the smoother (0 assembled 27-point block rows, 1 on the
+ * fly from the fine moduli); ms[] = the autotuner's timings (block residual, on-the-fly residual, block
+ * sweep, on-the-fly sweep; 0 where it did not run). Implementation detail of this library: the reference
+ * always applies level 1 on the fly (MultigridSolver.hh:483-491). */
+int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[4]);
 /* Replay the steady-state PCG iteration of vf_mg_pcg from a HIP graph captured on its second iteration
  * (enable = 1) instead of launching its kernels one by one (0, the default). Needs the solver's stream
  * to be a created stream, not the null stream; otherwise the iterations run eagerly. Same results. */

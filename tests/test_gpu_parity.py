@@ -74,12 +74,21 @@ def test_fine_applyK_masked_layers(pv):
     assert rel(y, ref) < 1e-12
 
 
-@pytest.mark.parametrize("levels", [1, 2, 3])
-def test_level_operators(pv, levels):
+@pytest.mark.parametrize("levels,variant", [(1, 0), (2, 0), (3, 0), (2, 1), (3, 1)])
+def test_level_operators(pv, levels, variant):
+    """Every level's K u, residual, one smoothing sweep, transfers and the coarse solve against the
+    oracle. variant 0 holds level 1 as assembled block rows; variant 1 applies and smooths level 1 on the
+    fly from the fine moduli (the reflection form of the coarsened element matrices)."""
+    from voxelfem_amd import capi
+
     ne = (16, 8, 16)
     tps, s = make_pair(pv, ne)
-    mg, om = tps.multigridSolver(levels), orc.MG(s, levels)
-    mg.updateStiffnessMatrices()
+    capi.set_kernel_variant(variant)
+    try:
+        mg, om = tps.multigridSolver(levels), orc.MG(s, levels)
+        mg.updateStiffnessMatrices()
+    finally:
+        capi.set_kernel_variant(0)
     om.update_stiffness()
     for l in range(levels + 1):
         nn = om.num_nodes(l)
@@ -99,6 +108,20 @@ def test_level_operators(pv, levels):
             assert rel(_interp(mg, l, xc, b, True), om.interpolate(l, xc, b, True)) < 1e-13
     bc_ = RNG.normal(size=(om.num_nodes(levels), 3))
     assert rel(_coarse(mg, bc_), om.coarse_solve(bc_)) < 1e-9
+
+
+@pytest.mark.parametrize("ne,levels", [((32, 16, 16), 2), ((24, 16, 16), 1)])
+def test_coarse_solve_many_band_blocks(pv, ne, levels):
+    """Coarsest direct solve whose reduced operator spans many 64-row band blocks (10 blocks with 2
+    off-diagonal block bands; 47 blocks with 5), so every stage of the blocked band factorisation and
+    of the strip-parallel inverse runs, against the oracle's dense Cholesky solve."""
+    tps, s = make_pair(pv, ne)
+    mg, om = tps.multigridSolver(levels), orc.MG(s, levels)
+    mg.updateStiffnessMatrices()
+    om.update_stiffness()
+    for _ in range(2):
+        bc_ = RNG.normal(size=(om.num_nodes(levels), 3))
+        assert rel(_coarse(mg, bc_), om.coarse_solve(bc_)) < 1e-9
 
 
 def _smooth(mg, l, u, b, fwd):

@@ -27,6 +27,7 @@ def lib():
         L.vf_bench_mg_kernel.argtypes = [vp, i32, i64, i64, i32, ctypes.POINTER(dbl)]
         L.vf_set_kernel_variant.argtypes = [i32]
         L.vf_mg_tuning.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(dbl)]
+        L.vf_mg_tuning_level1.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(dbl)]
         L.vf_sim_set_densities.argtypes = [vp, vp, i32]
         L.vf_sim_build_load.argtypes = [vp, vp, i32]
         L.vf_sim_compliance_gradient.argtypes = [vp, vp, vp, i32]
@@ -76,6 +77,16 @@ def tuning(mg_handle, precision=VF_PREC_FP64):
     ms = (ctypes.c_double * 6)()
     check(lib().vf_mg_tuning(ctypes.c_void_p(mg_handle), int(precision), ch, ms))
     return ("gather", "tiled", "walsh")[ch[0]], ("gather", "tiled", "gather-structured")[ch[1]], list(ms)
+
+
+def tuning_level1(mg_handle, precision=VF_PREC_FP64):
+    """(residual form, smoother form, timings ms [block residual, on-the-fly residual, block sweep,
+    on-the-fly sweep]) of level 1; a form is "blocks" or "on-the-fly"."""
+    ch = (ctypes.c_int * 2)()
+    ms = (ctypes.c_double * 4)()
+    check(lib().vf_mg_tuning_level1(ctypes.c_void_p(mg_handle), int(precision), ch, ms))
+    forms = ("blocks", "on-the-fly")
+    return forms[ch[0]], forms[ch[1]], list(ms)
 
 
 def bench_apply_K(sim_handle, reps, precision=VF_PREC_FP64):

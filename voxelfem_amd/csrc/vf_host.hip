@@ -643,10 +643,25 @@ struct vf_mg {
             const float tb = best([&] { launch_block_apply<T>(g1, blocksp<T>(1), u.data(), y.data(), rhs.data(), dm(1),
                                                               AM_RESIDUAL, s); });
             const float tf = best([&] { launch_l1_apply<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(),
-                                                           dm(1), AM_RESIDUAL, s); });
+                                                           dm(1), AM_RESIDUAL, sim->structured, s); });
             l1OnTheFly[p] = tf < tb;
             tuneMs[p][6] = tb;
             tuneMs[p][7] = tf;
+            // level 1 smoother: one forward sweep from the block rows or from the fine moduli
+            const size_t n1 = 3 * (size_t)g1.nnodes;
+            DBuf<T> u1(n1), b1(n1);
+            launch_fill<T>(n1, u1.data(), T(1e-3), s);
+            launch_fill<T>(n1, b1.data(), T(1), s);
+            const float sb = best([&] {
+                for (int c = 0; c < 8; ++c) launch_block_gs<T>(g1, blocksp<T>(1), u1.data(), b1.data(), dm(1), c, 1, s);
+            });
+            const float sf = best([&] {
+                for (int c = 0; c < 8; ++c)
+                    launch_l1_gs<T>(g1, g, Chatp<T>(), Ep<T>(), u1.data(), b1.data(), dm(1), c, 1, sim->structured, s);
+            });
+            l1GsOnTheFly[p] = sf < sb;
+            tuneMs[p][8] = sb;
+            tuneMs[p][9] = sf;
         }
         tuneMs[p][0] = ku[0];
         tuneMs[p][1] = ku[1];
@@ -655,8 +670,11 @@ struct vf_mg {
         tuneMs[p][4] = gsT;
         tuneMs[p][5] = gsF;
     }
-    float tuneMs[2][8] = {};
-    bool l1OnTheFly[2] = {false, false};  // level-1 residual on the fly instead of from the block rows
+    float tuneMs[2][10] = {};
+    bool l1OnTheFly[2] = {false, false};    // level-1 residual on the fly instead of from the block rows
+    bool l1GsOnTheFly[2] = {false, false};  // level-1 smoother on the fly instead of from the block rows
+    // level-1 block rows are read by nothing once the tuner put both level-1 operations on the fly
+    bool l1_blocks_unused(int p) const { return kuChoice[p] >= 0 && l1OnTheFly[p] && l1GsOnTheFly[p]; }
 
     // Ad = K d on the fine level (Dirichlet rows zero) and *dAd = <d, Ad> over the active nodes, fused into
     // the Walsh-basis product when that family is in use
@@ -669,7 +687,7 @@ struct vf_mg {
         else if (l == 0 && fam == 1) launch_fine_apply_tiled<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 1 && (!(l1Blocks && L >= 2) || l1OnTheFly[pidx<T>()]))
-            launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
+            launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, sim->structured, st());
         else if (l < L) launch_block_apply<T>(lv[l]->g, blocksp<T>(l), u, y, b, mask, mode, st());
         else {
             if constexpr (std::is_same<T, double>::value) {
@@ -707,8 +725,9 @@ struct vf_mg {
             if (l == 0 && gs_family<T>() == 2)
                 launch_fine_gs_gather_f<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
             else if (l == 0) launch_fine_gs<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
-            else if (l == 1 && !(l1Blocks && L >= 2))
-                launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, st());
+            else if (l == 1 && (!(l1Blocks && L >= 2) || l1GsOnTheFly[pidx<T>()]))
+                launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, sim->structured,
+                                st());
             else launch_block_gs<T>(lv[l]->g, blocksp<T>(l), u, b, dm(l), color, forward, st());
         }
     }
@@ -891,11 +910,11 @@ struct vf_mg {
         const int p = pidx<T>();
         if (blocksReady[p]) return;
         hipStream_t s = st();
-        if (L >= 2 && l1Blocks) {
+        if (L >= 2 && l1Blocks && !l1_blocks_unused(p)) {
             LevelData &c1 = *lv[1];
             DBuf<T> &buf = block_buf<T>(1);
             buf.resize((size_t)c1.g.nnodes * 243);
-            launch_assemble_l1_blocks<T>(c1.g, lv[0]->g, dChat.data(), sim->E.data(), buf.data(), s);
+            launch_assemble_l1_blocks<T>(c1.g, lv[0]->g, dChat.data(), sim->E.data(), buf.data(), sim->structured, s);
         }
         for (int l = 2; l < L; ++l) {
             LevelData &c = *lv[l];
@@ -2088,6 +2107,22 @@ int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]) {
         choice[1] = mg->gs_family<float>();
     }
     for (int i = 0; i < 6; ++i) ms[i] = mg->tuneMs[p][i];
+    VF_API_END
+}
+
+int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[4]) {
+    VF_API_BEGIN
+    need(mg && choice && ms, "null argument");
+    need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
+    mg->sync_masks();
+    mg->ensure_stiffness();
+    const int p = precision == VF_PREC_FP64 ? 0 : 1;
+    if (p == 0) mg->tune<double>();
+    else mg->tune<float>();
+    const bool blocks = mg->l1Blocks && mg->L >= 2;
+    choice[0] = blocks && !mg->l1OnTheFly[p] ? 0 : 1;
+    choice[1] = blocks && !mg->l1GsOnTheFly[p] ? 0 : 1;
+    for (int i = 0; i < 4; ++i) ms[i] = mg->tuneMs[p][6 + i];
     VF_API_END
 }
 

@@ -150,9 +150,11 @@ void launch_fine_gs_gather_f(const Grid &g, const T *F, const T *E, T *u, const 
 template <typename T>
 void launch_fine_gs_phase(const Grid &g, const T *K0, const T *E, const T *uin, const T *unb, T *uout, const T *b,
                           const uint8_t *dmask, int parity, int forward, hipStream_t s);
+// level-1 operator on the fly from the fine moduli (reflect: the element matrix is reflection-symmetric and
+// Chat holds child 0's matrix first, which then stands for all eight)
 template <typename T>
 void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
-                     const uint8_t *dmask, int mode, hipStream_t s);
+                     const uint8_t *dmask, int mode, bool reflect, hipStream_t s);
 // assembled 27-point block rows (levels >= 2)
 template <typename T>
 void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,
@@ -167,7 +169,7 @@ void launch_fine_gs(const Grid &g, const T *K0, const T *E, T *u, const T *b, co
                     int forward, hipStream_t s);
 template <typename T>
 void launch_l1_gs(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, T *u, const T *b, const uint8_t *dmask,
-                  int color, int forward, hipStream_t s);
+                  int color, int forward, bool reflect, hipStream_t s);
 template <typename T>
 void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uint8_t *dmask, int color, int forward,
                      hipStream_t s);
@@ -233,9 +235,10 @@ void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, 
 template <typename OutT>
 void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s);
 // level-1 block rows assembled from the fine moduli and the eight coarsened element matrices
+// (reflect: the element matrix is reflection-symmetric, so child 0's matrix gives all eight)
 template <typename OutT>
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
-                               hipStream_t s);
+                               bool reflect, hipStream_t s);
 // coarsest level: band Cholesky of the reduced operator (half-bandwidth kb blocks of 64) and the explicit
 // X = L^-1 (dense lower triangular, leading dimension n rounded up to 64) and its transpose XT (filled on
 // and above the diagonal); flag != 0 if not positive definite

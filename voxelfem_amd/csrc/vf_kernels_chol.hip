@@ -195,56 +195,65 @@ __global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, doub
         for (int j = 0; j < 4; ++j) Aij[(r0 + i) * NB + c0 + j] -= acc[i][j];
 }
 
-// ---- X = L^-1 (dense lower triangular, row-major, ld = nb*64): one workgroup per block column J.
-//      X_JJ = Linv_JJ; X_IJ = -Linv_II * sum_{q=1..min(kb,I-J)} L_{I,I-q} X_{I-q,J}.
+// ---- X = L^-1 (dense lower triangular, row-major, ld = nb*64), one workgroup per 16-column strip of a
+//      block column J: X_JJ = Linv_JJ; X_IJ = -Linv_II * sum_{q=1..min(kb,I-J)} L_{I,I-q} X_{I-q,J}.
+// Columns of X are independent, so the strips of one block column run in parallel and each workgroup's
+// sequential chain down the column is four times shorter in work than a whole block column's. Both products
+// are FP64 MFMA (v_mfma_f64_16x16x4f64): wave w owns rows 16w..16w+15 of the 64 x 16 strip; operand lane
+// layout A[i][k] at lane i + 16k, B[k][j] at lane j + 16k, result D[4 v + l / 16][l % 16] in element v of
+// lane l (the FP64 form interleaves the four result rows of a lane, unlike the FP32 16x16 forms). The operands come straight from global memory (L2-resident band blocks and earlier strips of X);
+// T = sum_q L X goes through LDS once so every wave sees all 64 of its rows as the K dimension of Linv T.
+typedef double dx4 __attribute__((ext_vector_type(4)));
+constexpr int SW = 16;
 __global__ void __launch_bounds__(TPB) k_band_inverse(int nb, int kb, const double *__restrict__ B,
                                                       const double *__restrict__ Linv, double *__restrict__ X) {
-    __shared__ double sP[NB][NB + 1];
-    __shared__ double sQ[NB][NB + 1];
-    const int J = blockIdx.x, t = threadIdx.x;
+    __shared__ double sT[NB][SW + 1];
+    const int J = blockIdx.x / (NB / SW), cs = (blockIdx.x % (NB / SW)) * SW;  // strip columns J*NB+cs ..
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
     const size_t ld = (size_t)nb * NB;
-    const int r0 = 4 * (t / 16), c0 = 4 * (t % 16);
-    auto Xblk = [&](int I) { return X + (size_t)I * NB * ld + (size_t)J * NB; };
+    const int col0 = J * NB + cs;
     {
         const double *Lj = Linv + (size_t)J * BB;
-        double *Xjj = Xblk(J);
-        for (int w = t; w < BB; w += TPB) Xjj[(size_t)(w / NB) * ld + w % NB] = Lj[w];
+        for (int e = t; e < NB * SW; e += TPB) {
+            const int r = e / SW, c = e % SW;
+            X[(size_t)(J * NB + r) * ld + col0 + c] = Lj[r * NB + cs + c];
+        }
     }
     __threadfence();
     __syncthreads();
+    const int rowA = 16 * w + li;
     for (int I = J + 1; I < nb; ++I) {
-        // T = sum_q L_{I,I-q} X_{I-q,J}   (T^T computed as gemm_nt with P = L, Q = X^T)
-        double acc[4][4] = {};
+        dx4 acc = {0.0, 0.0, 0.0, 0.0};
         const int qmax = min(kb, I - J);
         for (int q = 1; q <= qmax; ++q) {
-            const double *Lb = bandblk(B, kb, I, q);
-            const double *Xb = Xblk(I - q);
-            __syncthreads();
-            for (int w = t; w < BB; w += TPB) {
-                const int r = w / NB, c = w % NB;
-                sP[r][c] = Lb[w];
-                sQ[c][r] = Xb[(size_t)r * ld + c];  // sQ[col][m] = X[m][col]
+            const double *Lb = bandblk(B, kb, I, q) + rowA * NB + lk;
+            const double *Xb = X + ((size_t)(I - q) * NB + lk) * ld + col0 + li;
+            double a[16], b[16];
+#pragma unroll
+            for (int s = 0; s < 16; ++s) {
+                a[s] = Lb[4 * s];
+                b[s] = Xb[(size_t)(4 * s) * ld];
             }
-            __syncthreads();
-            gemm_nt_64(sP, sQ, t, acc);
+#pragma unroll
+            for (int s = 0; s < 16; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
         }
-        // X_IJ = -Linv_II T
+#pragma unroll
+        for (int v = 0; v < 4; ++v) sT[16 * w + 4 * v + lk][li] = acc[v];
         __syncthreads();
-        const double *Li = Linv + (size_t)I * BB;
-        for (int w = t; w < BB; w += TPB) sP[w / NB][w % NB] = Li[w];
+        const double *Li = Linv + (size_t)I * BB + rowA * NB + lk;
+        double a[16], b[16];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int s = 0; s < 16; ++s) {
+            a[s] = Li[4 * s];
+            b[s] = sT[4 * s + lk][li];
+        }
+        dx4 out = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-            for (int j = 0; j < 4; ++j) sQ[c0 + j][r0 + i] = acc[i][j];  // sQ[col][m] = T[m][col]
-        __syncthreads();
-        double out[4][4] = {};
-        gemm_nt_64(sP, sQ, t, out);
-        double *Xi = Xblk(I);
+        for (int s = 0; s < 16; ++s) out = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], out, 0, 0, 0);
+        double *Xi = X + ((size_t)I * NB + 16 * w + lk) * ld + col0 + li;
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) Xi[(size_t)(r0 + i) * ld + c0 + j] = -out[i][j];
-        // the block is read back by other work-items of this workgroup in the next kb steps
+        for (int v = 0; v < 4; ++v) Xi[(size_t)(4 * v) * ld] = -out[v];
+        // the strip is read back by the other waves in the next kb steps; sT is rewritten next step
         __threadfence();
         __syncthreads();
     }
@@ -344,7 +353,7 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
             hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B);
         }
     }
-    hipLaunchKernelGGL(k_band_inverse, dim3(nb), dim3(TPB), 0, s, nb, kb, B, Linv, X);
+    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X);
     hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT);
     VF_HIP(hipGetLastError());
 }

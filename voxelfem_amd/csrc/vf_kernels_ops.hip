@@ -271,6 +271,102 @@ __global__ void __launch_bounds__(TPB) k_l1_gs(Grid gc, Grid gf, const T *__rest
     gs_update(u, N, n, bms, M, dm, forward);
 }
 
+// Level-1 node row from the fine moduli when the element matrix is reflection-symmetric (box element,
+// orthotropic tensor): Chat_fi[(l, a), (m, c)] = S_fi(a) S_fi(c) Chat_0[(l ^ fi, a), (m ^ fi, c)] with
+// S_fi(a) = -1 iff fi has the bit of axis a set. With g = ln ^ fi and m' = m ^ fi the row of the centre
+// node (local index ln = e in element e) reads
+//   f[a] = sum_g sum_e E_{e, e^g} S(a) sum_{m', c} Chat_0[(g, a), (m', c)] S(c) u_{e, m' ^ e ^ g, c},
+// so each Chat_0 entry (a uniform, scalar operand) serves the eight elements in turn, the scalar working set
+// is one child's 24 x 24 matrix, and the signs are free source negations. Absent elements read zero moduli.
+template <typename T, bool WANT_M>
+__device__ __forceinline__ void l1_row_refl(const Grid &gc, const Grid &gf, const T *__restrict__ C0,
+                                            const T *__restrict__ Ef, const T *__restrict__ u, int ix, int iy,
+                                            int iz, T f[3], T M[9]) {
+    const int N = gc.nnodes, sx = gc.nn[1] * gc.nn[2], sy = gc.nn[2];
+    const int fz = gf.ne[2], fyz = gf.ne[1] * gf.ne[2];
+    const int n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
+    // one element at a time: its 24 nodal values and 8 child moduli are all a work-item holds besides the
+    // sums, which keeps the FP64 form at a few waves per SIMD (the node values come from L1/L2: the
+    // neighbouring work-items read the same nodes)
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+        const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
+        const int ex = ix - i, ey = iy - j, ez = iz - k;
+        const bool ok = (unsigned)ex < (unsigned)gc.ne[0] && (unsigned)ey < (unsigned)gc.ne[1] &&
+                        (unsigned)ez < (unsigned)gc.ne[2];
+        if (!ok) continue;
+        T ue[8][3];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int nm = n + (((m >> 2) & 1) - i) * sx + (((m >> 1) & 1) - j) * sy + ((m & 1) - k);
+#pragma unroll
+            for (int c = 0; c < 3; ++c) ue[m][c] = u[(size_t)c * N + nm];
+        }
+        const int fb = eidx(gf, 2 * ex, 2 * ey, 2 * ez);
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+            const int fi = e ^ g;
+            const T Efi = Ef[fb + ((fi >> 2) & 1) * fyz + ((fi >> 1) & 1) * fz + (fi & 1)];
+            T t[3] = {0, 0, 0};
+#pragma unroll
+            for (int mp = 0; mp < 8; ++mp)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T uv = ((fi >> (2 - c)) & 1) ? -ue[mp ^ fi][c] : ue[mp ^ fi][c];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) t[a] += C0[(3 * g + a) * 24 + 3 * mp + c] * uv;
+                }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) f[a] += (((fi >> (2 - a)) & 1) ? -Efi : Efi) * t[a];
+            if (WANT_M) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const bool neg = (bool)((fi >> (2 - a)) & 1) != (bool)((fi >> (2 - c)) & 1);
+                        M[3 * a + c] += (neg ? -Efi : Efi) * C0[(3 * g + a) * 24 + 3 * g + c];
+                    }
+            }
+        }
+    }
+}
+
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_l1_apply_r(Grid gc, Grid gf, const T *__restrict__ C0,
+                                                    const T *__restrict__ Ef, const T *__restrict__ u,
+                                                    T *__restrict__ y, const T *__restrict__ b,
+                                                    const uint8_t *__restrict__ dmask) {
+    const int nz = gc.nn[2], slab = gc.ydet * nz;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= gc.nn[0] * slab) return;
+    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
+    const int n = (ix * gc.nn[1] + iy) * nz + iz;
+    T f[3] = {0, 0, 0};
+    T M[9];
+    l1_row_refl<T, false>(gc, gf, C0, Ef, u, ix, iy, iz, f, M);
+    store_out(y, b, gc.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_l1_gs_r(Grid gc, Grid gf, const T *__restrict__ C0,
+                                                 const T *__restrict__ Ef, T *__restrict__ u, const T *__restrict__ b,
+                                                 const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
+                                                 int forward) {
+    const int mx = (gc.nn[0] - cx + 1) / 2, my = (gc.ydet - cy + 1) / 2, mz = (gc.nn[2] - cz + 1) / 2;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (t >= mx * my * mz) return;
+    const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
+    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
+    const int N = gc.nnodes, n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
+    const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
+    if (dm == 7) return;
+    T f[3] = {0, 0, 0};
+    T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    l1_row_refl<T, true>(gc, gf, C0, Ef, u, ix, iy, iz, f, M);
+    const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
+    gs_update(u, N, n, bms, M, dm, forward);
+}
+
 // ------------------------------------------------------------------ assembled block rows
 // Colour-major block storage (see BlockSlot). The neighbours are visited GROUP at a time with the
 // accumulators pinned after each group, which bounds how many of the 243 block entries and 81 u values
@@ -614,9 +710,13 @@ void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y,
 
 template <typename T>
 void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
-                     const uint8_t *dmask, int mode, hipStream_t s) {
+                     const uint8_t *dmask, int mode, bool reflect, hipStream_t s) {
     const long count = (long)gc.nn[0] * gc.ydet * gc.nn[2];
-    if (count > 0) { VF_DISPATCH_MODE(k_l1_apply, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask); }
+    if (count > 0 && reflect) {
+        VF_DISPATCH_MODE(k_l1_apply_r, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask);
+    } else if (count > 0) {
+        VF_DISPATCH_MODE(k_l1_apply, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask);
+    }
     if (mode == AM_SET && gc.ydet < gc.nn[1]) {
         const long cz = (long)gc.nn[0] * (gc.nn[1] - gc.ydet) * gc.nn[2];
         hipLaunchKernelGGL(k_zero_detached<T>, dim3(nblocks(cz)), dim3(TPB), 0, s, gc, y);
@@ -674,12 +774,16 @@ void launch_fine_gs(const Grid &g, const T *K0, const T *E, T *u, const T *b, co
 
 template <typename T>
 void launch_l1_gs(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, T *u, const T *b, const uint8_t *dmask,
-                  int color, int forward, hipStream_t s) {
+                  int color, int forward, bool reflect, hipStream_t s) {
     const int cx = (color >> 2) & 1, cy = (color >> 1) & 1, cz = color & 1;
     const long count = color_count(gc, cx, cy, cz);
     if (!count) return;
-    hipLaunchKernelGGL(k_l1_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx, cy, cz,
-                       forward);
+    if (reflect)
+        hipLaunchKernelGGL(k_l1_gs_r<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx, cy,
+                           cz, forward);
+    else
+        hipLaunchKernelGGL(k_l1_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx, cy,
+                           cz, forward);
     VF_HIP(hipGetLastError());
 }
 
@@ -719,14 +823,14 @@ void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int 
     template void launch_fine_apply<T>(const Grid &, const T *, const T *, const T *, T *, const T *, const uint8_t *,  \
                                        int, hipStream_t);                                                              \
     template void launch_l1_apply<T>(const Grid &, const Grid &, const T *, const T *, const T *, T *, const T *,       \
-                                     const uint8_t *, int, hipStream_t);                                               \
+                                     const uint8_t *, int, bool, hipStream_t);                                         \
     template void launch_block_apply<T>(const Grid &, const T *, const T *, T *, const T *, const uint8_t *, int,       \
                                         hipStream_t);                                                                  \
     template void launch_elem_apply<T>(const Grid &, const T *, const T *, T *, int, hipStream_t);                     \
     template void launch_fine_gs<T>(const Grid &, const T *, const T *, T *, const T *, const uint8_t *, int, int,      \
                                     hipStream_t);                                                                      \
     template void launch_l1_gs<T>(const Grid &, const Grid &, const T *, const T *, T *, const T *, const uint8_t *,    \
-                                  int, int, hipStream_t);                                                              \
+                                  int, int, bool, hipStream_t);                                                        \
     template void launch_block_gs<T>(const Grid &, const T *, T *, const T *, const uint8_t *, int, int, hipStream_t); \
     template void launch_restrict<T>(const Grid &, const Grid &, const T *, T *, hipStream_t);                        \
     template void launch_interpolate<T>(const Grid &, const Grid &, const T *, T *, int, hipStream_t);

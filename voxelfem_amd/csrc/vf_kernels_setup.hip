@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(576) k_galerkin_l1(Grid gc, Grid gf, const dou
 // Ke2(e) = sum_{k<64} E_{fine(e,k)} M2[k],  M2[8 fi2 + fi1] = P_fi2^T Chat_fi1 P_fi2.
 // One workgroup of 576 work-items (one per matrix entry) handles EPB elements, so each M2 entry
 // fetched from L2 is reused EPB times.
-constexpr int EPB = 8;
+constexpr int EPB = 16;
 __global__ void __launch_bounds__(576) k_galerkin_l2(Grid g2, Grid gf, const double *__restrict__ M2,
                                                      const double *__restrict__ Ef, double *__restrict__ Ke) {
     __shared__ double sE[EPB][64];
@@ -69,133 +69,234 @@ __global__ void __launch_bounds__(576) k_galerkin_l2(Grid g2, Grid gf, const dou
 
 // ---- one more level of Galerkin coarsening: Kc(e) = sum_fi P_fi^T Kf(child fi) P_fi
 // (accumulateCoarsenedStiffnessMatrix, MultigridSolver.hh:704-722). phi[fi][fine l][coarse l].
+// The eight child matrices are fetched together (eight independent loads per work-item) and staged in LDS,
+// so the workgroup waits on global memory once; T = Kf P goes through a double-buffered LDS tile, one
+// barrier per child.
 __global__ void __launch_bounds__(576) k_galerkin_recurse(Grid gc, Grid gf, const double *__restrict__ phi,
                                                           const double *__restrict__ Kf, double *__restrict__ Kc) {
-    __shared__ double sK[576];
-    __shared__ double sT[576];
+    __shared__ double sK[8][576];
+    __shared__ double sT[2][576];
     __shared__ double sphi[512];
     const int e = blockIdx.x;
     const int t = threadIdx.x;
     const int ez = e % gc.ne[2], ey = (e / gc.ne[2]) % gc.ne[1], ex = e / (gc.ne[1] * gc.ne[2]);
+    if (ey >= gc.ymask) {  // uniform over the workgroup
+        Kc[(size_t)e * 576 + t] = 0;
+        return;
+    }
     if (t < 512) sphi[t] = phi[t];
-    const int a = t / 24, bcol = t % 24;
-    double y = 0;
-    const bool masked = ey >= gc.ymask;
-    for (int fi = 0; fi < 8 && !masked; ++fi) {
+    double v[8];
+#pragma unroll
+    for (int fi = 0; fi < 8; ++fi) {
         const int fx = 2 * ex + ((fi >> 2) & 1), fy = 2 * ey + ((fi >> 1) & 1), fz = 2 * ez + (fi & 1);
-        const size_t ce = (size_t)((fx * gf.ne[1] + fy) * gf.ne[2] + fz);
-        __syncthreads();
-        sK[t] = Kf[ce * 576 + t];
-        __syncthreads();
-        // T = Kf * I  : T[a][3j+d] = sum_i Kf[a][3i+d] phi[i][j]
-        {
-            const int j = bcol / 3, d = bcol % 3;
-            double s = 0;
+        v[fi] = Kf[(size_t)((fx * gf.ne[1] + fy) * gf.ne[2] + fz) * 576 + t];
+    }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) s += sK[a * 24 + 3 * i + d] * sphi[fi * 64 + i * 8 + j];
-            sT[t] = s;
-        }
-        __syncthreads();
-        // Kc += I^T T : Kc[3i2+c][b] += sum_i phi[i][i2] T[3i+c][b]
-        {
-            const int i2 = a / 3, c = a % 3;
-            double s = 0;
+    for (int fi = 0; fi < 8; ++fi) sK[fi][t] = v[fi];
+    __syncthreads();
+    const int a = t / 24, bcol = t % 24;
+    const int j = bcol / 3, d = bcol % 3, i2 = a / 3, c = a % 3;
+    double y = 0;
+    for (int fi = 0; fi < 8; ++fi) {
+        // T = Kf P : T[a][3j+d] = sum_i Kf[a][3i+d] phi[i][j]
+        double s = 0;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) s += sphi[fi * 64 + i * 8 + i2] * sT[(3 * i + c) * 24 + bcol];
-            y += s;
-        }
+        for (int i = 0; i < 8; ++i) s += sK[fi][a * 24 + 3 * i + d] * sphi[fi * 64 + i * 8 + j];
+        sT[fi & 1][t] = s;
+        __syncthreads();
+        // Kc += P^T T : Kc[3i2+c][b] += sum_i phi[i][i2] T[3i+c][b]
+        double r = 0;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) r += sphi[fi * 64 + i * 8 + i2] * sT[fi & 1][(3 * i + c) * 24 + bcol];
+        y += r;
     }
     Kc[(size_t)e * 576 + t] = y;
 }
 
-// ---- block rows of an intermediate level from its element matrices.
-// block(n, slot)[a][c] = sum over elements e containing n and m = n + off(slot) of Ke_e[3 l_n + a][3 l_m + c],
-// accumulated in FP64 and stored in the precision of the cycle that reads it.
+// ---- block rows straight from element data, one work-item per node of one colour class.
+// Element (i, jj, k) "below" node n = (ix, iy, iz) is the one at (ix - i, iy - jj, iz - k); n is its local
+// node ln = 4 i + 2 jj + k, and m = n + o lies in it iff every i + o, jj + o, k + o stays in {0, 1}, with
+// lm = 4 (i + ox) + 2 (jj + oy) + (k + oz). The 27 slots split into three groups of nine by ox; in group ox
+// only the elements with lx = i + ox in {0, 1} contribute, and for one element and one matrix row 3 ln + a
+// the twelve entries 3 lm + c of that group (lx fixed) are contiguous. A work-item holds one group's 81
+// FP64 sums in registers, so every element datum it needs is read once per group, and the colour-major
+// block layout (BlockSlot) makes each slot entry a contiguous run over the class.
+struct ClassNode {
+    int ix, iy, iz, j, cnt;
+    size_t before;  // nodes of the lower colour classes
+};
+__device__ inline bool class_node(const Grid &g, int colour, ClassNode &r) {
+    const int cx = (colour >> 2) & 1, cy = (colour >> 1) & 1, cz = colour & 1;
+    const int mx = (g.nn[0] - cx + 1) / 2, my = (g.nn[1] - cy + 1) / 2, mz = (g.nn[2] - cz + 1) / 2;
+    r.cnt = mx * my * mz;
+    r.j = blockIdx.x * TPB + threadIdx.x;
+    if (r.j >= r.cnt) return false;
+    const int jx = r.j / (my * mz), q = r.j - jx * my * mz, jy = q / mz, jz = q - jy * mz;
+    r.ix = 2 * jx + cx;
+    r.iy = 2 * jy + cy;
+    r.iz = 2 * jz + cz;
+    r.before = 0;
+    for (int p = 0; p < colour; ++p) {
+        const long px = (g.nn[0] - ((p >> 2) & 1) + 1) / 2, py = (g.nn[1] - ((p >> 1) & 1) + 1) / 2,
+                   pz = (g.nn[2] - (p & 1) + 1) / 2;
+        r.before += (size_t)(px * py * pz);
+    }
+    return true;
+}
+
+// group sums of an intermediate level from its element matrices Ke (576 FP64 per element, row-major 24x24)
 template <typename OutT>
 __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *__restrict__ Ke,
                                                          OutT *__restrict__ blocks) {
-    const long t = (long)blockIdx.x * TPB + threadIdx.x;
-    if (t >= (long)g.nnodes * 27) return;
-    const int n = (int)(t % g.nnodes), slot = (int)(t / g.nnodes);
-    const int nz = g.nn[2], ix = n / (g.nn[1] * nz), iy = (n / nz) % g.nn[1], iz = n % nz;
-    const int off[3] = {slot / 9 - 1, (slot / 3) % 3 - 1, slot % 3 - 1};
-    const int id[3] = {ix, iy, iz};
-    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    bool inside = true;
-    for (int d = 0; d < 3; ++d) inside &= (unsigned)(id[d] + off[d]) < (unsigned)g.nn[d];
-    if (inside) {
-        int lo[3], hi[3];
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = max(max(id[d] - 1, id[d] + off[d] - 1), 0);
-            hi[d] = min(min(id[d], id[d] + off[d]), g.ne[d] - 1);
-        }
-        for (int ex = lo[0]; ex <= hi[0]; ++ex)
-            for (int ey = lo[1]; ey <= hi[1]; ++ey)
-                for (int ez = lo[2]; ez <= hi[2]; ++ez) {
-                    const int ln = 4 * (ix - ex) + 2 * (iy - ey) + (iz - ez);
-                    const int lm = 4 * (ix + off[0] - ex) + 2 * (iy + off[1] - ey) + (iz + off[2] - ez);
-                    const double *K = Ke + (size_t)((ex * g.ne[1] + ey) * g.ne[2] + ez) * 576;
-                    for (int a = 0; a < 3; ++a)
-                        for (int c = 0; c < 3; ++c) B[3 * a + c] += K[(3 * ln + a) * 24 + 3 * lm + c];
+    ClassNode n;
+    if (!class_node(g, blockIdx.y, n)) return;
+    OutT *out = blocks + n.before * 243 + n.j;
+    for (int gx = 0; gx < 3; ++gx) {
+        const int ox = gx - 1;
+        double acc[81];
+#pragma unroll
+        for (int q = 0; q < 81; ++q) acc[q] = 0;
+        for (int i = 0; i < 2; ++i) {
+            const int lx = i + ox;
+            if (lx < 0 || lx > 1) continue;
+            const int ex = n.ix - i;
+            const bool xin = (unsigned)ex < (unsigned)g.ne[0];
+#pragma unroll
+            for (int jj = 0; jj < 2; ++jj)
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    const int ey = n.iy - jj, ez = n.iz - k;
+                    if (!xin || (unsigned)ey >= (unsigned)g.ne[1] || (unsigned)ez >= (unsigned)g.ne[2]) continue;
+                    const int ln = 4 * i + 2 * jj + k;
+                    const double *K = Ke + (size_t)((ex * g.ne[1] + ey) * g.ne[2] + ez) * 576 + 12 * lx;
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) {
+                        const double2 *row = reinterpret_cast<const double2 *>(K + (3 * ln + a) * 24);
+                        double v[12];
+#pragma unroll
+                        for (int h = 0; h < 6; ++h) {
+                            const double2 w = row[h];
+                            v[2 * h] = w.x;
+                            v[2 * h + 1] = w.y;
+                        }
+#pragma unroll
+                        for (int ly = 0; ly < 2; ++ly)
+#pragma unroll
+                            for (int lz = 0; lz < 2; ++lz) {
+                                const int s9 = (ly - jj + 1) * 3 + (lz - k + 1);
+#pragma unroll
+                                for (int c = 0; c < 3; ++c) acc[s9 * 9 + 3 * a + c] += v[6 * ly + 3 * lz + c];
+                            }
+                    }
                 }
+        }
+#pragma unroll
+        for (int q = 0; q < 81; ++q) out[(size_t)(gx * 81 + q) * n.cnt] = (OutT)acc[q];
     }
-    const BlockSlot bs = block_slot(g, ix, iy, iz);
-    for (int k = 0; k < 9; ++k) blocks[bs.base + (size_t)(slot * 9 + k) * bs.cnt + bs.j] = (OutT)B[k];
 }
 
-// ---- level-1 block rows straight from the fine moduli: the block of coarse node n and neighbour slot
-// is sum over the coarse elements holding both nodes (unmasked: ey < ymask) of sum_fi E_child(fi) *
-// Chat_fi[(3 ln + a), (3 lm + c)], i.e. the assembled form of the level-1 operator that
-// MultigridSolver.hh:483-491 applies on the fly. One launch covers every (colour class, slot): a
-// work-item owns one node of one class, so the colour-major output (BlockSlot) is written contiguously;
-// the slot and the element offset fix ln and lm for the whole workgroup, so the Chat entries are uniform
-// (scalar) loads and only the child moduli are per work-item. Sums in FP64, stored in OutT.
-template <typename OutT>
+// ---- level-1 block rows straight from the fine moduli: the element datum is sum_fi E_child(fi) *
+// Chat_fi[(3 ln + a), (3 lm + c)] over the coarse elements holding both nodes (unmasked: ey < ymask), i.e.
+// the assembled form of the level-1 operator that MultigridSolver.hh:483-491 applies on the fly. The
+// Chat rows are uniform over the workgroup (scalar loads); only the eight child moduli per element are
+// per work-item. Sums in FP64, stored in OutT.
+// REFL: the element matrix has the reflection symmetry of a box element with an orthotropic tensor
+// (K0[(p ^ e_d, a), (q ^ e_d, c)] = S_d(a) S_d(c) K0[(p, a), (q, c)], S_d(a) = -1 iff a == d), so reflecting
+// child 0 gives every child: Chat_fi[(l, a), (m, c)] = S_fi(a) S_fi(c) Chat_0[(l ^ fi, a), (m ^ fi, c)],
+// S_fi(a) = -1 iff fi has the bit of axis a set. The scalar working set is then Chat_0 (4.6 KB) instead of
+// all eight children (37 KB), which keeps it resident in the scalar cache; the sign is a free source
+// negation of the FMA.
+template <typename OutT, bool REFL>
 __global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, const double *__restrict__ Chat,
                                                             const double *__restrict__ Ef,
                                                             OutT *__restrict__ blocks) {
-    const int slot = blockIdx.y, colour = blockIdx.z;
-    const int cx = (colour >> 2) & 1, cy = (colour >> 1) & 1, cz = colour & 1;
-    const int mx = (gc.nn[0] - cx + 1) / 2, my = (gc.nn[1] - cy + 1) / 2, mz = (gc.nn[2] - cz + 1) / 2;
-    const int cnt = mx * my * mz;
-    const int j = blockIdx.x * TPB + threadIdx.x;
-    if (j >= cnt) return;
-    const int jx = j / (my * mz), r = j - jx * my * mz, jy = r / mz, jz = r - jy * mz;
-    const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
-    const int ox = slot / 9 - 1, oy = (slot / 3) % 3 - 1, oz = slot % 3 - 1;
+    ClassNode n;
+    if (!class_node(gc, blockIdx.y, n)) return;
     const int eyEnd = min(gc.ne[1], gc.ymask);
-    const int fyz = gf.ne[1] * gf.ne[2];
-    double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    for (int e = 0; e < 8; ++e) {
-        // element at node - (i, jj, k); it holds m = n + o iff every local coordinate i + o stays in {0, 1}
-        const int i = (e >> 2) & 1, jj = (e >> 1) & 1, k = e & 1;
-        if (i + ox < 0 || i + ox > 1 || jj + oy < 0 || jj + oy > 1 || k + oz < 0 || k + oz > 1) continue;
-        const int ex = ix - i, ey = iy - jj, ez = iz - k;
-        if ((unsigned)ex >= (unsigned)gc.ne[0] || (unsigned)ey >= (unsigned)eyEnd || (unsigned)ez >= (unsigned)gc.ne[2])
-            continue;
-        const int ln = e, lm = 4 * (i + ox) + 2 * (jj + oy) + (k + oz);
-        const size_t fb = ((size_t)(2 * ex) * gf.ne[1] + 2 * ey) * gf.ne[2] + 2 * ez;
-        double Ec[8];
+    const int fz = gf.ne[2], fyz = gf.ne[1] * gf.ne[2];
+    OutT *out = blocks + n.before * 243 + n.j;
+    // slot groups (ox, oy): three slots (oz) of nine entries each, 27 FP64 sums in registers
+    for (int gxy = 0; gxy < 9; ++gxy) {
+        const int ox = gxy / 3 - 1, oy = gxy % 3 - 1;
+        double acc[27];
 #pragma unroll
-        for (int fi = 0; fi < 8; ++fi) Ec[fi] = Ef[fb + ((fi >> 2) & 1) * fyz + ((fi >> 1) & 1) * gf.ne[2] + (fi & 1)];
+        for (int q = 0; q < 27; ++q) acc[q] = 0;
+        for (int i = 0; i < 2; ++i) {
+            const int lx = i + ox;
+            if (lx < 0 || lx > 1) continue;
+            for (int jj = 0; jj < 2; ++jj) {
+                const int ly = jj + oy;
+                if (ly < 0 || ly > 1) continue;
+                const int ex = n.ix - i, ey = n.iy - jj;
+                const bool xyin = (unsigned)ex < (unsigned)gc.ne[0] && (unsigned)ey < (unsigned)eyEnd;
+                // child moduli of the two elements (i, jj, k); an absent element counts as zero moduli
+                double E[2][8];
 #pragma unroll
-        for (int fi = 0; fi < 8; ++fi) {
-            const double *C = Chat + fi * 576 + (3 * ln) * 24 + 3 * lm;
+                for (int k = 0; k < 2; ++k) {
+                    const int ez = n.iz - k;
+                    const bool in = xyin && (unsigned)ez < (unsigned)gc.ne[2];
+                    const size_t fb = in ? ((size_t)(2 * ex) * gf.ne[1] + 2 * ey) * fz + 2 * ez : 0;
 #pragma unroll
-            for (int a = 0; a < 3; ++a)
+                    for (int fi = 0; fi < 8; ++fi) {
+                        const double w = Ef[fb + ((fi >> 2) & 1) * fyz + ((fi >> 1) & 1) * fz + (fi & 1)];
+                        E[k][fi] = in ? w : 0.0;
+                    }
+                }
+                if (REFL) {
+                    // Chat_0 row (3 l + a) and column block (lx', ly') with the x and y bits reflected for
+                    // the children with fx = 1 and fy = 1: a runtime base per (fx, fy) plus compile-time
+                    // offsets; each row serves both elements k (child fi = (fx, fy, k ^ rz))
 #pragma unroll
-                for (int c = 0; c < 3; ++c) B[3 * a + c] += Ec[fi] * C[a * 24 + c];
+                    for (int fx = 0; fx < 2; ++fx)
+#pragma unroll
+                        for (int fy = 0; fy < 2; ++fy) {
+                            const double *base = Chat + 288 * (i ^ fx) + 144 * (jj ^ fy) + 12 * (lx ^ fx) +
+                                                 6 * (ly ^ fy);
+#pragma unroll
+                            for (int rz = 0; rz < 2; ++rz)
+#pragma unroll
+                                for (int a = 0; a < 3; ++a) {
+                                    const double *row = base + (3 * rz + a) * 24;
+#pragma unroll
+                                    for (int k = 0; k < 2; ++k) {
+                                        const int fzb = k ^ rz, fi = 4 * fx + 2 * fy + fzb;
+                                        const bool sa = (fi >> (2 - a)) & 1;
+#pragma unroll
+                                        for (int lz = 0; lz < 2; ++lz) {
+                                            const int s3 = lz - k + 1;
+#pragma unroll
+                                            for (int c = 0; c < 3; ++c) {
+                                                const double e = E[k][fi];
+                                                const bool neg = sa != (bool)((fi >> (2 - c)) & 1);
+                                                acc[s3 * 9 + 3 * a + c] += (neg ? -e : e) * row[3 * (lz ^ fzb) + c];
+                                            }
+                                        }
+                                    }
+                                }
+                        }
+                } else {
+                    const double *base = Chat + 288 * i + 144 * jj + 12 * lx + 6 * ly;
+#pragma unroll
+                    for (int k = 0; k < 2; ++k)
+#pragma unroll
+                        for (int fi = 0; fi < 8; ++fi)
+#pragma unroll
+                            for (int a = 0; a < 3; ++a) {
+                                // row 3 ln + a, columns 3 lm + c (lm = 4 lx + 2 ly + lz) of child fi's matrix
+                                const double *row = base + fi * 576 + (3 * k + a) * 24;
+#pragma unroll
+                                for (int lz = 0; lz < 2; ++lz) {
+                                    const int s3 = lz - k + 1;
+#pragma unroll
+                                    for (int c = 0; c < 3; ++c) acc[s3 * 9 + 3 * a + c] += E[k][fi] * row[3 * lz + c];
+                                }
+                            }
+                }
+            }
         }
-    }
-    size_t before = 0;
-    for (int q = 0; q < colour; ++q) {
-        const long qx = (gc.nn[0] - ((q >> 2) & 1) + 1) / 2, qy = (gc.nn[1] - ((q >> 1) & 1) + 1) / 2,
-                   qz = (gc.nn[2] - (q & 1) + 1) / 2;
-        before += (size_t)(qx * qy * qz);
-    }
-    OutT *out = blocks + before * 243 + (size_t)slot * 9 * cnt + j;
 #pragma unroll
-    for (int q = 0; q < 9; ++q) out[(size_t)q * cnt] = (OutT)B[q];
+        for (int q = 0; q < 27; ++q) out[(size_t)(gxy * 27 + q) * n.cnt] = (OutT)acc[q];
+    }
 }
 
 }  // namespace
@@ -220,24 +321,29 @@ void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, 
 
 template <typename OutT>
 void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_assemble_blocks<OutT>, dim3(nblocks((long)g.nnodes * 27)), dim3(TPB), 0, s, g, Ke, blocks);
+    const long maxClass = (long)((g.nn[0] + 1) / 2) * ((g.nn[1] + 1) / 2) * ((g.nn[2] + 1) / 2);
+    hipLaunchKernelGGL(k_assemble_blocks<OutT>, dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, g, Ke, blocks);
     VF_HIP(hipGetLastError());
 }
 
 template <typename OutT>
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
-                               hipStream_t s) {
+                               bool reflect, hipStream_t s) {
     const long maxClass = (long)((gc.nn[0] + 1) / 2) * ((gc.nn[1] + 1) / 2) * ((gc.nn[2] + 1) / 2);
-    hipLaunchKernelGGL(k_assemble_l1_blocks<OutT>, dim3(nblocks(maxClass), 27, 8), dim3(TPB), 0, s, gc, gf, Chat, Ef,
-                       blocks);
+    if (reflect)
+        hipLaunchKernelGGL((k_assemble_l1_blocks<OutT, true>), dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, gc, gf,
+                           Chat, Ef, blocks);
+    else
+        hipLaunchKernelGGL((k_assemble_l1_blocks<OutT, false>), dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, gc, gf,
+                           Chat, Ef, blocks);
     VF_HIP(hipGetLastError());
 }
 
 template void launch_assemble_blocks<double>(const Grid &, const double *, double *, hipStream_t);
 template void launch_assemble_blocks<float>(const Grid &, const double *, float *, hipStream_t);
 template void launch_assemble_l1_blocks<double>(const Grid &, const Grid &, const double *, const double *, double *,
-                                                hipStream_t);
+                                                bool, hipStream_t);
 template void launch_assemble_l1_blocks<float>(const Grid &, const Grid &, const double *, const double *, float *,
-                                               hipStream_t);
+                                               bool, hipStream_t);
 
 }  // namespace vf
