@@ -202,8 +202,9 @@ int vf_mean(int64_t n, const double *x, int where, void *stream, double *out);
  * family on the solver's grid (timed once, on first use); 1 = register-gather kernels with one launch
  * per colour; 2 = LDS-tiled kernels on the structured element matrix with the two-launch smoother;
  * 3 = element-centric Walsh-basis K u with the LDS-tiled smoother; 4 = Walsh-basis K u with the
- * register-gather smoother on the structured element matrix. All families compute the same operators;
- * the tests compare them. Variant 1 also keeps level 1 matrix-free. */
+ * register-gather smoother on the structured element matrix; 5 = as 3 with level 1 on the fly on the
+ * matrix cores. All families compute the same operators; the tests compare them. Variant 1 keeps level 1
+ * on the fly on the vector units. */
 int vf_set_kernel_variant(int variant);
 /* Fine-level kernel families the solver uses in `precision` (running the autotuner if it has not run):
  * choice[0] for K u (0 register-gather, 1 LDS-tiled, 2 Walsh), choice[1] for the smoother
@@ -213,10 +214,11 @@ int vf_set_kernel_variant(int variant);
 int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]);
 /* Level-1 operator representation the solver uses in `precision` (running the autotuner if it has not
  * run): choice[0] for the residual, choice[1] for the smoother (0 assembled 27-point block rows, 1 on the
- * fly from the fine moduli); ms[] = the autotuner's timings (block residual, on-the-fly residual, block
- * sweep, on-the-fly sweep; 0 where it did not run). Implementation detail of this library: the reference
- * always applies level 1 on the fly (MultigridSolver.hh:483-491). */
-int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[4]);
+ * fly from the fine moduli on the vector units, 2 on the fly on the matrix cores); ms[] = the autotuner's
+ * timings (residual: blocks, vector, matrix; one sweep: blocks, vector, matrix; 0 where it did not run).
+ * Implementation detail of this library: the reference always applies level 1 on the fly
+ * (MultigridSolver.hh:483-491). */
+int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[6]);
 /* Replay the steady-state PCG iteration of vf_mg_pcg from a HIP graph captured on its second iteration
  * (enable = 1) instead of launching its kernels one by one (0, the default). Needs the solver's stream
  * to be a created stream, not the null stream; otherwise the iterations run eagerly. Same results. */

@@ -74,11 +74,12 @@ def test_fine_applyK_masked_layers(pv):
     assert rel(y, ref) < 1e-12
 
 
-@pytest.mark.parametrize("levels,variant", [(1, 0), (2, 0), (3, 0), (2, 1), (3, 1)])
+@pytest.mark.parametrize("levels,variant", [(1, 0), (2, 0), (3, 0), (2, 1), (3, 1), (2, 5), (3, 5)])
 def test_level_operators(pv, levels, variant):
     """Every level's K u, residual, one smoothing sweep, transfers and the coarse solve against the
     oracle. variant 0 holds level 1 as assembled block rows; variant 1 applies and smooths level 1 on the
-    fly from the fine moduli (the reflection form of the coarsened element matrices)."""
+    fly from the fine moduli on the vector units (the reflection form of the coarsened element matrices),
+    variant 5 on the matrix cores (FP64 MFMA)."""
     from voxelfem_amd import capi
 
     ne = (16, 8, 16)

@@ -80,12 +80,12 @@ def tuning(mg_handle, precision=VF_PREC_FP64):
 
 
 def tuning_level1(mg_handle, precision=VF_PREC_FP64):
-    """(residual form, smoother form, timings ms [block residual, on-the-fly residual, block sweep,
-    on-the-fly sweep]) of level 1; a form is "blocks" or "on-the-fly"."""
+    """(residual form, smoother form, timings ms [residual: blocks, vector, matrix; sweep: blocks, vector,
+    matrix]) of level 1; a form is "blocks", "on-the-fly" (vector units) or "on-the-fly-mfma"."""
     ch = (ctypes.c_int * 2)()
-    ms = (ctypes.c_double * 4)()
+    ms = (ctypes.c_double * 6)()
     check(lib().vf_mg_tuning_level1(ctypes.c_void_p(mg_handle), int(precision), ch, ms))
-    forms = ("blocks", "on-the-fly")
+    forms = ("blocks", "on-the-fly", "on-the-fly-mfma")
     return forms[ch[0]], forms[ch[1]], list(ms)
 
 

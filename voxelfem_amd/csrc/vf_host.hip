@@ -636,32 +636,38 @@ struct vf_mg {
             if (ku[i] < ku[kbest]) kbest = i;
         kuChoice[p] = kbest;
         gsChoice[p] = gsT <= gsG ? (gsT <= gsF ? 1 : 2) : (gsG <= gsF ? 0 : 2);
-        // level 1 residual: assembled block rows or the on-the-fly operator, whichever is faster here
-        l1OnTheFly[p] = false;
+        // level 1: residual and smoother each from the assembled block rows, on the fly on the vector units
+        // or on the fly on the matrix cores, whichever is fastest here
+        l1Tuned[p][0] = l1Tuned[p][1] = -1;
         if (l1Blocks && L >= 2) {
             const Grid &g1 = lv[1]->g;
-            const float tb = best([&] { launch_block_apply<T>(g1, blocksp<T>(1), u.data(), y.data(), rhs.data(), dm(1),
-                                                              AM_RESIDUAL, s); });
-            const float tf = best([&] { launch_l1_apply<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(),
-                                                           dm(1), AM_RESIDUAL, sim->structured, s); });
-            l1OnTheFly[p] = tf < tb;
-            tuneMs[p][6] = tb;
-            tuneMs[p][7] = tf;
-            // level 1 smoother: one forward sweep from the block rows or from the fine moduli
+            float tr[3], ts[3];
+            tr[0] = best([&] { launch_block_apply<T>(g1, blocksp<T>(1), u.data(), y.data(), rhs.data(), dm(1),
+                                                     AM_RESIDUAL, s); });
+            tr[1] = best([&] { launch_l1_apply<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(), dm(1),
+                                                  AM_RESIDUAL, sim->structured, s); });
+            tr[2] = best([&] { launch_l1_apply_mfma<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(),
+                                                       dm(1), AM_RESIDUAL, s); });
+            // one forward sweep, from the same start each time
             const size_t n1 = 3 * (size_t)g1.nnodes;
             DBuf<T> u1(n1), b1(n1);
-            launch_fill<T>(n1, u1.data(), T(1e-3), s);
             launch_fill<T>(n1, b1.data(), T(1), s);
-            const float sb = best([&] {
-                for (int c = 0; c < 8; ++c) launch_block_gs<T>(g1, blocksp<T>(1), u1.data(), b1.data(), dm(1), c, 1, s);
-            });
-            const float sf = best([&] {
-                for (int c = 0; c < 8; ++c)
-                    launch_l1_gs<T>(g1, g, Chatp<T>(), Ep<T>(), u1.data(), b1.data(), dm(1), c, 1, sim->structured, s);
-            });
-            l1GsOnTheFly[p] = sf < sb;
-            tuneMs[p][8] = sb;
-            tuneMs[p][9] = sf;
+            auto sweep = [&](int form) {
+                launch_fill<T>(n1, u1.data(), T(1e-3), s);
+                for (int c = 0; c < 8; ++c) {
+                    if (form == 0) launch_block_gs<T>(g1, blocksp<T>(1), u1.data(), b1.data(), dm(1), c, 1, s);
+                    else if (form == 1)
+                        launch_l1_gs<T>(g1, g, Chatp<T>(), Ep<T>(), u1.data(), b1.data(), dm(1), c, 1, sim->structured, s);
+                    else launch_l1_gs_mfma<T>(g1, g, Chatp<T>(), Ep<T>(), u1.data(), b1.data(), dm(1), c, 1, s);
+                }
+            };
+            for (int f = 0; f < 3; ++f) ts[f] = best([&] { sweep(f); });
+            l1Tuned[p][0] = tr[0] <= tr[1] ? (tr[0] <= tr[2] ? 0 : 2) : (tr[1] <= tr[2] ? 1 : 2);
+            l1Tuned[p][1] = ts[0] <= ts[1] ? (ts[0] <= ts[2] ? 0 : 2) : (ts[1] <= ts[2] ? 1 : 2);
+            for (int f = 0; f < 3; ++f) {
+                tuneMs[p][6 + f] = tr[f];
+                tuneMs[p][9 + f] = ts[f];
+            }
         }
         tuneMs[p][0] = ku[0];
         tuneMs[p][1] = ku[1];
@@ -670,11 +676,16 @@ struct vf_mg {
         tuneMs[p][4] = gsT;
         tuneMs[p][5] = gsF;
     }
-    float tuneMs[2][10] = {};
-    bool l1OnTheFly[2] = {false, false};    // level-1 residual on the fly instead of from the block rows
-    bool l1GsOnTheFly[2] = {false, false};  // level-1 smoother on the fly instead of from the block rows
+    float tuneMs[2][12] = {};
+    // level-1 form of the residual ([p][0]) and of the smoother ([p][1]) the tuner chose: 0 assembled block
+    // rows, 1 on the fly on the vector units, 2 on the fly on the matrix cores; -1 not tuned
+    int l1Tuned[2][2] = {{-1, -1}, {-1, -1}};
+    int l1_form(int p, int op) const {
+        if (!(l1Blocks && L >= 2)) return g_kernel_variant == 5 ? 2 : 1;  // no block rows: on the fly
+        return l1Tuned[p][op] >= 0 ? l1Tuned[p][op] : 0;
+    }
     // level-1 block rows are read by nothing once the tuner put both level-1 operations on the fly
-    bool l1_blocks_unused(int p) const { return kuChoice[p] >= 0 && l1OnTheFly[p] && l1GsOnTheFly[p]; }
+    bool l1_blocks_unused(int p) const { return l1Tuned[p][0] > 0 && l1Tuned[p][1] > 0; }
 
     // Ad = K d on the fine level (Dirichlet rows zero) and *dAd = <d, Ad> over the active nodes, fused into
     // the Walsh-basis product when that family is in use
@@ -686,8 +697,10 @@ struct vf_mg {
         if (l == 0 && fam == 2) launch_fine_apply_walsh<T>(lv[0]->g, Wp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 0 && fam == 1) launch_fine_apply_tiled<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
-        else if (l == 1 && (!(l1Blocks && L >= 2) || l1OnTheFly[pidx<T>()]))
+        else if (l == 1 && l1_form(pidx<T>(), 0) == 1)
             launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, sim->structured, st());
+        else if (l == 1 && l1_form(pidx<T>(), 0) == 2)
+            launch_l1_apply_mfma<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l < L) launch_block_apply<T>(lv[l]->g, blocksp<T>(l), u, y, b, mask, mode, st());
         else {
             if constexpr (std::is_same<T, double>::value) {
@@ -725,9 +738,11 @@ struct vf_mg {
             if (l == 0 && gs_family<T>() == 2)
                 launch_fine_gs_gather_f<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
             else if (l == 0) launch_fine_gs<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, b, dm(0), color, forward, st());
-            else if (l == 1 && (!(l1Blocks && L >= 2) || l1GsOnTheFly[pidx<T>()]))
+            else if (l == 1 && l1_form(pidx<T>(), 1) == 1)
                 launch_l1_gs<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, sim->structured,
                                 st());
+            else if (l == 1 && l1_form(pidx<T>(), 1) == 2)
+                launch_l1_gs_mfma<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, b, dm(1), color, forward, st());
             else launch_block_gs<T>(lv[l]->g, blocksp<T>(l), u, b, dm(l), color, forward, st());
         }
     }
@@ -881,7 +896,7 @@ struct vf_mg {
         } else {
             // level 1 as assembled block rows (HBM holds them easily; the reference applies this level
             // on the fly to save memory), unless the register-gather family is forced
-            l1Blocks = g_kernel_variant != 1;
+            l1Blocks = g_kernel_variant != 1 && g_kernel_variant != 5;
             if (!l1Blocks) {
                 lv[1]->blocks.release();
                 lv[1]->blocks32.release();
@@ -1308,9 +1323,9 @@ const char *vf_version(void) { return "voxelfem-mi355x 0.1 (gfx950)"; }
 
 int vf_set_kernel_variant(int variant) {
     VF_API_BEGIN
-    need(variant >= 0 && variant <= 4,
-         "kernel variant must be 0 (autotuned), 1 (register-gather), 2 (LDS-tiled), 3 (Walsh K u, LDS-tiled smoother) "
-         "or 4 (Walsh K u, structured register-gather smoother)");
+    need(variant >= 0 && variant <= 5,
+         "kernel variant must be 0 (autotuned), 1 (register-gather), 2 (LDS-tiled), 3 (Walsh K u, LDS-tiled smoother), "
+         "4 (Walsh K u, structured register-gather smoother) or 5 (as 3, level 1 on the fly on the matrix cores)");
     g_kernel_variant = variant;
     VF_API_END
 }
@@ -1907,7 +1922,7 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
             m.useGraphs = useGraphs != 0;
             std::copy(mg->kuChoice, mg->kuChoice + 2, m.kuChoice);
             std::copy(mg->gsChoice, mg->gsChoice + 2, m.gsChoice);
-            std::copy(mg->l1OnTheFly, mg->l1OnTheFly + 2, m.l1OnTheFly);
+            std::copy(&mg->l1Tuned[0][0], &mg->l1Tuned[0][0] + 4, &m.l1Tuned[0][0]);
             pipes[p].f.resize(n);
             pipes[p].u.resize(n);
         }
@@ -2110,7 +2125,7 @@ int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]) {
     VF_API_END
 }
 
-int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[4]) {
+int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[6]) {
     VF_API_BEGIN
     need(mg && choice && ms, "null argument");
     need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
@@ -2119,10 +2134,9 @@ int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[4]) {
     const int p = precision == VF_PREC_FP64 ? 0 : 1;
     if (p == 0) mg->tune<double>();
     else mg->tune<float>();
-    const bool blocks = mg->l1Blocks && mg->L >= 2;
-    choice[0] = blocks && !mg->l1OnTheFly[p] ? 0 : 1;
-    choice[1] = blocks && !mg->l1GsOnTheFly[p] ? 0 : 1;
-    for (int i = 0; i < 4; ++i) ms[i] = mg->tuneMs[p][6 + i];
+    choice[0] = mg->l1_form(p, 0);
+    choice[1] = mg->l1_form(p, 1);
+    for (int i = 0; i < 6; ++i) ms[i] = mg->tuneMs[p][6 + i];
     VF_API_END
 }
 

@@ -155,6 +155,16 @@ void launch_fine_gs_phase(const Grid &g, const T *K0, const T *E, const T *uin, 
 template <typename T>
 void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
                      const uint8_t *dmask, int mode, bool reflect, hipStream_t s);
+// level-1 operator on the fly with the matrix cores (full Chat, any element matrix)
+template <typename T>
+void launch_l1_apply_mfma(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
+                          const uint8_t *dmask, int mode, hipStream_t s);
+template <typename T>
+void launch_l1_gs_mfma(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, T *u, const T *b,
+                       const uint8_t *dmask, int color, int forward, hipStream_t s);
+// zero the rows of y at and above the first detached row (ydet)
+template <typename T>
+void launch_zero_detached(const Grid &g, T *y, hipStream_t s);
 // assembled 27-point block rows (levels >= 2)
 template <typename T>
 void launch_block_apply(const Grid &g, const T *blocks, const T *u, T *y, const T *b, const uint8_t *dmask, int mode,

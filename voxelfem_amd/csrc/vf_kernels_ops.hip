@@ -697,6 +697,14 @@ inline unsigned nblocks(long count) { return (unsigned)((count + TPB - 1) / TPB)
     }
 
 template <typename T>
+void launch_zero_detached(const Grid &g, T *y, hipStream_t s) {
+    if (g.ydet >= g.nn[1]) return;
+    const long cz = (long)g.nn[0] * (g.nn[1] - g.ydet) * g.nn[2];
+    hipLaunchKernelGGL(k_zero_detached<T>, dim3(nblocks(cz)), dim3(TPB), 0, s, g, y);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y, const T *b, const uint8_t *dmask,
                        int mode, hipStream_t s) {
     const long count = (long)g.nn[0] * g.ydet * g.nn[2];
@@ -820,6 +828,7 @@ void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int 
 }
 
 #define VF_INST(T)                                                                                                     \
+    template void launch_zero_detached<T>(const Grid &, T *, hipStream_t);                                             \
     template void launch_fine_apply<T>(const Grid &, const T *, const T *, const T *, T *, const T *, const uint8_t *,  \
                                        int, hipStream_t);                                                              \
     template void launch_l1_apply<T>(const Grid &, const Grid &, const T *, const T *, const T *, T *, const T *,       \
