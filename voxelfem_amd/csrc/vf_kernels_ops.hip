@@ -347,22 +347,98 @@ __global__ void __launch_bounds__(TPB) k_l1_apply_r(Grid gc, Grid gf, const T *_
     store_out(y, b, gc.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
+// The same row split over two work-items: work-item h of a pair takes the four elements with x bit i = h.
+// Its element loop stays unrolled over (j, k); the runtime x bit is folded away by loading the element's
+// node values in x-mirrored order when h = 1 (node m' holds local node m' ^ 4h, x component negated), so
+// Chat_0 rows, register indices and signs stay compile-time, and the remaining runtime sign (-1)^h is
+// applied once to the x-row of f and the x-off-diagonal entries of M. The pair sums its halves with one
+// lane exchange: twice the work-items per colour and half the work each, so a colour's launch has twice as
+// many waves in flight (the smoother's launches cover one colour, an eighth of the nodes, each).
+template <typename T, bool WANT_M>
+__device__ __forceinline__ void l1_row_refl_half(const Grid &gc, const Grid &gf, const T *__restrict__ C0,
+                                                 const T *__restrict__ Ef, const T *__restrict__ u, int ix, int iy,
+                                                 int iz, int h, T f[3], T M[9]) {
+    const int N = gc.nnodes, sx = gc.nn[1] * gc.nn[2], sy = gc.nn[2];
+    const int fz = gf.ne[2], fyz = gf.ne[1] * gf.ne[2];
+    const int n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
+    const int sxh = h ? -sx : sx;
+    const int fyzA = h ? fyz : 0, fyzB = h ? 0 : fyz;  // x offset of a child whose reflected x bit is 0 / 1
+    const T hs = h ? T(-1) : T(1);
+    const int ex = ix - h;
+    const bool xin = (unsigned)ex < (unsigned)gc.ne[0];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int j = (e >> 1) & 1, k = e & 1;
+        const int ey = iy - j, ez = iz - k;
+        const bool ok = xin && (unsigned)ey < (unsigned)gc.ne[1] && (unsigned)ez < (unsigned)gc.ne[2];
+        if (!ok) continue;
+        T ue[8][3];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            // local node m ^ 4h of the element at x offset (m_x - h) = (m >> 2 & 1) * (h ? -1 : 1)
+            const int nm = n + ((m >> 2) & 1) * sxh + (((m >> 1) & 1) - j) * sy + ((m & 1) - k);
+            ue[m][0] = hs * u[nm];
+            ue[m][1] = u[(size_t)N + nm];
+            ue[m][2] = u[(size_t)2 * N + nm];
+        }
+        const int fb = eidx(gf, 2 * ex, 2 * ey, 2 * ez);
+#pragma unroll
+        for (int g = 0; g < 8; ++g) {
+            const int fis = (g & 4) | ((e ^ g) & 3);  // the child index with its x bit taken from g
+            const T Efi = Ef[fb + ((g >> 2) & 1 ? fyzB : fyzA) + ((fis >> 1) & 1) * fz + (fis & 1)];
+            T t[3] = {0, 0, 0};
+#pragma unroll
+            for (int mp = 0; mp < 8; ++mp)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const T uv = ((fis >> (2 - c)) & 1) ? -ue[mp ^ fis][c] : ue[mp ^ fis][c];
+#pragma unroll
+                    for (int a = 0; a < 3; ++a) t[a] += C0[(3 * g + a) * 24 + 3 * mp + c] * uv;
+                }
+#pragma unroll
+            for (int a = 0; a < 3; ++a) f[a] += (((fis >> (2 - a)) & 1) ? -Efi : Efi) * t[a];
+            if (WANT_M) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) {
+                        const bool neg = (bool)((fis >> (2 - a)) & 1) != (bool)((fis >> (2 - c)) & 1);
+                        M[3 * a + c] += (neg ? -Efi : Efi) * C0[(3 * g + a) * 24 + 3 * g + c];
+                    }
+            }
+        }
+    }
+    f[0] *= hs;
+    if (WANT_M) {
+        M[1] *= hs;
+        M[2] *= hs;
+        M[3] *= hs;
+        M[6] *= hs;
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) f[a] += __shfl_xor(f[a], 1, 64);
+    if (WANT_M)
+#pragma unroll
+        for (int q = 0; q < 9; ++q) M[q] += __shfl_xor(M[q], 1, 64);
+}
+
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_l1_gs_r(Grid gc, Grid gf, const T *__restrict__ C0,
-                                                 const T *__restrict__ Ef, T *__restrict__ u, const T *__restrict__ b,
-                                                 const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
-                                                 int forward) {
+__global__ void __launch_bounds__(TPB) k_l1_gs_r2(Grid gc, Grid gf, const T *__restrict__ C0,
+                                                  const T *__restrict__ Ef, T *__restrict__ u, const T *__restrict__ b,
+                                                  const uint8_t *__restrict__ dmask, int cx, int cy, int cz,
+                                                  int forward) {
     const int mx = (gc.nn[0] - cx + 1) / 2, my = (gc.ydet - cy + 1) / 2, mz = (gc.nn[2] - cz + 1) / 2;
-    const int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= mx * my * mz) return;
+    const int tt = blockIdx.x * TPB + threadIdx.x, t = tt >> 1, h = tt & 1;
+    if (t >= mx * my * mz) return;  // both work-items of a pair
     const int jx = t / (my * mz), r = t - jx * my * mz, jy = r / mz, jz = r - jy * mz;
     const int ix = 2 * jx + cx, iy = 2 * jy + cy, iz = 2 * jz + cz;
     const int N = gc.nnodes, n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
     const uint8_t dm = dmask ? dmask[n] : uint8_t(0);
-    if (dm == 7) return;
+    if (dm == 7) return;  // both work-items of a pair
     T f[3] = {0, 0, 0};
     T M[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    l1_row_refl<T, true>(gc, gf, C0, Ef, u, ix, iy, iz, f, M);
+    l1_row_refl_half<T, true>(gc, gf, C0, Ef, u, ix, iy, iz, h, f, M);
+    if (h) return;
     const T bms[3] = {b[n] - f[0], b[N + n] - f[1], b[2 * N + n] - f[2]};
     gs_update(u, N, n, bms, M, dm, forward);
 }
@@ -721,6 +797,8 @@ void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef,
                      const uint8_t *dmask, int mode, bool reflect, hipStream_t s) {
     const long count = (long)gc.nn[0] * gc.ydet * gc.nn[2];
     if (count > 0 && reflect) {
+        // one work-item per node: over all nodes the launch has waves enough, and the whole row per work-item
+        // measures faster than the split pair here
         VF_DISPATCH_MODE(k_l1_apply_r, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask);
     } else if (count > 0) {
         VF_DISPATCH_MODE(k_l1_apply, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask);
@@ -787,8 +865,8 @@ void launch_l1_gs(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, T 
     const long count = color_count(gc, cx, cy, cz);
     if (!count) return;
     if (reflect)
-        hipLaunchKernelGGL(k_l1_gs_r<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx, cy,
-                           cz, forward);
+        hipLaunchKernelGGL(k_l1_gs_r2<T>, dim3(nblocks(2 * count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx,
+                           cy, cz, forward);
     else
         hipLaunchKernelGGL(k_l1_gs<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gc, gf, Chat, Ef, u, b, dmask, cx, cy,
                            cz, forward);
