@@ -223,6 +223,12 @@ int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[6]);
  * (enable = 1) instead of launching its kernels one by one (0, the default). Needs the solver's stream
  * to be a created stream, not the null stream; otherwise the iterations run eagerly. Same results. */
 int vf_mg_set_use_graphs(vf_mg *mg, int enable);
+/* Time the fine-level K d of every PCG iteration where it runs (enable = 1; resets the totals): HIP events
+ * on the solver's stream around each launch, also inside the replayed graph, read after each iteration's
+ * host synchronisation. Only the fused iteration (no host callback, mgIterations > 0) is timed.
+ * vf_mg_kernel_timing returns the number of timed launches and their summed milliseconds. */
+int vf_mg_set_kernel_timing(vf_mg *mg, int enable);
+int vf_mg_kernel_timing(vf_mg *mg, int64_t *count, double *total_ms);
 /* Average time of one level-`level` operation of the hierarchy over `reps` launches (HIP events on
  * the solver's stream, vectors allocated and filled beforehand): which = 0 K u, 1 residual b - K u,
  * 2 one forward smoothing sweep (all eight colours). */

@@ -174,6 +174,40 @@ def test_pcg_graph_replay_matches_eager(vf, prec):
         assert h == h0
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_pcg_kernel_timing_in_replayed_graph(vf, prec):
+    """Timing the fine K d of the fused PCG where it runs (events on the solver's stream, captured into the
+    replayed graph) times one launch per iteration and leaves the iterates unchanged."""
+    import torch
+
+    from voxelfem_amd import capi
+
+    pv = vf.pyVoxelFEM
+    ne = (32, 16, 16)
+    tps = wl.configure(pv, ne, wl.CANTILEVER_BC, wl.noise_density(ne, 8))
+    mg = tps.multigridSolver(2)
+    sim_h, mg_h = tps._handle(), mg._handle()
+    nn = tps.numNodes()
+    dev = torch.device("cuda:0")
+    f = torch.zeros(3 * nn, dtype=torch.float64, device=dev)
+    capi.check(capi.lib().vf_sim_build_load(sim_h, f.data_ptr(), capi.VF_DEVICE))
+    p = capi.VF_PREC_FP64 if prec == "fp64" else capi.VF_PREC_FP32
+    out = []
+    for graphs, timing in ((False, False), (True, True), (False, True)):
+        mg.useGraphs = graphs
+        capi.set_kernel_timing(mg_h, timing)
+        x = torch.zeros_like(f)
+        st = capi.pcg(mg_h, x.data_ptr(), f.data_ptr(), 500, 1e-10, 1, 1, False, p)
+        torch.cuda.synchronize()
+        n, ms = capi.kernel_timing(mg_h)
+        capi.set_kernel_timing(mg_h, False)
+        out.append(x.cpu().numpy())
+        assert n == (st.iterations if timing else 0)
+        assert (ms > 0) == timing
+    mg.useGraphs = False
+    assert np.array_equal(out[1], out[0]) and np.array_equal(out[2], out[0])
+
+
 # ----------------------------------------------------------------------------- batched layer-by-layer
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
 def test_lbl_batched_matches_sequential_and_oracle(vf, prec):

@@ -28,6 +28,8 @@ def lib():
         L.vf_set_kernel_variant.argtypes = [i32]
         L.vf_mg_tuning.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(dbl)]
         L.vf_mg_tuning_level1.argtypes = [vp, i32, ctypes.POINTER(i32), ctypes.POINTER(dbl)]
+        L.vf_mg_set_kernel_timing.argtypes = [vp, i32]
+        L.vf_mg_kernel_timing.argtypes = [vp, ctypes.POINTER(i64), ctypes.POINTER(dbl)]
         L.vf_sim_set_densities.argtypes = [vp, vp, i32]
         L.vf_sim_build_load.argtypes = [vp, vp, i32]
         L.vf_sim_compliance_gradient.argtypes = [vp, vp, vp, i32]
@@ -87,6 +89,19 @@ def tuning_level1(mg_handle, precision=VF_PREC_FP64):
     check(lib().vf_mg_tuning_level1(ctypes.c_void_p(mg_handle), int(precision), ch, ms))
     forms = ("blocks", "on-the-fly", "on-the-fly-mfma")
     return forms[ch[0]], forms[ch[1]], list(ms)
+
+
+def set_kernel_timing(mg_handle, enable):
+    """Start (True, totals reset) or stop timing the fine K d launches of PCG where they run."""
+    check(lib().vf_mg_set_kernel_timing(ctypes.c_void_p(mg_handle), 1 if enable else 0))
+
+
+def kernel_timing(mg_handle):
+    """(number of timed K d launches, their summed milliseconds)."""
+    n = ctypes.c_int64()
+    ms = ctypes.c_double()
+    check(lib().vf_mg_kernel_timing(ctypes.c_void_p(mg_handle), ctypes.byref(n), ctypes.byref(ms)))
+    return n.value, ms.value
 
 
 def bench_apply_K(sim_handle, reps, precision=VF_PREC_FP64):

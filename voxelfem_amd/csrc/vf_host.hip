@@ -433,7 +433,15 @@ struct vf_mg {
         }
         if (evGalerkin) (void)hipEventDestroy(evGalerkin);
         if (evFactor) (void)hipEventDestroy(evFactor);
+        for (hipEvent_t e : kdEv)
+            if (e) (void)hipEventDestroy(e);
     }
+    // measurement of the fine K d of PCG where it runs: HIP events on the solver's stream around each launch
+    // (recorded into the replayed graph as well), summed after each iteration's host read of <r,r>
+    bool timeKd = false;
+    hipEvent_t kdEv[2] = {nullptr, nullptr};
+    double kdMs = 0;
+    int64_t kdCount = 0;
     vf_mg(const vf_mg &) = delete;
     vf_mg &operator=(const vf_mg &) = delete;
 
@@ -1113,7 +1121,9 @@ struct vf_mg {
                     launch_pcg_dots_s<double>(g, r, sv, adp, part, sc, s);
                     launch_pcg_direction_x<double>(g, sv, d, x, sc, flexible ? 1 : 0, firstIt ? 1 : 0, s);
                 }
+                if (timeKd) VF_HIP(hipEventRecord(kdEv[0], s));
                 apply_dot(d, Ad, part, sc + 3);
+                if (timeKd) VF_HIP(hipEventRecord(kdEv[1], s));
                 launch_pcg_scalars(sc, 1, s);  // alpha of this iteration
                 launch_pcg_residual(g, r, Ad, sc, part, sc + 4, r32, s);
                 return;
@@ -1161,6 +1171,12 @@ struct vf_mg {
                 body(first);
             }
             rr = sim->read_scalar(sc + 4);
+            if (timeKd && fused) {  // the stream is idle here: both events of this iteration have completed
+                float ms = 0;
+                VF_HIP(hipEventElapsedTime(&ms, kdEv[0], kdEv[1]));
+                kdMs += ms;
+                ++kdCount;
+            }
             if (std::isnan(rr)) {
                 double v[6];
                 d2h(v, sc, 6, s);
@@ -2102,6 +2118,26 @@ int vf_mg_set_use_graphs(vf_mg *mg, int enable) {
     VF_API_BEGIN
     need(mg != nullptr, "null solver");
     mg->useGraphs = enable != 0;
+    VF_API_END
+}
+
+int vf_mg_set_kernel_timing(vf_mg *mg, int enable) {
+    VF_API_BEGIN
+    need(mg != nullptr, "null solver");
+    if (enable)
+        for (hipEvent_t &e : mg->kdEv)
+            if (!e) VF_HIP(hipEventCreate(&e));
+    mg->timeKd = enable != 0;
+    mg->kdMs = 0;
+    mg->kdCount = 0;
+    VF_API_END
+}
+
+int vf_mg_kernel_timing(vf_mg *mg, int64_t *count, double *total_ms) {
+    VF_API_BEGIN
+    need(mg && count && total_ms, "null argument");
+    *count = mg->kdCount;
+    *total_ms = mg->kdMs;
     VF_API_END
 }
 
