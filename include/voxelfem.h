@@ -170,6 +170,20 @@ int vf_mg_pcg(vf_mg *mg, double *x, const double *b, int64_t maxIter, double tol
 int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
                     int64_t smoothing, int fullMultigrid, int precision, double *objective, double *grad,
                     int gradWhere, double *perLayerCompliance, int64_t *perLayerIterations);
+/* LayerByLayerEvaluator::run with the reference's initial guesses (LayerByLayer.hh:24-217,223-300):
+ * initMethod VF_LBL_INIT_ZERO; VF_LBL_INIT_FD with history 1 (the previous layer's solution, "constant")
+ * or 2 (2 u_0 - u_1, "fd"); VF_LBL_INIT_SUBSPACE with history N ("N=N", the reference's default N = 3): the
+ * Galerkin projection of the next layer's system onto the last N solutions, its coefficients from the
+ * pseudo-inverse of the N x N subspace matrix. uFull (optional, in memory `uFullWhere`): the full-design
+ * solution of a previous run; with zeroInit = 0 and uFullValid = 1 the full-design solve starts from it
+ * (LayerByLayer.hh:234,263-264), and on return it holds this run's full-design solution (282-283). */
+#define VF_LBL_INIT_ZERO 0
+#define VF_LBL_INIT_FD 1
+#define VF_LBL_INIT_SUBSPACE 2
+int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations, int64_t smoothing,
+               int fullMultigrid, int precision, int initMethod, int64_t history, int zeroInit, double *uFull,
+               int uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
+               double *perLayerCompliance, int64_t *perLayerIterations);
 /* The same evaluation with the layer solves batched: the layers are spread over `pipelines`
  * concurrent device pipelines (each a copy of the simulator and hierarchy on its own stream, driven by
  * its own host thread; 0 = 8), and with useGraphs the steady-state PCG iteration of each layer solve

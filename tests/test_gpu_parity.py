@@ -243,3 +243,41 @@ def test_layer_by_layer_zero_init(pv):
     obj, grad, comp, its = om.lbl_run_zero(2, 500, 1e-10)
     assert ev.objective() == pytest.approx(obj, rel=1e-9)
     assert rel(ev.gradient(), grad) < 1e-8
+
+
+def test_layer_by_layer_initial_guesses(pv):
+    """The reference's initial guesses (LayerByLayer.hh:24-221): every guess converges to the same layer
+    solutions, so objective and gradient match the zero-guess oracle; the subspace guess (the default,
+    "N=3") needs fewer PCG iterations than zero; a second run with zeroInit=False starts the full design
+    from the previous run's full-design solution, which already meets the tolerance."""
+    ne = (16, 8, 8)
+    rho = wl.noise_density(ne, 4)
+    full = wl.configure(pv, ne, None, rho)
+    lbl_sim = full.getIntermediateFabricationShape(1.0, False, pv.InterpolationLaw.SIMP)
+    mg = lbl_sim.multigridSolver(2)
+    s = orc.Sim(ne)
+    s.set_etensor(1.0, 0.3)
+    s.set_law(0, 1.0, 1e-9, 3.0, 3.0)
+    s.set_densities(rho)
+    X = wl._node_positions(ne)
+    s.set_dirichlet(np.where(X[:, 1] == 0, 7, 0).astype(np.uint8))
+    s.set_gravity([0.0, -1.0, 0.0])
+    obj, grad, comp, its_ref = orc.MG(s, 2).lbl_run_zero(1, 500, 1e-12)
+    iters = {}
+    for method in ("zero", "constant", "fd", "N=1", "N=3"):
+        ev = pv.LayerByLayerEvaluator(lbl_sim)
+        if method != "N=3":  # "N=3" is the default
+            ev.selectInitMethod(method)
+        ev.batched = False
+        ev.run(mg, True, 1, 500, 1e-12)
+        assert ev.objective() == pytest.approx(obj, rel=1e-10), method
+        assert rel(ev.gradient(), grad) < 1e-9, method
+        assert np.allclose(ev.perLayerCompliance(), comp, rtol=1e-9, atol=0), method
+        iters[method] = sum(ev.perLayerIterations())
+        if method == "N=3":
+            ev.run(mg, False, 1, 500, 1e-12)  # warm start of the full design
+            assert ev.perLayerIterations()[0] <= 1
+            assert ev.objective() == pytest.approx(obj, rel=1e-10)
+    assert iters["N=3"] < iters["zero"]
+    with pytest.raises(Exception, match="Unrecognized method"):
+        pv.LayerByLayerEvaluator(lbl_sim).selectInitMethod("N=x")

@@ -11,7 +11,8 @@ pyVoxelFEM API, on the device.
   c2  128x64x64 MBB half-beam, hat filter rmin=2.5 + projection beta=8: autograd gradient against
       central differences (h=1e-6, 20 seeded N(0,1) directions), FP64 and FP32 V-cycle.
   c3  160x80x80 layer-by-layer (40 layers of 2 along the build axis y, clamped base, self-weight):
-      batched evaluator vs the 40 sequential solves, FP64 and FP32 V-cycle.
+      batched evaluator vs the 40 sequential solves, FP64 and FP32 V-cycle, and the reference's default
+      sequential evaluator with the subspace initial guess (N=3).
   c4  384x192x192 cantilever, OC loop (filter radius 3, vf=0.3, m=0.2, p=0.5) with the FP32 V-cycle in
       the FP64 PCG, the FP64-only loop beside it, and the FP64 gradient at every mixed iterate.
 """
@@ -158,6 +159,18 @@ def c3():
                 res[name]["gradient_rel_err_vs_fp64_sequential"] = float(
                     np.abs(ev.gradient() - ref[1]).max() / np.abs(ref[1]).max())
         res["speedup_batched"] = res["sequential"]["s"] / res["batched"]["s"]
+        # the reference's default evaluator: sequential solves, each started from the subspace guess of the
+        # last three solutions
+        ev.selectInitMethod("N=3")
+        ev.batched = False
+        ev.run(mg, True, 2, 1000, 1e-8)
+        ev.run(mg, True, 2, 1000, 1e-8)
+        res["subspace_N3_sequential"] = {
+            "s": ev.lastRunSeconds, "objective": ev.objective(),
+            "pcg_iterations_total": int(np.sum(ev.perLayerIterations())),
+            "objective_rel_err_vs_fp64_sequential": abs(ev.objective() - ref[0]) / abs(ref[0])}
+        res["speedup_batched_vs_subspace"] = res["subspace_N3_sequential"]["s"] / res["batched"]["s"]
+        ev.selectInitMethod("zero")
         out[prec] = res
     return out
 
@@ -205,6 +218,8 @@ def c4(iters=50):
                 gerr.append(float(np.abs(g - gs).max() / np.abs(gs).max()))
                 t_shadow += time.perf_counter() - ts
             oc.step()
+            if (k + 1) % 10 == 0:
+                print(f"c4 {prec}: {k + 1} OC iterations", file=sys.stderr, flush=True)
         final = top.evaluateObjective()
         _sync()
         dt = time.perf_counter() - t0 - t_shadow

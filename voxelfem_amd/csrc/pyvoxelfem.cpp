@@ -632,12 +632,28 @@ struct LBL {
     int64_t pipelines = 0;
     bool useGraphs = true;
     double seconds = 0;
+    int initMethod = VF_LBL_INIT_SUBSPACE;
+    int64_t history = 3;
+    std::vector<double> uFullDesign;  // full-design solution of the last run (LayerByLayer.hh:282-283)
     explicit LBL(TPSPtr t) : tps(std::move(t)) {}
+    // LayerByLayer.hh:215-221
     void selectInitMethod(const std::string &m) {
-        if (m == "zero" || m == "constant" || m == "fd" || m.rfind("N=", 0) == 0) { method = m; return; }
-        throw std::runtime_error("Unrecognized method " + m);
+        if (m == "zero") { initMethod = VF_LBL_INIT_ZERO; history = 0; }
+        else if (m == "constant") { initMethod = VF_LBL_INIT_FD; history = 1; }
+        else if (m == "fd") { initMethod = VF_LBL_INIT_FD; history = 2; }
+        else if (m.rfind("N=", 0) == 0) {
+            size_t used = 0;
+            long k = 0;
+            try { k = std::stol(m.substr(2), &used); } catch (...) { used = 0; }
+            if (used == 0 || used != m.size() - 2 || k < 1) throw std::runtime_error("Unrecognized method " + m);
+            initMethod = VF_LBL_INIT_SUBSPACE;
+            history = k;
+        } else {
+            throw std::runtime_error("Unrecognized method " + m);
+        }
+        method = m;
     }
-    void run(MG &solver, bool /*zeroInit*/, size_t layerIncrement, size_t maxIter, double tol,
+    void run(MG &solver, bool zeroInit, size_t layerIncrement, size_t maxIter, double tol,
              std::function<void(size_t, py::array_t<double>, py::array_t<double>)> it_cb, size_t mgIterations,
              size_t mgSmoothingIterations, bool fmg, bool verbose, py::object lblCallback) {
         if (it_cb) throw std::runtime_error("per-iteration callbacks are not supported by the batched layer evaluator");
@@ -649,15 +665,24 @@ struct LBL {
         perLayerIters.assign(nl + 1, 0);
         double obj = 0;
         const auto t0 = std::chrono::steady_clock::now();
-        if (batched)
+        // the zero guess makes the layer solves independent, so they can run batched; the other guesses
+        // chain each layer to the previous solutions and run one after the other
+        const size_t nDof = 3 * (size_t)tps->nn;
+        const bool fullValid = uFullDesign.size() == nDof;
+        if (batched && initMethod == VF_LBL_INIT_ZERO && (zeroInit || !fullValid)) {
             check(vf_lbl_run_zero_batched(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol,
                                           (int64_t)mgIterations, (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision,
                                           pipelines, useGraphs ? 1 : 0, &obj, grad.data(), VF_HOST, perLayer.data(),
                                           perLayerIters.data()));
-        else
-            check(vf_lbl_run_zero(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol, (int64_t)mgIterations,
-                                  (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision, &obj, grad.data(), VF_HOST,
-                                  perLayer.data(), perLayerIters.data()));
+        } else {
+            std::vector<double> uf(nDof);
+            if (fullValid) uf = uFullDesign;
+            check(vf_lbl_run(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol, (int64_t)mgIterations,
+                             (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision, initMethod, history,
+                             zeroInit ? 1 : 0, uf.data(), fullValid ? 1 : 0, VF_HOST, &obj, grad.data(), VF_HOST,
+                             perLayer.data(), perLayerIters.data()));
+            uFullDesign.swap(uf);
+        }
         seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
         total = obj;
         if (verbose) {

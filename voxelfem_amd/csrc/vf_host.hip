@@ -15,6 +15,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <deque>
 #include <limits>
 #include <memory>
 #include <thread>
@@ -1830,19 +1831,84 @@ int vf_mg_pcg(vf_mg *mg, double *x, const double *b, int64_t maxIter, double tol
     VF_API_END
 }
 
-int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
-                    int64_t smoothing, int fullMultigrid, int precision, double *objective, double *grad, int gradWhere,
+// c = A^+ b for a symmetric s x s matrix A (row-major): eigen-decomposition by cyclic Jacobi rotations, then
+// the pseudo-inverse that drops the singular values |lambda| below s * eps * max |lambda| (the default
+// rank threshold of the Jacobi SVD solve the reference uses, LayerByLayer.hh:120-121).
+static std::vector<double> sym_pinv_solve(std::vector<double> A, const std::vector<double> &b, int s) {
+    std::vector<double> V((size_t)s * s, 0.0);
+    for (int i = 0; i < s; ++i) V[(size_t)i * s + i] = 1.0;
+    for (int sweep = 0; sweep < 64; ++sweep) {
+        double off = 0, diag = 0;
+        for (int p = 0; p < s; ++p) {
+            diag += A[(size_t)p * s + p] * A[(size_t)p * s + p];
+            for (int q = p + 1; q < s; ++q) off += A[(size_t)p * s + q] * A[(size_t)p * s + q];
+        }
+        if (off <= 1e-32 * diag || off == 0) break;
+        for (int p = 0; p < s; ++p)
+            for (int q = p + 1; q < s; ++q) {
+                const double apq = A[(size_t)p * s + q];
+                if (apq == 0) continue;
+                const double theta = (A[(size_t)q * s + q] - A[(size_t)p * s + p]) / (2 * apq);
+                const double t = (theta >= 0 ? 1.0 : -1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1));
+                const double c = 1 / std::sqrt(t * t + 1), sn = t * c;
+                for (int k = 0; k < s; ++k) {  // columns p, q
+                    const double akp = A[(size_t)k * s + p], akq = A[(size_t)k * s + q];
+                    A[(size_t)k * s + p] = c * akp - sn * akq;
+                    A[(size_t)k * s + q] = sn * akp + c * akq;
+                }
+                for (int k = 0; k < s; ++k) {  // rows p, q
+                    const double apk = A[(size_t)p * s + k], aqk = A[(size_t)q * s + k];
+                    A[(size_t)p * s + k] = c * apk - sn * aqk;
+                    A[(size_t)q * s + k] = sn * apk + c * aqk;
+                }
+                for (int k = 0; k < s; ++k) {
+                    const double vkp = V[(size_t)k * s + p], vkq = V[(size_t)k * s + q];
+                    V[(size_t)k * s + p] = c * vkp - sn * vkq;
+                    V[(size_t)k * s + q] = sn * vkp + c * vkq;
+                }
+            }
+    }
+    double smax = 0;
+    for (int k = 0; k < s; ++k) smax = std::max(smax, std::fabs(A[(size_t)k * s + k]));
+    const double thr = std::max(smax * s * std::numeric_limits<double>::epsilon(), std::numeric_limits<double>::min());
+    std::vector<double> c(s, 0.0);
+    for (int k = 0; k < s; ++k) {
+        const double lam = A[(size_t)k * s + k];
+        if (!(std::fabs(lam) >= thr)) continue;
+        double vb = 0;
+        for (int i = 0; i < s; ++i) vb += V[(size_t)i * s + k] * b[i];
+        for (int i = 0; i < s; ++i) c[i] += V[(size_t)i * s + k] * vb / lam;
+    }
+    return c;
+}
+
+// LayerByLayerEvaluator::run (LayerByLayer.hh:223-300) with its initial guesses (LayerByLayer.hh:24-217):
+// zero; the previous layers' solutions extrapolated (history 1: copied, history 2: 2 u_0 - u_1); or the
+// Galerkin projection of the next system onto the last N solutions, g = sum_i c_i u_i with
+// A c = b, A_ij = u_i . K u_j and b_i = u_i . f over the non-detached nodes (the reference updates A and b
+// by recurrences that equal these products, LayerByLayer.hh:140-208). Guesses are zero on detached rows
+// and keep the solutions' zero Dirichlet components.
+static void lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                    int64_t smoothing, int fullMultigrid, int precision, int method, int64_t history, bool zeroInit,
+                    double *uFull, bool uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
                     double *perLayerCompliance, int64_t *perLayerIterations) {
-    VF_API_BEGIN
-    need(mg && objective && grad, "null argument");
-    need(layerIncrement >= 1, "layerIncrement must be positive");
     vf_sim *sim = mg->sim;
     hipStream_t s = mg->st();
     const int64_t numLayers = sim->ne[1];
     const size_t n = 3 * sim->nn(), ne = sim->nel();
-    DBuf<double> f(n), u(n), g(ne);
+    const int H = method == VF_LBL_INIT_ZERO ? 0 : (int)history;
+    DBuf<double> f(n), u(n), g(ne), work(n);
+    std::vector<DBuf<double>> hist(H), Ku(method == VF_LBL_INIT_SUBSPACE ? H : 0);
+    for (auto &h : hist) h.resize(n);
+    for (auto &k : Ku) k.resize(n);
+    std::deque<int> order;  // history slots, newest first
     launch_fill<double>(ne, g.data(), 0.0, s);
     sim->set_mask_height(sim->h[1] * (double)numLayers, true);
+    double *sc = sim->scalars();
+    auto dot = [&](const double *a, const double *b2) {
+        launch_dot_active(sim->g, a, b2, sim->red_partials(reduction_partials(sim->g)), sc + 10, s);
+        return sim->read_scalar(sc + 10);
+    };
     double total = 0;
     int64_t count = 0;
     for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) {
@@ -1861,18 +1927,56 @@ int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
         else launch_fill<double>(n, f.data(), 0.0, s);
         if (sim->grav[0] != 0 || sim->grav[1] != 0 || sim->grav[2] != 0)
             launch_self_weight(sim->g, sim->rho.data(), sim->grav, sim->h[0] * sim->h[1] * sim->h[2], f.data(), s);
-        launch_fill<double>(n, u.data(), 0.0, s);
+        // initial guess (the full design may start from the previous run's full-design solution)
+        const int hs = (int)order.size();
+        if (l == numLayers && !zeroInit && uFull && uFullValid) {
+            if (uFullWhere == VF_DEVICE) d2d(u.data(), uFull, n, s);
+            else h2d(u.data(), uFull, n, s);
+        } else if (hs == 0) {
+            launch_fill<double>(n, u.data(), 0.0, s);
+        } else if (method == VF_LBL_INIT_FD) {
+            d2d(u.data(), hist[order[0]].data(), n, s);
+            if (hs >= 2) {  // 2 u_0 - u_1
+                launch_axpy_plain(n, 1.0, hist[order[0]].data(), u.data(), s);
+                launch_axpy_plain(n, -1.0, hist[order[1]].data(), u.data(), s);
+            }
+        } else {  // subspace
+            for (int j = 0; j < hs; ++j) sim_apply_K<double>(sim, hist[order[j]].data(), Ku[j].data(), s);
+            std::vector<double> A((size_t)hs * hs), bb(hs);
+            for (int i = 0; i < hs; ++i) {
+                bb[i] = dot(hist[order[i]].data(), f.data());
+                for (int j = 0; j <= i; ++j) A[(size_t)i * hs + j] = A[(size_t)j * hs + i] = dot(hist[order[i]].data(), Ku[j].data());
+            }
+            const std::vector<double> c = sym_pinv_solve(A, bb, hs);
+            launch_fill<double>(n, u.data(), 0.0, s);
+            for (int i = 0; i < hs; ++i) launch_axpy_plain(n, c[i], hist[order[i]].data(), u.data(), s);
+        }
+        launch_zero_detached<double>(sim->g, u.data(), s);
         const int64_t its = mg->pcg(u.data(), f.data(), maxIter, tol, (int)mgIterations, (int)smoothing,
                                     fullMultigrid != 0, precision, true, nullptr, nullptr, nullptr);
-        double *sc = sim->scalars();
-        launch_dot_active(sim->g, f.data(), u.data(), sim->red_partials(reduction_partials(sim->g)), sc + 10, s);
-        const double comp = sim->read_scalar(sc + 10);
+        const double comp = dot(f.data(), u.data());
         if (perLayerCompliance) perLayerCompliance[count] = comp;
         if (perLayerIterations) perLayerIterations[count] = its;
         total += comp;
         launch_compliance_gradient(sim->g, sim->K0.data(), sim->rho.data(), u.data(), g.data(), sim->law, sim->E0,
                                    sim->Emin, sim->gamma, sim->q, sim->grav, sim->h[0] * sim->h[1] * sim->h[2], 1, s);
         ++count;
+        if (l == numLayers && uFull) {
+            if (uFullWhere == VF_DEVICE) d2d(uFull, u.data(), n, s);
+            else { d2h(uFull, u.data(), n, s); sync(s); }
+        }
+        // finalizeLayer: the solution joins the history (the oldest slot is reused once the history is full)
+        if (H > 0 && l >= layerIncrement) {
+            int slot;
+            if ((int)order.size() < H) {
+                slot = (int)order.size();
+            } else {
+                slot = order.back();
+                order.pop_back();
+            }
+            d2d(hist[slot].data(), u.data(), n, s);
+            order.push_front(slot);
+        }
     }
     *objective = 0.5 * total / (double)count;
     std::vector<double> hg(ne);
@@ -1881,6 +1985,33 @@ int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
     for (auto &v : hg) v /= (double)count;
     if (gradWhere == VF_DEVICE) { h2d(grad, hg.data(), ne, s); sync(s); }
     else std::copy(hg.begin(), hg.end(), grad);
+}
+
+int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                    int64_t smoothing, int fullMultigrid, int precision, double *objective, double *grad, int gradWhere,
+                    double *perLayerCompliance, int64_t *perLayerIterations) {
+    VF_API_BEGIN
+    need(mg && objective && grad, "null argument");
+    need(layerIncrement >= 1, "layerIncrement must be positive");
+    lbl_run(mg, layerIncrement, maxIter, tol, mgIterations, smoothing, fullMultigrid, precision, VF_LBL_INIT_ZERO, 0,
+            true, nullptr, false, VF_HOST, objective, grad, gradWhere, perLayerCompliance, perLayerIterations);
+    VF_API_END
+}
+
+int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations, int64_t smoothing,
+               int fullMultigrid, int precision, int initMethod, int64_t history, int zeroInit, double *uFull,
+               int uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
+               double *perLayerCompliance, int64_t *perLayerIterations) {
+    VF_API_BEGIN
+    need(mg && objective && grad, "null argument");
+    need(layerIncrement >= 1, "layerIncrement must be positive");
+    need(initMethod == VF_LBL_INIT_ZERO || initMethod == VF_LBL_INIT_FD || initMethod == VF_LBL_INIT_SUBSPACE,
+         "initMethod must be VF_LBL_INIT_ZERO, VF_LBL_INIT_FD or VF_LBL_INIT_SUBSPACE");
+    need(initMethod != VF_LBL_INIT_FD || history == 1 || history == 2, "the extrapolating guess keeps 1 or 2 solutions");
+    need(initMethod != VF_LBL_INIT_SUBSPACE || (history >= 1 && history <= 64), "subspace history must be 1..64");
+    lbl_run(mg, layerIncrement, maxIter, tol, mgIterations, smoothing, fullMultigrid, precision, initMethod, history,
+            zeroInit != 0, uFull, uFullValid != 0, uFullWhere, objective, grad, gradWhere, perLayerCompliance,
+            perLayerIterations);
     VF_API_END
 }
 
