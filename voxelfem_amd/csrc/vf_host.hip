@@ -725,12 +725,15 @@ struct vf_mg {
             }
         }
     }
-    // one sweep from uin into uout (uout != uin) on levels with a two-launch smoother
+    // one sweep from uin into uout (uout != uin) on levels with a two-launch smoother; zeroIn: uin is zero
+    // (its memory is not read, only its detached rows need to hold zeros)
     template <typename T>
-    void sweep_to(int l, const T *uin, T *uout, const T *b, bool forward) {
+    void sweep_to(int l, const T *uin, T *uout, const T *b, bool forward, bool zeroIn = false) {
         const int p0 = forward ? 0 : 1;
-        launch_fine_gs_phase<T>(lv[0]->g, Fp<T>(), Ep<T>(), uin, uin, uout, b, dm(0), p0, forward, st());
-        launch_fine_gs_phase<T>(lv[0]->g, Fp<T>(), Ep<T>(), uin, uout, uout, b, dm(0), 1 - p0, forward, st());
+        launch_fine_gs_phase<T>(lv[0]->g, Fp<T>(), Ep<T>(), uin, uin, uout, b, dm(0), p0, forward, st(),
+                                zeroIn ? 3 : 0);
+        launch_fine_gs_phase<T>(lv[0]->g, Fp<T>(), Ep<T>(), uin, uout, uout, b, dm(0), 1 - p0, forward, st(),
+                                zeroIn ? 1 : 0);
     }
     template <typename T>
     bool paired(int l) const { return l == 0 && tiled_gs<T>(); }
@@ -778,20 +781,21 @@ struct vf_mg {
         }
     }
     template <typename T>
-    void vcycle(int l, int nu, bool residual = true) {
+    // xZero: the caller left x at zero (the restriction zeroes the coarse iterate), so its Dirichlet
+    // components need no clearing
+    void vcycle(int l, int nu, bool residual = true, bool xZero = false) {
         if (l == L) { coarse_solve<T>(B<T>(l), X<T>(l)); return; }
         T *x = X<T>(l);
         const T *b = B<T>(l);
-        dirichlet_for_cycle<T>(l, x, residual);
+        if (!xZero) dirichlet_for_cycle<T>(l, x, residual);
         if (paired<T>(l)) {
             // sweeps alternate between x and the residual buffer; the residual goes to whichever of
             // the two does not hold the iterate, and 2 nu sweeps bring the iterate back to x
             T *cur = x, *alt = R<T>(l);
-            for (int i = 0; i < nu; ++i) { sweep_to<T>(l, cur, alt, b, true); std::swap(cur, alt); }
+            for (int i = 0; i < nu; ++i) { sweep_to<T>(l, cur, alt, b, true, xZero && i == 0); std::swap(cur, alt); }
             apply<T>(l, cur, alt, b, AM_RESIDUAL, true);
-            launch_restrict<T>(lv[l]->g, lv[l + 1]->g, alt, B<T>(l + 1), st());
-            launch_fill<T>(3 * (size_t)lv[l + 1]->g.nnodes, X<T>(l + 1), T(0), st());
-            vcycle<T>(l + 1, nu);
+            launch_restrict<T>(lv[l]->g, lv[l + 1]->g, alt, B<T>(l + 1), st(), X<T>(l + 1));
+            vcycle<T>(l + 1, nu, true, true);
             launch_interpolate<T>(lv[l]->g, lv[l + 1]->g, X<T>(l + 1), cur, 1, st());
             for (int i = 0; i < nu; ++i) { sweep_to<T>(l, cur, alt, b, !symmetricGS); std::swap(cur, alt); }
             if (cur != x) d2d(x, cur, 3 * (size_t)lv[l]->g.nnodes, st());
@@ -799,9 +803,8 @@ struct vf_mg {
         }
         for (int i = 0; i < nu; ++i) smooth<T>(l, x, b, true);
         apply<T>(l, x, R<T>(l), b, AM_RESIDUAL, true);
-        launch_restrict<T>(lv[l]->g, lv[l + 1]->g, R<T>(l), B<T>(l + 1), st());
-        launch_fill<T>(3 * (size_t)lv[l + 1]->g.nnodes, X<T>(l + 1), T(0), st());
-        vcycle<T>(l + 1, nu);
+        launch_restrict<T>(lv[l]->g, lv[l + 1]->g, R<T>(l), B<T>(l + 1), st(), X<T>(l + 1));
+        vcycle<T>(l + 1, nu, true, true);
         launch_interpolate<T>(lv[l]->g, lv[l + 1]->g, X<T>(l + 1), x, 1, st());
         for (int i = 0; i < nu; ++i) smooth<T>(l, x, b, !symmetricGS);
     }
@@ -849,6 +852,13 @@ struct vf_mg {
     // applyPreconditionerInv (MultigridSolver.hh:577-580) on device vectors (SoA, fine level)
     // resident (FP32 cycles only): the right-hand side is already in lv[0]->b32 and the result stays in
     // lv[0]->x32 (r and s are not touched)
+    // the fine iterate of the first cycle starts at zero: with the two-launch smoother its first sweep
+    // reads it as zero without loading it, so only the detached rows (which no sweep writes) are cleared
+    template <typename T>
+    void zero_start(T *x, bool fullMG) {
+        if (paired<T>(0) && !fullMG) launch_zero_detached<T>(lv[0]->g, x, st());
+        else launch_fill<T>(3 * (size_t)lv[0]->g.nnodes, x, T(0), st());
+    }
     void precondition(const double *r, double *s, int mgIter, int nu, bool fullMG, int prec, bool resident = false) {
         sync_masks();
         const size_t n0 = 3 * (size_t)lv[0]->g.nnodes;
@@ -860,19 +870,19 @@ struct vf_mg {
             tune<double>();
             b0v = const_cast<double *>(r);
             x0v = s;
-            launch_fill<double>(n0, s, 0.0, st());
+            zero_start<double>(s, fullMG);
             for (int i = 0; i < mgIter; ++i) {
                 if (fullMG && i == 0) fmg<double>(0, nu);
-                else vcycle<double>(0, nu);
+                else vcycle<double>(0, nu, true, i == 0);
             }
         } else {
             ensure_buffers<float>(0);
             tune<float>();
             if (!resident) launch_convert<float, double>(n0, r, lv[0]->b32.data(), st());
-            launch_fill<float>(n0, lv[0]->x32.data(), 0.0f, st());
+            zero_start<float>(lv[0]->x32.data(), fullMG);
             for (int i = 0; i < mgIter; ++i) {
                 if (fullMG && i == 0) fmg<float>(0, nu);
-                else vcycle<float>(0, nu);
+                else vcycle<float>(0, nu, true, i == 0);
             }
             if (!resident) launch_convert<double, float>(n0, lv[0]->x32.data(), s, st());
         }

@@ -149,7 +149,7 @@ void launch_fine_gs_gather_f(const Grid &g, const T *F, const T *E, T *u, const 
 // unb = uin then parity 1 with unb = uout; a backward sweep runs parity 1 first.
 template <typename T>
 void launch_fine_gs_phase(const Grid &g, const T *K0, const T *E, const T *uin, const T *unb, T *uout, const T *b,
-                          const uint8_t *dmask, int parity, int forward, hipStream_t s);
+                          const uint8_t *dmask, int parity, int forward, hipStream_t s, int zin = 0);
 // level-1 operator on the fly from the fine moduli (reflect: the element matrix is reflection-symmetric and
 // Chat holds child 0's matrix first, which then stands for all eight)
 template <typename T>
@@ -186,7 +186,7 @@ void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uin
 
 // transfers (MultigridSolver.hh:178-275)
 template <typename T>
-void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s);
+void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s, T *xzero = nullptr);
 template <typename T>
 void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int accumulate, hipStream_t s);
 

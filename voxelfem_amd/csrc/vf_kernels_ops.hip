@@ -680,7 +680,9 @@ __global__ void __launch_bounds__(TPB) k_elem_apply(Grid g, const T *__restrict_
 // r_c(n) = sum_{o in {-1,0,1}^3} prod_d (1 - |o_d|/2) r_f(2n + o) over active fine nodes
 // (MultigridSolver.hh:214-275); every detached coarse row is zeroed.
 template <typename T>
-__global__ void __launch_bounds__(TPB) k_restrict(Grid gf, Grid gc, const T *__restrict__ rf, T *__restrict__ rc) {
+// xc (optional): the coarse iterate, zeroed in the same pass (the cycle starts the coarse level from zero)
+__global__ void __launch_bounds__(TPB) k_restrict(Grid gf, Grid gc, const T *__restrict__ rf, T *__restrict__ rc,
+                                                  T *__restrict__ xc) {
     const int t = blockIdx.x * TPB + threadIdx.x;
     if (t >= gc.nnodes) return;
     const int nz = gc.nn[2], ix = t / (gc.nn[1] * nz), iy = (t / nz) % gc.nn[1], iz = t % nz;
@@ -711,6 +713,11 @@ __global__ void __launch_bounds__(TPB) k_restrict(Grid gf, Grid gc, const T *__r
     rc[t] = v0;
     rc[gc.nnodes + t] = v1;
     rc[2 * gc.nnodes + t] = v2;
+    if (xc) {
+        xc[t] = T(0);
+        xc[gc.nnodes + t] = T(0);
+        xc[2 * gc.nnodes + t] = T(0);
+    }
 }
 
 // x_f (+)= P x_c, trilinear (MultigridSolver.hh:162-212). accumulate: active fine rows only
@@ -892,8 +899,8 @@ void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uin
 }
 
 template <typename T>
-void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s) {
-    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(gc.nnodes)), dim3(TPB), 0, s, gf, gc, rf, rc);
+void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s, T *xzero) {
+    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(gc.nnodes)), dim3(TPB), 0, s, gf, gc, rf, rc, xzero);
     VF_HIP(hipGetLastError());
 }
 
@@ -919,7 +926,7 @@ void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int 
     template void launch_l1_gs<T>(const Grid &, const Grid &, const T *, const T *, T *, const T *, const uint8_t *,    \
                                   int, int, bool, hipStream_t);                                                        \
     template void launch_block_gs<T>(const Grid &, const T *, T *, const T *, const uint8_t *, int, int, hipStream_t); \
-    template void launch_restrict<T>(const Grid &, const Grid &, const T *, T *, hipStream_t);                        \
+    template void launch_restrict<T>(const Grid &, const Grid &, const T *, T *, hipStream_t, T *);                   \
     template void launch_interpolate<T>(const Grid &, const Grid &, const T *, T *, int, hipStream_t);
 
 VF_INST(double)

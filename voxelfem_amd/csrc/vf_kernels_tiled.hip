@@ -418,7 +418,7 @@ __global__ void __launch_bounds__(NT, sizeof(T) == 4 ? 4 : 1) k_gs_phase(Grid g,
                                                  const T *__restrict__ uin, const T *__restrict__ unb,
                                                  T *__restrict__ uout, const T *__restrict__ b,
                                                  const uint8_t *__restrict__ dmask, int parity, int ntY, int ntZ,
-                                                 int pchunk) {
+                                                 int pchunk, int zin) {
     constexpr int HY = 2, HZ = 4;                          // u halo: colour halo + one neighbour
     constexpr int RYM = TY + 2 * HY, RZM = TZ + 2 * HZ;
     constexpr int EYM = TY + 3, EZM = TZ + 8;              // element rows [y0-2, y1], cols [z0-4, z1+2] (+1)
@@ -527,10 +527,13 @@ __global__ void __launch_bounds__(NT, sizeof(T) == 4 ? 4 : 1) k_gs_phase(Grid g,
         const bool ok = w < ES && (unsigned)gy < (unsigned)g.ne[1] && (unsigned)gz < (unsigned)g.ne[2];
         eo[i] = ok ? (gy * g.ne[2] + gz) * (int)sizeof(T) : OOB;
     }
+    // zin bit 0 / bit 1: uin / unb is known to be zero; an empty buffer range makes every load of it
+    // return zero without touching memory
+    const int ubytes = (int)((size_t)3 * N * sizeof(T));
     const __amdgpu_buffer_rsrc_t rin =
-        __builtin_amdgcn_make_buffer_rsrc((void *)uin, 0, (int)((size_t)3 * N * sizeof(T)), 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc((void *)uin, 0, (zin & 1) ? 0 : ubytes, 0x00020000);
     const __amdgpu_buffer_rsrc_t rnb =
-        __builtin_amdgcn_make_buffer_rsrc((void *)unb, 0, (int)((size_t)3 * N * sizeof(T)), 0x00020000);
+        __builtin_amdgcn_make_buffer_rsrc((void *)unb, 0, (zin & 2) ? 0 : ubytes, 0x00020000);
     const int NEL = g.ne[0] * g.ne[1] * g.ne[2];
     const __amdgpu_buffer_rsrc_t rE =
         __builtin_amdgcn_make_buffer_rsrc((void *)E, 0, (int)((size_t)NEL * sizeof(T)), 0x00020000);
@@ -1062,7 +1065,7 @@ void launch_fine_apply_tiled(const Grid &g, const T *F, const T *E, const T *u, 
 
 template <typename T>
 void launch_fine_gs_phase(const Grid &g, const T *F, const T *E, const T *uin, const T *unb, T *uout, const T *b,
-                          const uint8_t *dmask, int parity, int forward, hipStream_t s) {
+                          const uint8_t *dmask, int parity, int forward, hipStream_t s, int zin) {
     if (uin == uout) throw std::runtime_error("the two-phase smoother needs distinct input and output vectors");
     const int nplanes = (g.nn[0] - parity + 1) / 2;
     if (nplanes <= 0) return;
@@ -1079,10 +1082,10 @@ void launch_fine_gs_phase(const Grid &g, const T *F, const T *E, const T *uin, c
         throw std::runtime_error("field too large for the two-phase smoother's 32-bit buffer offsets");
     if (forward)
         hipLaunchKernelGGL((k_gs_phase<T, GS_TY, GS_TZ, GS_NT, true>), dim3(tiles, chunks), dim3(GS_NT), 0, s, g, F, E,
-                           uin, unb, uout, b, dmask, parity, ntY, ntZ, pchunk);
+                           uin, unb, uout, b, dmask, parity, ntY, ntZ, pchunk, zin);
     else
         hipLaunchKernelGGL((k_gs_phase<T, GS_TY, GS_TZ, GS_NT, false>), dim3(tiles, chunks), dim3(GS_NT), 0, s, g, F,
-                           E, uin, unb, uout, b, dmask, parity, ntY, ntZ, pchunk);
+                           E, uin, unb, uout, b, dmask, parity, ntY, ntZ, pchunk, zin);
     VF_HIP(hipGetLastError());
 }
 
@@ -1110,8 +1113,8 @@ template void launch_fine_apply_tiled<float>(const Grid &, const float *, const 
                                              const float *, const uint8_t *, int, hipStream_t);
 template void launch_fine_gs_phase<double>(const Grid &, const double *, const double *, const double *,
                                            const double *, double *, const double *, const uint8_t *, int, int,
-                                           hipStream_t);
+                                           hipStream_t, int);
 template void launch_fine_gs_phase<float>(const Grid &, const float *, const float *, const float *, const float *,
-                                          float *, const float *, const uint8_t *, int, int, hipStream_t);
+                                          float *, const float *, const uint8_t *, int, int, hipStream_t, int);
 
 }  // namespace vf
