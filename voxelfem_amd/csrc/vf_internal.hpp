@@ -130,6 +130,12 @@ void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y,
 template <typename T>
 void launch_fine_apply_tiled(const Grid &g, const T *K0, const T *E, const T *u, T *y, const T *b,
                              const uint8_t *dmask, int mode, hipStream_t s);
+// Workgroups of `threads` work-items of `kernel` (static LDS only) that the current device holds at once.
+int resident_workgroups(const void *kernel, int threads);
+// Chunk count along x for a launch of `tiles` x chunks workgroups that march over `n` planes: about
+// `want` workgroups, never more than `resident` (a workgroup left waiting for a free slot would run its
+// whole chunk after the others finish).
+int plane_chunks(int tiles, int n, int want, int resident);
 // Element-centric fine K u in the Walsh basis (AM_SET or AM_RESIDUAL); C = the 8 x 16 table of the
 // nonzero Walsh-basis element stiffness entries divided by 64 (see vf_kernels_walsh.hip).
 // With dotOut (SET mode only), *dotOut = <u, y> is also formed from per-workgroup partials (dotPartials

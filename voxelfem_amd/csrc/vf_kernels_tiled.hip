@@ -20,6 +20,8 @@
 #include "vf_internal.hpp"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 
 namespace vf {
 namespace {
@@ -1035,6 +1037,28 @@ constexpr int GS_TY = 16, GS_TZ = 32, GS_NT = 256;
 
 }  // namespace
 
+int resident_workgroups(const void *kernel, int threads) {
+    static std::mutex m;
+    static std::map<std::pair<const void *, int>, int> cache;
+    std::lock_guard<std::mutex> lock(m);
+    const auto key = std::make_pair(kernel, threads);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int per = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1) {
+        (void)hipGetLastError();
+        per = 1;
+    }
+    return cache[key] = per * cu_count();
+}
+
+int plane_chunks(int tiles, int n, int want, int resident) {
+    int chunks = std::max(1, std::min(n, ceil_div(want, tiles)));
+    if (tiles * chunks > resident) chunks = std::max(1, std::min(n, resident / tiles));
+    // equal chunks; the count may drop when the rounded-up chunk length covers n in fewer pieces
+    return ceil_div(n, ceil_div(n, chunks));
+}
+
 template <typename T>
 void launch_fine_apply_tiled(const Grid &g, const T *F, const T *E, const T *u, T *y, const T *b,
                              const uint8_t *dmask, int mode, hipStream_t s) {
@@ -1043,10 +1067,10 @@ void launch_fine_apply_tiled(const Grid &g, const T *F, const T *E, const T *u, 
         const int ntY = ceil_div(g.ydet, KU_TY), ntZ = ceil_div(g.nn[2], KU_TZ);
         const int tiles = ntY * ntZ;
         // a few workgroups per CU; each x-chunk reloads one halo plane at each end
-        const int want = 6 * cu_count();
-        int chunks = std::max(1, std::min(g.nn[0], ceil_div(want, tiles)));
+        const void *kern = mode == AM_RESIDUAL ? (const void *)k_ku_tiled<T, AM_RESIDUAL, KU_TY, KU_TZ, KU_NT>
+                                               : (const void *)k_ku_tiled<T, AM_SET, KU_TY, KU_TZ, KU_NT>;
+        const int chunks = plane_chunks(tiles, g.nn[0], 6 * cu_count(), resident_workgroups(kern, KU_NT));
         const int xchunk = ceil_div(g.nn[0], chunks);
-        chunks = ceil_div(g.nn[0], xchunk);
         if (chunks > 65535) throw std::runtime_error("grid too large for the tiled K u");
         const dim3 grid(tiles, chunks);
         if (mode == AM_RESIDUAL)
@@ -1071,12 +1095,13 @@ void launch_fine_gs_phase(const Grid &g, const T *F, const T *E, const T *uin, c
     if (nplanes <= 0) return;
     const int ntY = ceil_div(g.nn[1], GS_TY), ntZ = ceil_div(g.nn[2], GS_TZ);
     const int tiles = ntY * ntZ;
-    // as many workgroups as fit at once (LDS: two per CU in FP64, four in FP32); each chunk of planes
-    // pays one unpipelined load of its first plane step
-    const int want = (sizeof(T) == 8 ? 2 : 4) * cu_count();
-    int chunks = std::max(1, std::min(nplanes, ceil_div(want, tiles)));
+    // as many workgroups as fit at once (LDS: two per CU in FP64, four in FP32) and no more; each chunk
+    // of planes pays one unpipelined load of its first plane step
+    const void *kern = forward ? (const void *)k_gs_phase<T, GS_TY, GS_TZ, GS_NT, true>
+                               : (const void *)k_gs_phase<T, GS_TY, GS_TZ, GS_NT, false>;
+    const int resident = resident_workgroups(kern, GS_NT);
+    const int chunks = plane_chunks(tiles, nplanes, resident, resident);
     const int pchunk = ceil_div(nplanes, chunks);
-    chunks = ceil_div(nplanes, pchunk);
     if (chunks > 65535) throw std::runtime_error("grid too large for the two-phase smoother");
     if ((size_t)3 * g.nnodes * sizeof(T) >= ((size_t)1 << 30) || (size_t)g.nelems * sizeof(T) >= ((size_t)1 << 30))
         throw std::runtime_error("field too large for the two-phase smoother's 32-bit buffer offsets");

@@ -280,10 +280,9 @@ void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, 
         // tiles cover node columns [TN*t, TN*t + TN); rows beyond ydet are skipped by the owners
         const int ntY = cdiv(g.ydet, TN), ntZ = cdiv(g.nn[2], TN);
         const int tiles = ntY * ntZ;
-        const int want = 4 * cu_count_w();
-        int chunks = std::max(1, std::min(g.nn[0], cdiv(want, tiles)));
+        const void *kern = mode == AM_RESIDUAL ? (const void *)k_ku_walsh<T, AM_RESIDUAL> : (const void *)k_ku_walsh<T, AM_SET>;
+        const int chunks = plane_chunks(tiles, g.nn[0], 4 * cu_count_w(), resident_workgroups(kern, NTW));
         const int xchunk = cdiv(g.nn[0], chunks);
-        chunks = cdiv(g.nn[0], xchunk);
         if (chunks > 65535) throw std::runtime_error("grid too large for the Walsh K u");
         const dim3 grid(tiles, chunks);
         if (mode == AM_RESIDUAL)
