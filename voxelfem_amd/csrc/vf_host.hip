@@ -16,6 +16,7 @@
 #include <cmath>
 #include <cstring>
 #include <deque>
+#include <exception>
 #include <limits>
 #include <memory>
 #include <thread>
@@ -399,6 +400,24 @@ struct vf_mg {
     hipStream_t fstream = nullptr;
     hipEvent_t evGalerkin = nullptr, evFactor = nullptr;
     bool factorPending = false;
+    // With useGraphs the factorization's ~90 small launches are replayed from a captured HIP graph: issued
+    // one by one they keep the host busy for milliseconds while the main stream has nothing queued. The
+    // graph is captured again when its operands (buffers, sizes, coarsest grid) change.
+    GraphGuard factorGraph;
+    std::string factorKey;
+    // The factorization is issued from a worker thread: even replayed from a graph, its launches take the
+    // issuing thread milliseconds while the device is busy, time in which the main stream would have
+    // nothing queued. finish_factor joins the thread before the main stream waits for evFactor.
+    std::thread factorThread;
+    std::exception_ptr factorError;
+    void join_factor() {
+        if (factorThread.joinable()) factorThread.join();
+        if (factorError) {
+            std::exception_ptr e = factorError;
+            factorError = nullptr;
+            std::rethrow_exception(e);
+        }
+    }
     int bandKb = 1;
     DBuf<int> dRedToFull, dRed;
     int nred = 0, ldA = 0;
@@ -428,6 +447,7 @@ struct vf_mg {
     }
 
     ~vf_mg() {
+        if (factorThread.joinable()) factorThread.join();
         if (fstream) {
             (void)hipStreamSynchronize(fstream);
             (void)hipStreamDestroy(fstream);
@@ -963,6 +983,7 @@ struct vf_mg {
     void finish_factor() {
         if (!factorPending) return;
         factorPending = false;
+        join_factor();
         VF_HIP(hipStreamWaitEvent(st(), evFactor, 0));
         int flag = 0;
         d2h(&flag, cholFlag.data(), 1, fstream);
@@ -1036,7 +1057,7 @@ struct vf_mg {
     }
     void launch_factor() {
         if (nred == 0) return;  // nothing to solve: the coarse correction is zero
-        const Grid &g = lv[L]->g;
+        join_factor();
         hipStream_t s = st();
         if (!fstream) {
             VF_HIP(hipStreamCreateWithFlags(&fstream, hipStreamNonBlocking));
@@ -1045,11 +1066,58 @@ struct vf_mg {
         }
         // the coarsest element matrices and the index maps above are complete on the main stream
         VF_HIP(hipEventRecord(evGalerkin, s));
-        VF_HIP(hipStreamWaitEvent(fstream, evGalerkin, 0));
-        band_cholesky_inverse(g, lv[L]->Ke.data(), dRed.data(), std::max(nred, 1), bandKb, band.data(),
-                              invScratch.data(), Ainv.data(), AinvT.data(), cholFlag.data(), fstream);
-        VF_HIP(hipEventRecord(evFactor, fstream));
+        int device = 0;
+        VF_HIP(hipGetDevice(&device));
+        factorThread = std::thread([this, device] {
+            try {
+                VF_HIP(hipSetDevice(device));
+                issue_factor();
+            } catch (...) {
+                factorError = std::current_exception();
+            }
+        });
         factorPending = true;
+    }
+    // the factorization's launches on fstream, after the coarsest element matrices (evGalerkin)
+    void issue_factor() {
+        const Grid &g = lv[L]->g;
+        VF_HIP(hipStreamWaitEvent(fstream, evGalerkin, 0));
+        auto factor = [&] {
+            band_cholesky_inverse(g, lv[L]->Ke.data(), dRed.data(), std::max(nred, 1), bandKb, band.data(),
+                                  invScratch.data(), Ainv.data(), AinvT.data(), cholFlag.data(), fstream);
+        };
+        if (useGraphs) {
+            const void *ptrs[7] = {lv[L]->Ke.data(), dRed.data(), band.data(), invScratch.data(),
+                                   Ainv.data(), AinvT.data(), cholFlag.data()};
+            const int sizes[2] = {nred, bandKb};
+            std::string key((const char *)&g, sizeof(Grid));
+            key.append((const char *)ptrs, sizeof(ptrs));
+            key.append((const char *)sizes, sizeof(sizes));
+            if (!factorGraph.exec || key != factorKey) {
+                if (factorGraph.exec) VF_HIP(hipGraphExecDestroy(factorGraph.exec));
+                if (factorGraph.graph) VF_HIP(hipGraphDestroy(factorGraph.graph));
+                factorGraph.exec = nullptr;
+                factorGraph.graph = nullptr;
+                factorKey.clear();
+                hipGraph_t gr = nullptr;
+                VF_HIP(hipStreamBeginCapture(fstream, hipStreamCaptureModeThreadLocal));
+                try {
+                    factor();
+                } catch (...) {
+                    (void)hipStreamEndCapture(fstream, &gr);
+                    if (gr) (void)hipGraphDestroy(gr);
+                    throw;
+                }
+                VF_HIP(hipStreamEndCapture(fstream, &gr));
+                factorGraph.graph = gr;
+                VF_HIP(hipGraphInstantiate(&factorGraph.exec, gr, nullptr, nullptr, 0));
+                factorKey = key;
+            }
+            VF_HIP(hipGraphLaunch(factorGraph.exec, fstream));
+        } else {
+            factor();
+        }
+        VF_HIP(hipEventRecord(evFactor, fstream));
     }
 
     // ---------------------------------------------------------------- PCG (MultigridSolver.hh:1046-1152)
@@ -1155,7 +1223,23 @@ struct vf_mg {
         // With useGraphs, the steady-state iteration (second and later) is captured once into a HIP graph and
         // replayed: the first iteration runs eagerly (it also builds the hierarchy, tunes kernels and
         // allocates work buffers, none of which may happen under capture).
+        // The capture is host work only: it is done right after the eager first iteration has been queued,
+        // while the device runs it, instead of at the second iteration with the device idle.
         GraphGuard graph;
+        auto capture = [&] {
+            hipGraph_t gr = nullptr;
+            VF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            try {
+                body(false);
+            } catch (...) {
+                (void)hipStreamEndCapture(s, &gr);
+                if (gr) (void)hipGraphDestroy(gr);
+                throw;
+            }
+            VF_HIP(hipStreamEndCapture(s, &gr));
+            graph.graph = gr;
+            VF_HIP(hipGraphInstantiate(&graph.exec, gr, nullptr, nullptr, 0));
+        };
         while (it < maxIter && rr > tol * tol * bb) {
             ++it;
             // the reference rebuilds the hierarchy at the first iteration of every solve
@@ -1163,23 +1247,11 @@ struct vf_mg {
             // build (every change of densities, law, material or mask layers bumps the simulator's version)
             if (mgIter > 0 && !updated) { ensure_stiffness(); updated = true; }
             if (useGraphs && !first && s != nullptr) {
-                if (!graph.exec) {
-                    hipGraph_t gr = nullptr;
-                    VF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                    try {
-                        body(false);
-                    } catch (...) {
-                        (void)hipStreamEndCapture(s, &gr);
-                        if (gr) (void)hipGraphDestroy(gr);
-                        throw;
-                    }
-                    VF_HIP(hipStreamEndCapture(s, &gr));
-                    graph.graph = gr;
-                    VF_HIP(hipGraphInstantiate(&graph.exec, gr, nullptr, nullptr, 0));
-                }
+                if (!graph.exec) capture();
                 VF_HIP(hipGraphLaunch(graph.exec, s));
             } else {
                 body(first);
+                if (useGraphs && first && s != nullptr && it < maxIter) capture();
             }
             rr = sim->read_scalar(sc + 4);
             if (timeKd && fused) {  // the stream is idle here: both events of this iteration have completed

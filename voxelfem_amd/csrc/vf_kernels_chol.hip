@@ -69,59 +69,94 @@ __global__ void k_band_pad(int n, int nb, int kb, double *B) {
     if (i < nb * NB) bandblk(B, kb, nb - 1, 0)[(i % NB) * NB + i % NB] = 1.0;
 }
 
-// ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1 by one wavefront (64 work-items), registers only.
-// Work-item r holds row r of the block (64 doubles). Column j: the pivot A[j][j] and the entries A[c][j]
-// the update needs are broadcast from their owners with readlane, so every FMA takes a uniform operand and
-// no barrier or LDS round trip sits on the 64-step dependency chain. The inverse X = L^-1 is then formed
-// column-parallel: work-item c holds column c of X and reads the (uniform) entries of L from the rows.
+// ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1 by one workgroup of 256 work-items, four per row.
+// Work-item t holds row r = t % 64 at the columns [16 q, 16 q + 16), q = t / 64 (one quarter of the
+// columns per wavefront), in registers. The factorisation runs in the unscaled (LDL^T) form
+//   A[r][c] -= A[r][j] A[c][j] / A[j][j]   for c > j, r >= c,   j = 0 .. 63,
+// with one barrier per step: the wavefront owning column j (q = j / 16, known at compile time in the
+// unrolled loop) publishes the column in LDS (double-buffered, so the next step's publication cannot
+// overwrite entries still being read) and every work-item then updates its own entries. The pivots
+// d_j = A[j][j] give L[r][c] = A[r][c] / sqrt(d_c). The inverse is X = L^-1 = D^-1/2 Y with Y the inverse of
+// the unit lower factor L1[r][m] = A[r][m] / d_m, formed right-looking from Y = I:
+//   Y[r][c] -= L1[r][m] Y[m][c]   for r > m, c <= m,   m = 0 .. 63,
+// again one barrier per step (row m of Y and column m of L1 published in LDS).
 // flag != 0 on a non-positive pivot.
-constexpr int PTPB = 64;
-__device__ __forceinline__ double bcast(double v, int lane) {
-    const unsigned long long b = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffu), lane);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
-}
+constexpr int PTPB = 4 * NB;
+constexpr int PQ = NB / 4;  // columns per work-item
 __global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
                                                      int *__restrict__ flag) {
-    const int t = threadIdx.x;
+    __shared__ double sCol[2][NB];
+    __shared__ double sRow[2][NB];
+    __shared__ double sPiv[NB], sInvPiv[NB], sSqrt[NB], sInvSqrt[NB];
+    // q is uniform over the wavefront: the column-range tests below are scalar branches
+    const int t = threadIdx.x, r = t % NB, q = __builtin_amdgcn_readfirstlane(t / NB);
+    const int c0 = PQ * q;
     double *Dk = bandblk(B, kb, k, 0);
-    double row[NB];
+    double a[PQ];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) row[c] = Dk[t * NB + c];
+    for (int i = 0; i < PQ; ++i) a[i] = Dk[r * NB + c0 + i];
     bool ok = true;
 #pragma unroll
     for (int j = 0; j < NB; ++j) {
-        const double piv = bcast(row[j], j);
+        const int b = j & 1;
+        if (q == j / PQ) {
+            if (r >= j) sCol[b][r] = a[j % PQ];
+            if (r == j) sPiv[j] = a[j % PQ];
+        }
+        __syncthreads();
+        const double piv = sCol[b][j];
         ok = ok && piv > 0;
-        const double ljj = sqrt(piv > 0 ? piv : 1.0);
-        if (t > j) row[j] *= 1.0 / ljj;
-        if (t == j) row[j] = ljj;
+        // rows at or above j take f = 0 (a select: their sCol entries are stale); entries right of the
+        // diagonal (c > r) are updated too but never used
+        const double f = r > j ? sCol[b][r] * (1.0 / (piv > 0 ? piv : 1.0)) : 0.0;
+        if (q > j / PQ) {
 #pragma unroll
-        for (int c = j + 1; c < NB; ++c) {
-            const double acj = bcast(row[j], c);  // A[c][j], scaled
-            if (t >= c) row[c] -= row[j] * acj;
+            for (int i = 0; i < PQ; ++i) a[i] -= f * sCol[b][c0 + i];
+        } else if (q == j / PQ) {
+#pragma unroll
+            for (int i = j % PQ + 1; i < PQ; ++i) a[i] -= f * sCol[b][c0 + i];
         }
     }
     if (!ok) {
         if (t == 0) atomicOr(flag, 1);
         return;  // uniform: every work-item saw the same pivots
     }
-    // X = L^-1: column c = t. X[r][c] = (e_rc - sum_{m<r} L[r][m] X[m][c]) / L[r][r]
-    double x[NB];
+    if (t < NB) {
+        const double d = sPiv[t], sq = sqrt(d);
+        sSqrt[t] = sq;
+        sInvSqrt[t] = 1.0 / sq;
+        sInvPiv[t] = 1.0 / d;
+    }
+    __syncthreads();
+    double y[PQ];
 #pragma unroll
-    for (int r = 0; r < NB; ++r) {
-        double sum = 0;
+    for (int i = 0; i < PQ; ++i) y[i] = (c0 + i == r) ? 1.0 : 0.0;
 #pragma unroll
-        for (int m = 0; m < r; ++m) sum += bcast(row[m], r) * x[m];
-        const double lrr = bcast(row[r], r);
-        x[r] = t > r ? 0.0 : ((r == t ? 1.0 : 0.0) - sum) / lrr;
+    for (int m = 0; m < NB; ++m) {
+        const int b = m & 1;
+        if (r == m) {
+#pragma unroll
+            for (int i = 0; i < PQ; ++i) sRow[b][c0 + i] = y[i];
+        }
+        if (q == m / PQ && r > m) sCol[b][r] = a[m % PQ] * sInvPiv[m];
+        __syncthreads();
+        const double f = r > m ? sCol[b][r] : 0.0;
+        if (q < m / PQ) {
+#pragma unroll
+            for (int i = 0; i < PQ; ++i) y[i] -= f * sRow[b][c0 + i];
+        } else if (q == m / PQ) {
+#pragma unroll
+            for (int i = 0; i <= m % PQ; ++i) y[i] -= f * sRow[b][c0 + i];
+        }
     }
     double *Lk = Linv + (size_t)k * BB;
+    const double isr = sInvSqrt[r];
 #pragma unroll
-    for (int c = 0; c < NB; ++c) Dk[t * NB + c] = c <= t ? row[c] : 0.0;
-#pragma unroll
-    for (int r = 0; r < NB; ++r) Lk[r * NB + t] = x[r];
+    for (int i = 0; i < PQ; ++i) {
+        const int c = c0 + i;
+        Dk[r * NB + c] = c < r ? a[i] * sInvSqrt[c] : (c == r ? sSqrt[r] : 0.0);
+        Lk[r * NB + c] = c <= r ? y[i] * isr : 0.0;
+    }
 }
 
 // C (+)= alpha * P * Q^T for 64 x 64 row-major blocks, 256 work-items with 4 x 4 register tiles.
@@ -340,9 +375,10 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
                            double *X, double *XT, int *flag, hipStream_t s) {
     const int nb = (n + NB - 1) / NB;
     const size_t ld = (size_t)nb * NB;
-    VF_HIP(hipMemsetAsync(B, 0, band_storage(n, kb) * sizeof(double), s));
-    VF_HIP(hipMemsetAsync(X, 0, ld * ld * sizeof(double), s));
-    VF_HIP(hipMemsetAsync(flag, 0, sizeof(int), s));
+    // fill kernels rather than memsets: the sequence may be captured into a HIP graph
+    launch_fill<double>(band_storage(n, kb), B, 0.0, s);
+    launch_fill<double>(ld * ld, X, 0.0, s);
+    launch_fill<int>(1, flag, 0, s);
     hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B);
     if ((int)ld > n) hipLaunchKernelGGL(k_band_pad, dim3(1), dim3(NB), 0, s, n, nb, kb, B);
     for (int k = 0; k < nb; ++k) {

@@ -705,6 +705,7 @@ template void launch_fill_active<double>(const Grid &, int, double *, double, hi
 template void launch_fill_active<float>(const Grid &, int, float *, float, hipStream_t);
 template void launch_fill<double>(size_t, double *, double, hipStream_t);
 template void launch_fill<float>(size_t, float *, float, hipStream_t);
+template void launch_fill<int>(size_t, int *, int, hipStream_t);
 template void launch_convert<float, double>(size_t, const double *, float *, hipStream_t);
 template void launch_convert<double, float>(size_t, const float *, double *, hipStream_t);
 template void launch_copy_active<double, double>(const Grid &, int, const double *, double *, hipStream_t);
