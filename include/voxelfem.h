@@ -116,7 +116,8 @@ int vf_mg_level_nodes(vf_mg *mg, int64_t l, int64_t nn[3]);
 int vf_mg_level_dirichlet(vf_mg *mg, int64_t l, uint8_t *mask);
 /* setSymmetricGaussSeidel (MultigridSolver.hh:123-125) */
 int vf_mg_set_symmetric_gs(vf_mg *mg, int symmetric);
-/* updateStiffnessMatrices (MultigridSolver.hh:846-905): Galerkin hierarchy + coarse factorization */
+/* updateStiffnessMatrices (MultigridSolver.hh:846-905): Galerkin hierarchy + coarse factorization,
+   queued on the solver stream (returns before the device work completes) */
 int vf_mg_update_stiffness(vf_mg *mg);
 /* setFabricationMaskHeightByLayer (MultigridSolver.hh:1022-1028) */
 int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer);

@@ -385,10 +385,19 @@ struct vf_mg {
     std::vector<std::unique_ptr<LevelData>> lv;  // lv[0] mirrors the fine simulator
     double phi[512];
     double Chat[8 * 576];
-    DBuf<double> dChat, dphi, dM2, dKe0;
-    DBuf<float> dChat32;
+    DBuf<double> dChat, dphi, dM2, dKe0, dCw;
+    DBuf<float> dChat32, dCw32;
     bool symmetricGS = true;
     bool useGraphs = false;  // replay the steady-state PCG iteration from a captured HIP graph
+    // the last solve's iteration graph, destroyed during the next solve's capture (while the device is busy)
+    // rather than with the device idle at the end of its own solve
+    GraphGuard retiredGraph;
+    void retire_graph() {
+        if (retiredGraph.exec) VF_HIP(hipGraphExecDestroy(retiredGraph.exec));
+        if (retiredGraph.graph) VF_HIP(hipGraphDestroy(retiredGraph.graph));
+        retiredGraph.exec = nullptr;
+        retiredGraph.graph = nullptr;
+    }
     bool l1Blocks = false;  // level 1 held as assembled block rows (set by update_stiffness)
     // block rows of the current stiffness in FP64 / FP32, assembled on first use by a cycle of that precision
     bool blocksReady[2] = {false, false};
@@ -518,6 +527,31 @@ struct vf_mg {
         std::vector<float> c32(Chat, Chat + 8 * 576);
         h2d(dChat.data(), Chat, 8 * 576, st());
         h2d(dChat32.data(), c32.data(), 8 * 576, st());
+        // child 0's matrix in the Walsh basis for the element-wise level-1 product: Cw = W^T Chat_0 W / 64
+        {
+            auto wal = [](int l, int k) {
+                int v = 1;
+                for (int a = 0; a < 3; ++a)
+                    if ((k >> (2 - a)) & 1) v *= ((l >> (2 - a)) & 1) ? 1 : -1;
+                return v;
+            };
+            std::vector<double> cw(576, 0.0);
+            for (int k = 0; k < 8; ++k)
+                for (int a = 0; a < 3; ++a)
+                    for (int kp = 0; kp < 8; ++kp)
+                        for (int c = 0; c < 3; ++c) {
+                            double acc = 0;
+                            for (int l = 0; l < 8; ++l)
+                                for (int m = 0; m < 8; ++m)
+                                    acc += wal(l, k) * Chat[(3 * l + a) * 24 + 3 * m + c] * wal(m, kp);
+                            cw[(3 * k + a) * 24 + 3 * kp + c] = acc / 64.0;
+                        }
+            std::vector<float> cw32(cw.begin(), cw.end());
+            dCw.resize(576);
+            dCw32.resize(576);
+            h2d(dCw.data(), cw.data(), 576, st());
+            h2d(dCw32.data(), cw32.data(), 576, st());
+        }
         h2d(dphi.data(), phi, 512, st());
         if (L >= 2) {
             std::vector<double> M2(64 * 576, 0.0);
@@ -564,6 +598,8 @@ struct vf_mg {
     const T *Ep() const;
     template <typename T>
     const T *Chatp() const;
+    template <typename T>
+    const T *Cwp() const;
     template <typename T>
     const T *blocksp(int l) const;
 
@@ -673,7 +709,7 @@ struct vf_mg {
             float tr[3], ts[3];
             tr[0] = best([&] { launch_block_apply<T>(g1, blocksp<T>(1), u.data(), y.data(), rhs.data(), dm(1),
                                                      AM_RESIDUAL, s); });
-            tr[1] = best([&] { launch_l1_apply<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(), dm(1),
+            tr[1] = best([&] { launch_l1_apply<T>(g1, g, Chatp<T>(), Cwp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(), dm(1),
                                                   AM_RESIDUAL, sim->structured, s); });
             tr[2] = best([&] { launch_l1_apply_mfma<T>(g1, g, Chatp<T>(), Ep<T>(), u.data(), y.data(), rhs.data(),
                                                        dm(1), AM_RESIDUAL, s); });
@@ -727,7 +763,8 @@ struct vf_mg {
         else if (l == 0 && fam == 1) launch_fine_apply_tiled<T>(lv[0]->g, Fp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 0) launch_fine_apply<T>(lv[0]->g, K0p<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l == 1 && l1_form(pidx<T>(), 0) == 1)
-            launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, sim->structured, st());
+            launch_l1_apply<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Cwp<T>(), Ep<T>(), u, y, b, mask, mode, sim->structured,
+                               st());
         else if (l == 1 && l1_form(pidx<T>(), 0) == 2)
             launch_l1_apply_mfma<T>(lv[1]->g, lv[0]->g, Chatp<T>(), Ep<T>(), u, y, b, mask, mode, st());
         else if (l < L) launch_block_apply<T>(lv[l]->g, blocksp<T>(l), u, y, b, mask, mode, st());
@@ -1227,6 +1264,7 @@ struct vf_mg {
         // while the device runs it, instead of at the second iteration with the device idle.
         GraphGuard graph;
         auto capture = [&] {
+            retire_graph();
             hipGraph_t gr = nullptr;
             VF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
             try {
@@ -1271,6 +1309,13 @@ struct vf_mg {
             first = false;
         }
         if (fused && it > 0) launch_pcg_finish_x(g, x, d, sc, s);
+        if (graph.exec) {
+            retire_graph();
+            retiredGraph.exec = graph.exec;
+            retiredGraph.graph = graph.graph;
+            graph.exec = nullptr;
+            graph.graph = nullptr;
+        }
         if (stats) {
             stats->iterations = it;
             stats->residual_norm = std::sqrt(rr);
@@ -1300,6 +1345,8 @@ template <> const double *vf_mg::Ep<double>() const { return sim->E.data(); }
 template <> const float *vf_mg::Ep<float>() const { return sim->E32.data(); }
 template <> const double *vf_mg::Chatp<double>() const { return dChat.data(); }
 template <> const float *vf_mg::Chatp<float>() const { return dChat32.data(); }
+template <> const double *vf_mg::Cwp<double>() const { return dCw.data(); }
+template <> const float *vf_mg::Cwp<float>() const { return dCw32.data(); }
 template <> const double *vf_mg::blocksp<double>(int l) const {
     if (!blocksReady[0]) throw std::runtime_error("internal: FP64 block rows used before assembly");
     return lv[l]->blocks.data();
@@ -1731,8 +1778,9 @@ int vf_mg_set_symmetric_gs(vf_mg *mg, int symmetric) {
 int vf_mg_update_stiffness(vf_mg *mg) {
     VF_API_BEGIN
     need(mg, "null mg");
+    // asynchronous on the solver's stream: later calls are ordered after it, and a failed factorisation
+    // is reported by the first coarse solve
     mg->update_stiffness();
-    sync(mg->st());
     VF_API_END
 }
 

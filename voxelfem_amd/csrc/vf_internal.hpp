@@ -156,11 +156,12 @@ void launch_fine_gs_gather_f(const Grid &g, const T *F, const T *E, T *u, const 
 template <typename T>
 void launch_fine_gs_phase(const Grid &g, const T *K0, const T *E, const T *uin, const T *unb, T *uout, const T *b,
                           const uint8_t *dmask, int parity, int forward, hipStream_t s, int zin = 0);
-// level-1 operator on the fly from the fine moduli (reflect: the element matrix is reflection-symmetric and
-// Chat holds child 0's matrix first, which then stands for all eight)
+// level-1 operator on the fly from the fine moduli (reflect: the element matrix is reflection-symmetric;
+// the product then runs element by element in the Walsh basis from Cw = W^T Chat_0 W / 64, rows (3 k + a),
+// columns (3 k' + c), W[l][k] = prod over the axes a in k of the side of node l along a; otherwise from Chat)
 template <typename T>
-void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
-                     const uint8_t *dmask, int mode, bool reflect, hipStream_t s);
+void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Cw, const T *Ef, const T *u, T *y,
+                     const T *b, const uint8_t *dmask, int mode, bool reflect, hipStream_t s);
 // level-1 operator on the fly with the matrix cores (full Chat, any element matrix)
 template <typename T>
 void launch_l1_apply_mfma(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,

@@ -8,6 +8,8 @@
 // which the compiler turns into scalar loads shared by the whole wavefront.
 #include "vf_internal.hpp"
 
+#include <climits>
+
 namespace vf {
 namespace {
 
@@ -345,6 +347,144 @@ __global__ void __launch_bounds__(TPB) k_l1_apply_r(Grid gc, Grid gf, const T *_
     T M[9];
     l1_row_refl<T, false>(gc, gf, C0, Ef, u, ix, iy, iz, f, M);
     store_out(y, b, gc.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
+}
+
+// ---- level-1 K u element by element in the Walsh basis (reflection-symmetric element matrix).
+// With the Walsh basis W[l, k] = prod_{axes a in k} s_a(l) (s_a(l) = +-1 the side of local node l along a,
+// per displacement component), the child matrices Chat_fi = R_fi Chat_0 R_fi (R_fi: reflection across the
+// axes of fi, which permutes the nodes and flips the reflected displacement components) become sign flips
+// of Chat_0 in that basis, so the coarse element matrix sum_fi E_fi Chat_fi reads
+//   (W^T K_E W)[(k, a), (k', c)] = Cw[(k, a), (k', c)] Ehat[k ^ k' ^ e_a ^ e_c],
+//   Cw = W^T Chat_0 W,  Ehat[d] = sum_fi (-1)^popcount(d & fi) E_fi,  e_a = the bit of axis a,
+// and K_E u_E = W (Cw o Ehat) W^T u_E / 64 costs about 800 operations for all 24 outputs instead of
+// 8 x 576 for the child products (W W^T = 8 I; the 1/64 is folded into the table Cw).
+// A workgroup owns a tile of WX x WY x WZ nodes, evaluates every element touching it into LDS and sums
+// each owned node's eight element contributions in a fixed order (no atomics: deterministic).
+constexpr int WX = 4, WY = 8, WZ = 8, WNT = WX * WY * WZ;
+constexpr int WEX = WX + 1, WEY = WY + 1, WEZ = WZ + 1, WNE = WEX * WEY * WEZ;
+static_assert(WNT == TPB, "one owned node per work-item");
+
+// v: the element's nodal values [l][a] on entry, K_E v on exit
+template <typename T>
+__device__ __forceinline__ void l1_walsh_element(const T *__restrict__ Cw, const T Ef8[8], T v[8][3]) {
+    T Eh[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) Eh[i] = Ef8[i];
+    // Ehat = H E (H[d][fi] = (-1)^popcount(d & fi)), and w = W^T v, one axis at a time
+#pragma unroll
+    for (int bit = 4; bit >= 1; bit >>= 1)
+#pragma unroll
+        for (int lo = 0; lo < 8; ++lo) {
+            if (lo & bit) continue;
+            const int hi = lo | bit;
+            const T e0 = Eh[lo], e1 = Eh[hi];
+            Eh[lo] = e0 + e1;
+            Eh[hi] = e0 - e1;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const T x0 = v[lo][a], x1 = v[hi][a];
+                v[lo][a] = x0 + x1;
+                v[hi][a] = x1 - x0;
+            }
+        }
+    // out[k][a] = sum_d Ehat[d] sum_c Cw[(k, a), (k', c)] w[k'][c] with k' = k ^ d ^ e_a ^ e_c; rows and
+    // columns of the constant mode k = 0 vanish (rigid translations)
+    T out[8][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) out[0][a] = T(0);
+#pragma unroll
+    for (int k = 1; k < 8; ++k)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            T acc = T(0);
+#pragma unroll
+            for (int d = 0; d < 8; ++d) {
+                T inner = T(0);
+                bool any = false;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) {
+                    const int kp = k ^ d ^ (4 >> a) ^ (4 >> c);
+                    if (kp == 0) continue;
+                    inner += Cw[(3 * k + a) * 24 + 3 * kp + c] * v[kp][c];
+                    any = true;
+                }
+                if (any) acc += Eh[d] * inner;
+            }
+            out[k][a] = acc;
+        }
+    // back to the nodes: v = W out
+#pragma unroll
+    for (int bit = 4; bit >= 1; bit >>= 1)
+#pragma unroll
+        for (int lo = 0; lo < 8; ++lo) {
+            if (lo & bit) continue;
+            const int hi = lo | bit;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const T x0 = out[lo][a], x1 = out[hi][a];
+                out[lo][a] = x0 - x1;
+                out[hi][a] = x0 + x1;
+            }
+        }
+#pragma unroll
+    for (int l = 0; l < 8; ++l)
+#pragma unroll
+        for (int a = 0; a < 3; ++a) v[l][a] = out[l][a];
+}
+
+// y (mode) K u over the nodes of rows below ydet, like k_l1_apply_r; grid: one workgroup per node tile
+template <typename T, int MODE>
+__global__ void __launch_bounds__(TPB) k_l1_apply_w(Grid gc, Grid gf, const T *__restrict__ Cw,
+                                                    const T *__restrict__ Ef, const T *__restrict__ u,
+                                                    T *__restrict__ y, const T *__restrict__ b,
+                                                    const uint8_t *__restrict__ dmask, int ntY, int ntZ) {
+    __shared__ T sY[WNE * 24];
+    const int N = gc.nnodes, sx = gc.nn[1] * gc.nn[2], sy = gc.nn[2];
+    const int fz = gf.ne[2], fyz = gf.ne[1] * gf.ne[2];
+    const int tz = blockIdx.x % ntZ, r0 = blockIdx.x / ntZ, ty = r0 % ntY, tx = r0 / ntY;
+    const int x0 = tx * WX, y0 = ty * WY, z0 = tz * WZ;
+    const int t = threadIdx.x;
+    for (int el = t; el < WNE; el += TPB) {
+        const int lx = el / (WEY * WEZ), rr = el - lx * WEY * WEZ, ly = rr / WEZ, lz = rr - ly * WEZ;
+        const int ex = x0 - 1 + lx, ey = y0 - 1 + ly, ez = z0 - 1 + lz;
+        T v[8][3];
+        if ((unsigned)ex < (unsigned)gc.ne[0] && (unsigned)ey < (unsigned)gc.ne[1] && (unsigned)ez < (unsigned)gc.ne[2]) {
+            const int n0 = (ex * gc.nn[1] + ey) * gc.nn[2] + ez;
+            T Ef8[8];
+            const int fb = eidx(gf, 2 * ex, 2 * ey, 2 * ez);
+#pragma unroll
+            for (int l = 0; l < 8; ++l) {
+                const int nm = n0 + ((l >> 2) & 1) * sx + ((l >> 1) & 1) * sy + (l & 1);
+#pragma unroll
+                for (int a = 0; a < 3; ++a) v[l][a] = u[(size_t)a * N + nm];
+                Ef8[l] = Ef[fb + ((l >> 2) & 1) * fyz + ((l >> 1) & 1) * fz + (l & 1)];
+            }
+            l1_walsh_element(Cw, Ef8, v);
+        } else {
+#pragma unroll
+            for (int l = 0; l < 8; ++l)
+#pragma unroll
+                for (int a = 0; a < 3; ++a) v[l][a] = T(0);
+        }
+#pragma unroll
+        for (int l = 0; l < 8; ++l)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) sY[el * 24 + 3 * l + a] = v[l][a];
+    }
+    __syncthreads();
+    const int ox = t / (WY * WZ), orr = t - ox * WY * WZ, oy = orr / WZ, oz = orr - oy * WZ;
+    const int ix = x0 + ox, iy = y0 + oy, iz = z0 + oz;
+    if (ix >= gc.nn[0] || iy >= gc.ydet || iz >= gc.nn[2]) return;
+    T f[3] = {0, 0, 0};
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        // the element with this node at local corner l: local element coordinates (o + 1 - bit)
+        const int el = ((ox + 1 - ((l >> 2) & 1)) * WEY + (oy + 1 - ((l >> 1) & 1))) * WEZ + (oz + 1 - (l & 1));
+#pragma unroll
+        for (int a = 0; a < 3; ++a) f[a] += sY[el * 24 + 3 * l + a];
+    }
+    const int n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
+    store_out(y, b, N, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
 // The same row split over two work-items: work-item h of a pair takes the four elements with x bit i = h.
@@ -800,13 +940,15 @@ void launch_fine_apply(const Grid &g, const T *K0, const T *E, const T *u, T *y,
 }
 
 template <typename T>
-void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Ef, const T *u, T *y, const T *b,
-                     const uint8_t *dmask, int mode, bool reflect, hipStream_t s) {
+void launch_l1_apply(const Grid &gc, const Grid &gf, const T *Chat, const T *Cw, const T *Ef, const T *u, T *y,
+                     const T *b, const uint8_t *dmask, int mode, bool reflect, hipStream_t s) {
     const long count = (long)gc.nn[0] * gc.ydet * gc.nn[2];
     if (count > 0 && reflect) {
-        // one work-item per node: over all nodes the launch has waves enough, and the whole row per work-item
-        // measures faster than the split pair here
-        VF_DISPATCH_MODE(k_l1_apply_r, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask);
+        if (!Cw) throw std::runtime_error("the Walsh-basis level-1 product needs its table");
+        const int ntX = (gc.nn[0] + WX - 1) / WX, ntY = (gc.ydet + WY - 1) / WY, ntZ = (gc.nn[2] + WZ - 1) / WZ;
+        const long tiles = (long)ntX * ntY * ntZ;
+        if (tiles > INT_MAX) throw std::runtime_error("grid too large for the level-1 product");
+        VF_DISPATCH_MODE(k_l1_apply_w, T, mode, dim3((unsigned)tiles), gc, gf, Cw, Ef, u, y, b, dmask, ntY, ntZ);
     } else if (count > 0) {
         VF_DISPATCH_MODE(k_l1_apply, T, mode, dim3(nblocks(count)), gc, gf, Chat, Ef, u, y, b, dmask);
     }
@@ -916,8 +1058,8 @@ void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int 
     template void launch_zero_detached<T>(const Grid &, T *, hipStream_t);                                             \
     template void launch_fine_apply<T>(const Grid &, const T *, const T *, const T *, T *, const T *, const uint8_t *,  \
                                        int, hipStream_t);                                                              \
-    template void launch_l1_apply<T>(const Grid &, const Grid &, const T *, const T *, const T *, T *, const T *,       \
-                                     const uint8_t *, int, bool, hipStream_t);                                         \
+    template void launch_l1_apply<T>(const Grid &, const Grid &, const T *, const T *, const T *, const T *, T *,      \
+                                     const T *, const uint8_t *, int, bool, hipStream_t);                              \
     template void launch_block_apply<T>(const Grid &, const T *, const T *, T *, const T *, const uint8_t *, int,       \
                                         hipStream_t);                                                                  \
     template void launch_elem_apply<T>(const Grid &, const T *, const T *, T *, int, hipStream_t);                     \
