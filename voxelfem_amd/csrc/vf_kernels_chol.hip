@@ -83,8 +83,8 @@ __global__ void k_band_pad(int n, int nb, int kb, double *B) {
 // flag != 0 on a non-positive pivot.
 constexpr int PTPB = 4 * NB;
 constexpr int PQ = NB / 4;  // columns per work-item
-__global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
-                                                     int *__restrict__ flag) {
+// returns false (uniformly) on a non-positive pivot; the block is then left partly factored
+__device__ __forceinline__ bool potrf_block(int k, int kb, double *__restrict__ B, double *__restrict__ Linv) {
     __shared__ double sCol[2][NB];
     __shared__ double sRow[2][NB];
     __shared__ double sPiv[NB], sInvPiv[NB], sSqrt[NB], sInvSqrt[NB];
@@ -117,10 +117,7 @@ __global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__re
             for (int i = j % PQ + 1; i < PQ; ++i) a[i] -= f * sCol[b][c0 + i];
         }
     }
-    if (!ok) {
-        if (t == 0) atomicOr(flag, 1);
-        return;  // uniform: every work-item saw the same pivots
-    }
+    if (!ok) return false;  // uniform: every work-item saw the same pivots
     if (t < NB) {
         const double d = sPiv[t], sq = sqrt(d);
         sSqrt[t] = sq;
@@ -157,6 +154,11 @@ __global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__re
         Dk[r * NB + c] = c < r ? a[i] * sInvSqrt[c] : (c == r ? sSqrt[r] : 0.0);
         Lk[r * NB + c] = c <= r ? y[i] * isr : 0.0;
     }
+    return true;
+}
+__global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
+                                                     int *__restrict__ flag) {
+    if (!potrf_block(k, kb, B, Linv) && threadIdx.x == 0) atomicOr(flag, 1);
 }
 
 // C (+)= alpha * P * Q^T for 64 x 64 row-major blocks, 256 work-items with 4 x 4 register tiles.
