@@ -273,82 +273,6 @@ __global__ void __launch_bounds__(TPB) k_l1_gs(Grid gc, Grid gf, const T *__rest
     gs_update(u, N, n, bms, M, dm, forward);
 }
 
-// Level-1 node row from the fine moduli when the element matrix is reflection-symmetric (box element,
-// orthotropic tensor): Chat_fi[(l, a), (m, c)] = S_fi(a) S_fi(c) Chat_0[(l ^ fi, a), (m ^ fi, c)] with
-// S_fi(a) = -1 iff fi has the bit of axis a set. With g = ln ^ fi and m' = m ^ fi the row of the centre
-// node (local index ln = e in element e) reads
-//   f[a] = sum_g sum_e E_{e, e^g} S(a) sum_{m', c} Chat_0[(g, a), (m', c)] S(c) u_{e, m' ^ e ^ g, c},
-// so each Chat_0 entry (a uniform, scalar operand) serves the eight elements in turn, the scalar working set
-// is one child's 24 x 24 matrix, and the signs are free source negations. Absent elements read zero moduli.
-template <typename T, bool WANT_M>
-__device__ __forceinline__ void l1_row_refl(const Grid &gc, const Grid &gf, const T *__restrict__ C0,
-                                            const T *__restrict__ Ef, const T *__restrict__ u, int ix, int iy,
-                                            int iz, T f[3], T M[9]) {
-    const int N = gc.nnodes, sx = gc.nn[1] * gc.nn[2], sy = gc.nn[2];
-    const int fz = gf.ne[2], fyz = gf.ne[1] * gf.ne[2];
-    const int n = (ix * gc.nn[1] + iy) * gc.nn[2] + iz;
-    // one element at a time: its 24 nodal values and 8 child moduli are all a work-item holds besides the
-    // sums, which keeps the FP64 form at a few waves per SIMD (the node values come from L1/L2: the
-    // neighbouring work-items read the same nodes)
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-        const int i = (e >> 2) & 1, j = (e >> 1) & 1, k = e & 1;
-        const int ex = ix - i, ey = iy - j, ez = iz - k;
-        const bool ok = (unsigned)ex < (unsigned)gc.ne[0] && (unsigned)ey < (unsigned)gc.ne[1] &&
-                        (unsigned)ez < (unsigned)gc.ne[2];
-        if (!ok) continue;
-        T ue[8][3];
-#pragma unroll
-        for (int m = 0; m < 8; ++m) {
-            const int nm = n + (((m >> 2) & 1) - i) * sx + (((m >> 1) & 1) - j) * sy + ((m & 1) - k);
-#pragma unroll
-            for (int c = 0; c < 3; ++c) ue[m][c] = u[(size_t)c * N + nm];
-        }
-        const int fb = eidx(gf, 2 * ex, 2 * ey, 2 * ez);
-#pragma unroll
-        for (int g = 0; g < 8; ++g) {
-            const int fi = e ^ g;
-            const T Efi = Ef[fb + ((fi >> 2) & 1) * fyz + ((fi >> 1) & 1) * fz + (fi & 1)];
-            T t[3] = {0, 0, 0};
-#pragma unroll
-            for (int mp = 0; mp < 8; ++mp)
-#pragma unroll
-                for (int c = 0; c < 3; ++c) {
-                    const T uv = ((fi >> (2 - c)) & 1) ? -ue[mp ^ fi][c] : ue[mp ^ fi][c];
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) t[a] += C0[(3 * g + a) * 24 + 3 * mp + c] * uv;
-                }
-#pragma unroll
-            for (int a = 0; a < 3; ++a) f[a] += (((fi >> (2 - a)) & 1) ? -Efi : Efi) * t[a];
-            if (WANT_M) {
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) {
-                        const bool neg = (bool)((fi >> (2 - a)) & 1) != (bool)((fi >> (2 - c)) & 1);
-                        M[3 * a + c] += (neg ? -Efi : Efi) * C0[(3 * g + a) * 24 + 3 * g + c];
-                    }
-            }
-        }
-    }
-}
-
-template <typename T, int MODE>
-__global__ void __launch_bounds__(TPB) k_l1_apply_r(Grid gc, Grid gf, const T *__restrict__ C0,
-                                                    const T *__restrict__ Ef, const T *__restrict__ u,
-                                                    T *__restrict__ y, const T *__restrict__ b,
-                                                    const uint8_t *__restrict__ dmask) {
-    const int nz = gc.nn[2], slab = gc.ydet * nz;
-    const int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= gc.nn[0] * slab) return;
-    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
-    const int n = (ix * gc.nn[1] + iy) * nz + iz;
-    T f[3] = {0, 0, 0};
-    T M[9];
-    l1_row_refl<T, false>(gc, gf, C0, Ef, u, ix, iy, iz, f, M);
-    store_out(y, b, gc.nnodes, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
-}
-
 // ---- level-1 K u element by element in the Walsh basis (reflection-symmetric element matrix).
 // With the Walsh basis W[l, k] = prod_{axes a in k} s_a(l) (s_a(l) = +-1 the side of local node l along a,
 // per displacement component), the child matrices Chat_fi = R_fi Chat_0 R_fi (R_fi: reflection across the
@@ -432,7 +356,8 @@ __device__ __forceinline__ void l1_walsh_element(const T *__restrict__ Cw, const
         for (int a = 0; a < 3; ++a) v[l][a] = out[l][a];
 }
 
-// y (mode) K u over the nodes of rows below ydet, like k_l1_apply_r; grid: one workgroup per node tile
+// y (mode) K u over the nodes of rows below ydet (detached rows are left to the launcher); grid: one
+// workgroup per node tile
 template <typename T, int MODE>
 __global__ void __launch_bounds__(TPB) k_l1_apply_w(Grid gc, Grid gf, const T *__restrict__ Cw,
                                                     const T *__restrict__ Ef, const T *__restrict__ u,
@@ -487,7 +412,14 @@ __global__ void __launch_bounds__(TPB) k_l1_apply_w(Grid gc, Grid gf, const T *_
     store_out(y, b, N, n, f, MODE, dmask ? dmask[n] : uint8_t(0));
 }
 
-// The same row split over two work-items: work-item h of a pair takes the four elements with x bit i = h.
+// Level-1 node row from the fine moduli when the element matrix is reflection-symmetric (box element,
+// orthotropic tensor): Chat_fi[(l, a), (m, c)] = S_fi(a) S_fi(c) Chat_0[(l ^ fi, a), (m ^ fi, c)] with
+// S_fi(a) = -1 iff fi has the bit of axis a set. With g = ln ^ fi and m' = m ^ fi the row of the centre
+// node (local index ln = e in element e) reads
+//   f[a] = sum_g sum_e E_{e, e^g} S(a) sum_{m', c} Chat_0[(g, a), (m', c)] S(c) u_{e, m' ^ e ^ g, c},
+// so each Chat_0 entry (a uniform, scalar operand) serves the eight elements in turn, the scalar working set
+// is one child's 24 x 24 matrix, and the signs are free source negations. Absent elements read zero moduli.
+// The row is split over two work-items: work-item h of a pair takes the four elements with x bit i = h.
 // Its element loop stays unrolled over (j, k); the runtime x bit is folded away by loading the element's
 // node values in x-mirrored order when h = 1 (node m' holds local node m' ^ 4h, x component negated), so
 // Chat_0 rows, register indices and signs stay compile-time, and the remaining runtime sign (-1)^h is
