@@ -9,7 +9,9 @@ Prints one JSON line.
 import argparse
 import json
 import os
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -29,20 +31,52 @@ def parse():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--ku-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--no-lbl", action="store_true", help="skip the layer-by-layer batched-vs-sequential timing")
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0))))
     return ap.parse_args()
 
 
-def cpu_baseline(ne, rho_value, fp64_iters, threads):
-    """The oracle (CPU restatement of the reference algorithm) on the same workload: one PCG iteration
-    with setup and three PCG iterations are timed, and the solve time is extrapolated to the number of
-    iterations the FP64 GPU solve needed."""
+def _host_cpu():
+    """(model name, logical CPUs of the host, CPUs this process may run on)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, os.cpu_count(), len(os.sched_getaffinity(0))
+
+
+def _oracle_for_this_host():
+    """The oracle is timed on the host it runs on: build a copy tuned for this CPU (-march=native) into a
+    temporary directory; fall back to the in-tree, portable build if that fails."""
+    d = tempfile.mkdtemp(prefix="vf_oracle_")
+    out = os.path.join(d, "liboracle.so")
+    try:
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), f"OUT={out}", "ARCHFLAGS=-march=native"],
+                       check=True, capture_output=True, timeout=300)
+        return out, "built on the timing host with -O3 -march=native"
+    except Exception as e:  # noqa: BLE001
+        return None, f"in-tree portable build (-march=x86-64-v3; host build failed: {e})"
+
+
+def cpu_baseline(ne, rho_value, threads, gpu_solutions):
+    """The oracle (CPU restatement of the reference algorithm, FP64, OpenMP) on the same workload: one full
+    MG-PCG solve from zero (Galerkin setup, coarse Cholesky and every PCG iteration) timed end to end, the
+    solution compared with the GPU solutions, and the oracle's K.u throughput."""
     import numpy as np
 
     from oracle import oracle as orc
     from voxelfem_amd import workloads as wl
     from voxelfem_amd.mg_levels import optMGLevel
 
+    path, how = _oracle_for_this_host()
+    if path:
+        orc.use_library(path)
     orc.set_threads(threads)
     s = orc.Sim(ne)
     s.set_etensor(1.0, 0.3)
@@ -52,34 +86,67 @@ def cpu_baseline(ne, rho_value, fp64_iters, threads):
     f = wl.cantilever_load(ne)
     s.set_forces(f)
     mg = orc.MG(s, optMGLevel(ne, 3))
-    u0 = np.zeros_like(f)
     t0 = time.perf_counter()
-    mg.pcg(u0, f, 1, 1e-8)
-    t_one = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    mg.pcg(u0, f, 3, 1e-8)
-    t_three = time.perf_counter() - t0
-    per_iter = max((t_three - t_one) / 2.0, 1e-9)
-    est = t_one + per_iter * max(fp64_iters - 1, 0)
+    u, its = mg.pcg(np.zeros_like(f), f, 1000, 1e-8)
+    t_solve = time.perf_counter() - t0
+    c_ref = 0.5 * float(np.sum(f * u))
+    umax = float(np.abs(u).max())
+    errs = {}
+    for name, ug in gpu_solutions.items():
+        cg = 0.5 * float(np.sum(f * ug))
+        errs[name] = {"u_rel_err": float(np.abs(ug - u).max()) / umax, "compliance_rel_err": abs(cg - c_ref) / abs(c_ref)}
     # K.u throughput of the CPU restatement
-    u = np.random.default_rng(0).normal(size=f.shape)
-    s.apply_K(u)
-    t0 = time.perf_counter()
+    x = np.random.default_rng(0).normal(size=f.shape)
+    s.apply_K(x)
     reps = 3
+    t0 = time.perf_counter()
     for _ in range(reps):
-        s.apply_K(u)
+        s.apply_K(x)
     t_ku = (time.perf_counter() - t0) / reps
     nn = int(np.prod([n + 1 for n in ne]))
+    model, ncpu, avail = _host_cpu()
     return {
-        "value": est,
+        "value": t_solve,
         "unit": "s",
         "cores": orc.get_threads(),
         "kind": "port",
-        "sample": (f"oracle FP64 MG-PCG on the same {ne[0]}x{ne[1]}x{ne[2]} cantilever: PCG(1 iteration incl. "
-                   f"Galerkin setup + coarse Cholesky) {t_one:.2f} s and PCG(3 iterations) {t_three:.2f} s timed, "
-                   f"extrapolated to the {fp64_iters} iterations of the FP64 GPU solve"),
+        "sample": (f"oracle FP64 MG-PCG, one full solve from zero (setup + {its} iterations) of the same "
+                   f"{ne[0]}x{ne[1]}x{ne[2]} cantilever to 1e-8, {orc.get_threads()} OpenMP threads on '{model}' "
+                   f"({avail} of {ncpu} logical CPUs available to the process); oracle {how}"),
+        "iterations": its,
+        "compliance": c_ref,
+        "gpu_vs_oracle": errs,
         "ku_gdofs": 3 * nn / t_ku / 1e9,
     }
+
+
+def lbl_batched_vs_sequential(pv, precision):
+    """Layer-by-layer workload 160x80x80 (40 layers of 2 along the build axis, clamped base, self-weight,
+    smoothed density of seed 3): the 40 zero-guess layer solves and their gradients, run batched and one
+    after the other; seconds of the second of two runs each."""
+    import numpy as np
+
+    from voxelfem_amd import workloads as wl
+    from voxelfem_amd.mg_levels import optMGLevel
+
+    ne = wl.GRIDS["lbl_160x80x80"]
+    full = wl.configure(pv, ne, None, wl.smoothed_density(ne, 3))
+    sim = full.getIntermediateFabricationShape(1.0, False, pv.InterpolationLaw.SIMP)
+    mg = sim.multigridSolver(optMGLevel(ne, 3))
+    ev = pv.LayerByLayerEvaluator(sim)
+    ev.selectInitMethod("zero")
+    ev.precision = precision
+    out = {}
+    for name, batched in (("sequential", False), ("batched", True)):
+        ev.batched = batched
+        ev.run(mg, True, 2, 1000, 1e-8)
+        ev.run(mg, True, 2, 1000, 1e-8)
+        out[name] = {"s": ev.lastRunSeconds, "objective": ev.objective(),
+                     "pcg_iterations": int(np.sum(ev.perLayerIterations()))}
+    out["speedup"] = out["sequential"]["s"] / out["batched"]["s"]
+    out["objective_rel_diff"] = abs(out["batched"]["objective"] - out["sequential"]["objective"]) / abs(
+        out["sequential"]["objective"])
+    return out
 
 
 def main():
@@ -113,9 +180,13 @@ def main():
 
     L = capi.lib()
 
-    def setup():
-        # Galerkin hierarchy (level-1 operator, assembled block rows, coarsest dense FP64 inverse)
+    def setup(finish=False):
+        # Galerkin hierarchy (level-1 operator, assembled block rows, coarsest band Cholesky and inverse). The
+        # factorization is issued from a worker thread onto a second stream and overlaps the first cycle's
+        # descent inside a step; `finish` completes it (for timing the setup alone).
         capi.check(L.vf_mg_update_stiffness(mg_h))
+        if finish:
+            capi.check(L.vf_mg_finish_update(mg_h))
 
     def solve(prec):
         x.zero_()
@@ -131,6 +202,7 @@ def main():
         return (time.perf_counter() - t0) / reps, out
 
     results = {}
+    solutions = {}
     for name, prec in (("fp64", capi.VF_PREC_FP64), ("mixed", capi.VF_PREC_FP32)):
         for _ in range(args.warmup):
             setup()
@@ -142,7 +214,8 @@ def main():
         kd_n, kd_ms = capi.kernel_timing(mg_h)
         capi.set_kernel_timing(mg_h, False)
         comp = 0.5 * float(torch.dot(f, x).item())
-        t_setup, _ = timed(setup, 1)
+        solutions[name] = x.view(3, nn).t().contiguous().cpu().numpy()
+        t_setup, _ = timed(lambda: setup(True), 1)
         t_solve, _ = timed(lambda: solve(prec), args.steps)
         results[name] = {"s": dt, "setup_s": t_setup, "solve_s": t_solve, "iterations": int(stats.iterations),
                          "rel_residual": stats.residual_norm / stats.rhs_norm, "compliance": comp,
@@ -172,6 +245,14 @@ def main():
         rec = json.load(open(tfile))
         if tuple(rec.get("grid", ())) == ne:
             roof["traffic"] = rec["traffic_bytes"]
+            roof["traffic_source"] = ("not measured in this run: rocprofv3 FETCH_SIZE + WRITE_SIZE of the same kernel "
+                                      "on this grid, calibrated against a copy kernel, recorded in "
+                                      "profiles/ku_walsh_384x192x192_traffic.json")
+    # the compute side of the roofline of the reference's dense element formulation (F = 1152 flop per
+    # element in FP64; MI355X's FP64 vector and matrix peaks are the same 78.6 TFLOP/s)
+    if roof["bound"] == "mfma":
+        roof["bound_note"] = ("FP64 compute bound of the reference formulation (1152 flop/element); the kernel runs "
+                              "on the vector units, where the FP64 peak equals the matrix-core peak")
     # the same launch against the HBM roof alone (the Walsh-basis kernel does ~5x fewer flops than F)
     roof["hbm_frac"] = bytes64 / t64 / 1e9 / PEAK_HBM_GBS
     roof["kernel"] = "fine K.u, FP64, " + {"tiled": "k_ku_tiled<double,0> (LDS-tiled node rows)",
@@ -226,9 +307,11 @@ def main():
         "ku_gdofs": {"fp64": 3 * nn / t64 / 1e9, "fp32": 3 * nn / (ku_ms32 * 1e-3) / 1e9},
         "roofline": roof,
     }
+    if not args.no_lbl:
+        line["lbl_160x80x80"] = {"fp64": lbl_batched_vs_sequential(pv, "fp64"),
+                                 "mixed": lbl_batched_vs_sequential(pv, "fp32")}
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(ne, rho_value, results["fp64"]["iterations"], args.cpu_threads)
-        line["cpu_baseline"] = cb
+        line["cpu_baseline"] = cpu_baseline(ne, rho_value, args.cpu_threads, solutions)
     print(json.dumps(line), flush=True)
 
 

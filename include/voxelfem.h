@@ -104,8 +104,9 @@ int vf_sim_compliance_gradient(vf_sim *sim, const double *u, double *g, int wher
 int vf_sim_accumulate_compliance_gradient(vf_sim *sim, const double *u, double *g, int where);
 /* elementEnergyDensity (TensorProductSimulator.hh:1057-1073) */
 int vf_sim_energy_density(vf_sim *sim, const double *u, double *out, int where);
-/* compliance 1/2 f.u (TopologyOptimizationObjective.hh:41-43); masked=1 restricts to non-detached
- * nodes and drops the 1/2 (maskedNodalDotProduct, TensorProductSimulator.hh:445-447) */
+/* dot product a.b of two nodal fields (3*nn values): masked = 0 over every node (the compliance
+ * 1/2 f.u of TopologyOptimizationObjective.hh:41-43 is half of it), masked = 1 over the non-detached
+ * nodes only (maskedNodalDotProduct, TensorProductSimulator.hh:445-447) */
 int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int where, double *out);
 
 /* ---- multigrid: MultigridSolver (MultigridSolver.hh:35-121, binding VoxelFEM.cc:155-195) */
@@ -117,8 +118,16 @@ int vf_mg_level_dirichlet(vf_mg *mg, int64_t l, uint8_t *mask);
 /* setSymmetricGaussSeidel (MultigridSolver.hh:123-125) */
 int vf_mg_set_symmetric_gs(vf_mg *mg, int symmetric);
 /* updateStiffnessMatrices (MultigridSolver.hh:846-905): Galerkin hierarchy + coarse factorization,
-   queued on the solver stream (returns before the device work completes) */
+ * queued on the solver stream (returns before the device work completes; the factorization is issued
+ * from a worker thread onto a second stream). A non-positive-definite coarsest operator is reported by
+ * vf_mg_finish_update or by the next coarse solve, whichever comes first, and again by every later solve
+ * until the stiffness is updated. */
 int vf_mg_update_stiffness(vf_mg *mg);
+/* Completes the host side of the last vf_mg_update_stiffness: joins the worker thread, orders the
+ * solver's stream after the factorization and reports its failure (the reference's
+ * updateStiffnessMatrices reports it on return). A later vf_stream_sync of the solver's stream then
+ * covers the whole update. No-op when nothing is pending. */
+int vf_mg_finish_update(vf_mg *mg);
 /* setFabricationMaskHeightByLayer (MultigridSolver.hh:1022-1028) */
 int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer);
 /* applyK(l, u) (MultigridSolver.hh:471-504) */

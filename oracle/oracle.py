@@ -26,6 +26,15 @@ def build(force=False):
         subprocess.check_call(["make", "-s", "-C", _HERE])
 
 
+def use_library(path):
+    """Load the oracle from `path` (a copy built elsewhere, e.g. tuned for the timing host) instead of the
+    in-tree library; must be called before the first use."""
+    global _LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    _LIB_PATH = path
+
+
 def lib():
     global _lib
     if _lib is None:

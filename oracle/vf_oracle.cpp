@@ -909,19 +909,42 @@ void smoothing_filter(const long dims[3], double rmin, int type, const double *i
                 ws.push_back(w);
                 W += w;
             }
+    // flat offsets of the taps for voxels whose whole stencil lies inside the grid (no reflection):
+    // the same terms summed in the same order as the general path
+    std::vector<long> flat(offs.size());
+    for (size_t t = 0; t < offs.size(); ++t) flat[t] = (offs[t][0] * dims[1] + offs[t][1]) * dims[2] + offs[t][2];
     #pragma omp parallel for collapse(2)
     for (long i = 0; i < dims[0]; ++i)
-        for (long j = 0; j < dims[1]; ++j)
-            for (long k = 0; k < dims[2]; ++k) {
-                double s = 0;
+        for (long j = 0; j < dims[1]; ++j) {
+            const bool rowInside = i >= R && i + R < dims[0] && j >= R && j + R < dims[1];
+            const long row = (i * dims[1] + j) * dims[2];
+            // interior of the row: tap by tap over contiguous k (each output still sums its taps in order)
+            const long k0 = R, k1 = dims[2] - R;
+            if (rowInside && k1 > k0) {
+                std::vector<double> acc(dims[2], 0.0);
                 for (size_t t = 0; t < offs.size(); ++t) {
-                    const long ii = reflect_index(i + offs[t][0], dims[0]);
-                    const long jj = reflect_index(j + offs[t][1], dims[1]);
-                    const long kk = reflect_index(k + offs[t][2], dims[2]);
-                    s += ws[t] * in[(ii * dims[1] + jj) * dims[2] + kk];
+                    const double w = ws[t];
+                    const double *src = in + row + flat[t];
+                    for (long k = k0; k < k1; ++k) acc[k] += w * src[k];
                 }
-                out[(i * dims[1] + j) * dims[2] + k] = s / W;
+                for (long k = k0; k < k1; ++k) out[row + k] = acc[k] / W;
             }
+            for (long k = 0; k < dims[2]; ++k) {
+                const long o = row + k;
+                double s = 0;
+                if (rowInside && k >= k0 && k < k1) {
+                    continue;
+                } else {
+                    for (size_t t = 0; t < offs.size(); ++t) {
+                        const long ii = reflect_index(i + offs[t][0], dims[0]);
+                        const long jj = reflect_index(j + offs[t][1], dims[1]);
+                        const long kk = reflect_index(k + offs[t][2], dims[2]);
+                        s += ws[t] * in[(ii * dims[1] + jj) * dims[2] + kk];
+                    }
+                }
+                out[o] = s / W;
+            }
+        }
 }
 
 }  // namespace

@@ -4,11 +4,8 @@ TopologyOptimizationProblem + OCOptimizer API against the oracle's restatement.
 
 FD check (MBB half-beam 128x64x64, linear hat filter rmin=2.5, projection beta=8, SIMP p=3): 20
 seeded random directions v with N(0,1) entries, central differences with h=1e-6, solves to a relative
-residual of 1e-12, in FP64 and with the FP32 V-cycle. The error of a direction is |FD - g.v| relative to
-||g||_2, the RMS of g.v over such directions; it must be <= 1e-6 for every direction, and the median of
-|FD - g.v| / |g.v| must be <= 1e-6 as well. Per direction, |FD - g.v| / |g.v| can exceed 1e-6 only
-where |g.v| is a small fraction of ||g||: c(x+hv) - c(x-hv) = 2h g.v is then ~1e-8 of c, and the
-compliance of an FP64 solve carries an absolute error of ~1e-14 c (the printout shows both measures).
+residual of 1e-12, in FP64 and with the FP32 V-cycle. For every direction |FD - g.v| / |g.v| must be
+<= 1e-6 (and |FD - g.v| / ||g||_2 as well); the printout lists both per direction.
 """
 
 import numpy as np
@@ -108,8 +105,8 @@ def test_autograd_central_fd(vf, precision):
               f"|fd-g.v|/|g.v|={errs_dir[-1]:.2e} its={pipe.last_stats.iterations} (base {its0})")
     print(f"FD check {precision}: ||g||={gnorm:.4e} max |fd-g.v|/||g|| {max(errs):.3e}; |fd-g.v|/|g.v| "
           f"max {max(errs_dir):.3e} median {np.median(errs_dir):.3e}")
+    assert max(errs_dir) <= 1e-6
     assert max(errs) <= 1e-6
-    assert np.median(errs_dir) <= 1e-6
 
 
 def test_problem_oc_steps_match_oracle(vf):

@@ -311,3 +311,154 @@ def test_layer_by_layer_initial_guesses(pv):
     assert iters["N=3"] < iters["zero"]
     with pytest.raises(Exception, match="Unrecognized method"):
         pv.LayerByLayerEvaluator(lbl_sim).selectInitMethod("N=x")
+
+
+# ---------------------------------------------------------------- RAMP law (TensorProductSimulator.hh:2055-2060,
+# 997-1000): the law the reference's layer-by-layer drivers use by default (q = 3)
+def _ramp_pair(pv, ne, q=3.0, seed=5):
+    tps, s = make_pair(pv, ne, seed=seed)
+    tps.interpolationLaw = pv.InterpolationLaw.RAMP
+    tps.q = q
+    s.set_law(1, 1.0, 1e-9, 3.0, q)
+    return tps, s
+
+
+def test_ramp_young_and_gradient(pv):
+    ne = (16, 8, 8)
+    tps, s = _ramp_pair(pv, ne)
+    assert tps.interpolationLaw == pv.InterpolationLaw.RAMP and tps.q == 3.0
+    assert rel(tps.getYoungModulusScaleFactor(), s.E()) < 1e-14
+    u = RNG.normal(size=(tps.numNodes(), 3))
+    assert rel(tps.complianceGradient(u), s.compliance_gradient(u)) < 1e-12
+    tps.gravity = [0.0, -1.0, 0.0]
+    s.set_gravity([0.0, -1.0, 0.0])
+    assert rel(tps.buildLoadVector(), s.build_load()) < 1e-14
+    assert rel(tps.complianceGradient(u), s.compliance_gradient(u)) < 1e-12
+    # masked layers: moduli of masked elements are zero under either law
+    tps.setFabricationMaskHeightByLayer(5)
+    om = orc.MG(s, 1)
+    om.set_mask_layer(5)
+    assert rel(tps.getYoungModulusScaleFactor(), s.E()) < 1e-14
+    assert rel(tps.applyK(u), s.apply_K(u)) < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_ramp_pcg_matches_oracle(pv, precision):
+    ne = (32, 16, 16)
+    tps, s = _ramp_pair(pv, ne)
+    mg, om = tps.multigridSolver(2), orc.MG(s, 2)
+    f = tps.buildLoadVector()
+    u0 = np.zeros_like(f)
+    u = mg.preconditionedConjugateGradient(u0, f, 500, 1e-8, precision=precision)
+    ref, _ = om.pcg(u0, f, 500, 1e-8)
+    c, cref = 0.5 * np.sum(f * u), 0.5 * np.sum(f * ref)
+    if precision == "fp64":
+        assert rel(u, ref) < 1e-10
+        assert abs(c - cref) <= 1e-12 * abs(cref)
+    else:
+        assert abs(c - cref) <= 1e-9 * abs(cref)
+    g, gref = tps.complianceGradient(u), s.compliance_gradient(ref)
+    assert rel(g, gref) < (1e-10 if precision == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("method,batched", [("zero", True), ("zero", False), ("N=3", False)])
+def test_layer_by_layer_ramp(pv, method, batched):
+    """Layer-by-layer evaluation under RAMP: getIntermediateFabricationShape(..., law=RAMP) gives the
+    intermediate simulator the default RAMP factor q = 3 (TensorProductSimulator.hh:1876, 2138)."""
+    ne = (16, 8, 8)
+    rho = wl.noise_density(ne, 6)
+    full = wl.configure(pv, ne, None, rho)
+    full.q = 7.0  # not carried over to the intermediate shape
+    lbl_sim = full.getIntermediateFabricationShape(1.0, False, pv.InterpolationLaw.RAMP)
+    assert lbl_sim.interpolationLaw == pv.InterpolationLaw.RAMP and lbl_sim.q == 3.0
+    mg = lbl_sim.multigridSolver(2)
+    ev = pv.LayerByLayerEvaluator(lbl_sim)
+    ev.selectInitMethod(method)
+    ev.batched = batched
+    ev.run(mg, True, 2, 500, 1e-11)
+    s = orc.Sim(ne)
+    s.set_etensor(1.0, 0.3)
+    s.set_law(1, 1.0, 1e-9, 3.0, 3.0)
+    s.set_densities(rho)
+    X = wl._node_positions(ne)
+    s.set_dirichlet(np.where(X[:, 1] == 0, 7, 0).astype(np.uint8))
+    s.set_gravity([0.0, -1.0, 0.0])
+    obj, grad, comp, its = orc.MG(s, 2).lbl_run_zero(2, 500, 1e-11)
+    assert ev.objective() == pytest.approx(obj, rel=1e-9)
+    assert rel(ev.gradient(), grad) < 1e-8
+    assert np.allclose(ev.perLayerCompliance(), comp, rtol=1e-9, atol=0)
+
+
+# ---------------------------------------------------------------- OC update with non-finite ratios
+def test_oc_update_non_finite_ratios_match_oracle():
+    """OptimalityCriterion.hh:64-75 through vf_oc_update on device data: a NaN ratio (0/0, a positive
+    sensitivity under p = 0.5, 0 * inf) keeps x0, an infinite one is clamped to x0 + m."""
+    import ctypes
+
+    import torch
+
+    from voxelfem_amd import capi
+
+    n = 4096
+    x0 = RNG.uniform(0.0, 1.0, n)
+    dJ = -RNG.uniform(0.0, 2.0, n)
+    dc = np.full(n, -1.0 / n)
+    dJ[::7] = -dJ[::7]          # positive sensitivities: NaN ratios
+    dJ[3::11] = 0.0
+    dc[3::11] = 0.0             # 0/0
+    dc[5::13] = 0.0             # -1/0 = -inf, (-inf)^0.5 = inf
+    x0[5::26] = 0.0             # 0 * inf = NaN
+    dev = torch.device("cuda:0")
+    tx, tJ, tc = (torch.tensor(a, device=dev) for a in (x0, dJ, dc))
+    out = torch.empty_like(tx)
+    L = capi.lib()
+    L.vf_oc_update.argtypes = [ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                               ctypes.c_double, ctypes.c_double, ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+    capi.check(L.vf_oc_update(n, tx.data_ptr(), tJ.data_ptr(), tc.data_ptr(), 1.7, 0.2, 0.5, out.data_ptr(),
+                              capi.VF_DEVICE, None))
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    ref = orc.oc_update(x0, dJ, dc, 1.7, 0.2, 0.5)
+    assert np.array_equal(got, ref)
+    assert np.all(got[::7] == x0[::7])
+
+
+def test_oc_step_self_weight_matches_oracle(pv):
+    """One OC step of a self-weight problem (positive sensitivities where gravity dominates, i.e. NaN
+    ratios) through TopologyOptimizationProblem + OCOptimizer, against the oracle's OC step."""
+    import voxelfem_amd
+
+    ne = (16, 8, 8)
+    n = int(np.prod(ne))
+    vfrac = 0.3
+    x0 = wl.noise_density(ne, 8)
+    tps = wl.configure(pv, ne, wl.CANTILEVER_BC, x0)
+    tps.gravity = [0.0, 0.0, -0.05]
+    levels = voxelfem_amd.optMGLevel(ne, 3)
+    mg = tps.multigridSolver(levels)
+    obj = pv.MultigridComplianceObjective(mg)
+    obj.tol = 1e-12
+    obj.cgIter = 1000
+    top = pv.TopologyOptimizationProblem(tps, obj, [pv.TotalVolumeConstraint(vfrac)], [])
+    top.setVars(x0)
+    g = top.evaluateObjectiveGradient()
+    assert np.any(g > 0), "the test needs positive sensitivities"
+    s = orc.Sim(ne)
+    s.set_etensor(1.0, 0.3)
+    s.set_law(0, 1.0, 1e-9, 3.0, 3.0)
+    s.set_densities(x0)
+    s.set_dirichlet(wl.cantilever_mask(ne))
+    s.set_forces(wl.cantilever_load(ne))
+    s.set_gravity([0.0, 0.0, -0.05])
+    f = s.build_load()
+    u, _ = orc.MG(s, levels).pcg(np.zeros_like(f), f, 1000, 1e-12, 1, 2, True)
+    gref = s.compliance_gradient(u)
+    assert np.abs(g - gref).max() <= 1e-9 * np.abs(gref).max()
+    dc = np.full(n, -1.0 / (vfrac * n))
+    assert np.allclose(top.evaluateConstraintsJacobian()[0], dc, rtol=1e-12, atol=0)
+    pv.OCOptimizer(top).step()
+    xr = orc.OC().step(x0, gref, dc, vfrac, orc.Chain(ne))
+    got = top.getVars()
+    assert np.abs(got - xr).max() <= 1e-9
+    keep = gref > 0
+    assert np.all(got[keep] == x0[keep])

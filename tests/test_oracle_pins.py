@@ -331,6 +331,69 @@ def test_self_weight_gradient_matches_fd():
         assert g[e] == pytest.approx(fd, rel=2e-6, abs=1e-10)
 
 
+# ---------------------------------------------------------------- RAMP law (TensorProductSimulator.hh:2055-2060, 997-1000)
+def test_ramp_young_modulus_closed_forms():
+    """E(rho) = Emin + rho (E0 - Emin) / (1 + q (1 - rho)) at points where the value is known without the
+    formula: rho = 0 gives Emin, rho = 1 gives E0, q = 0 is the linear law (SIMP with p = 1), and
+    rho = 1/4, q = 3 gives Emin + (E0 - Emin)/13 (1 + q rho in the denominator would give 1/7, rho^q on top
+    1/64)."""
+    ne = (2, 2, 2)
+    E0, Emin = 2.0, 1e-3
+    rho = np.array([0.0, 1.0, 0.25, 0.5, 0.75, 0.1, 0.9, 0.3])
+    s = orc.Sim(ne)
+    s.set_etensor(1.0, 0.3)
+    s.set_law(1, E0, Emin, 3.0, 3.0)
+    s.set_densities(rho)
+    E = s.E()
+    assert E[0] == pytest.approx(Emin, rel=1e-15)
+    assert E[1] == pytest.approx(E0, rel=1e-15)
+    assert E[2] == pytest.approx(Emin + (E0 - Emin) / 13.0, rel=1e-15)
+    assert np.all(np.diff(E[[0, 5, 2, 7, 3, 4, 6, 1]]) > 0)  # increasing in rho
+    s.set_law(1, E0, Emin, 3.0, 0.0)
+    lin = s.E()
+    s.set_law(0, E0, Emin, 1.0, 3.0)
+    assert np.allclose(lin, s.E(), rtol=1e-15, atol=0)
+    assert np.allclose(lin, Emin + rho * (E0 - Emin), rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("gravity", [False, True])
+def test_ramp_compliance_gradient_matches_central_fd(gravity):
+    """dc/drho under RAMP q = 3 (the layer-by-layer drivers' default law), with and without self-weight,
+    against central differences of the compliance of dense solves."""
+    ne = (4, 2, 2) if not gravity else (2, 4, 2)
+    rho = RNG.uniform(0.3, 0.9, 16)
+    s = make_sim(ne, rho=rho)
+    s.set_law(1, 1.0, 1e-9, 3.0, 3.0)
+    nn = s.nn
+    idx = np.arange(s.nnodes)
+    ix, iy, iz = idx // (nn[1] * nn[2]), (idx // nn[2]) % nn[1], idx % nn[2]
+    if gravity:
+        mask = np.where(iy == 0, 7, 0).astype(np.uint8)
+        s.set_gravity([0.0, -1.0, 0.0])
+    else:
+        mask = np.where(ix == 0, 7, 0).astype(np.uint8)
+        f = np.zeros((s.nnodes, 3))
+        load = (ix == nn[0] - 1) & (iz == ne[2] // 2)
+        f[load, 2] = -1.0 / load.sum()
+        s.set_forces(f)
+    s.set_dirichlet(mask)
+
+    def comp(r):
+        s.set_densities(r)
+        f = s.build_load()
+        u = _reduced_solve(fr.assemble_K(ne, s.K0(), s.E()), mask, f)
+        return 0.5 * fr.to_dof(f) @ u, u
+
+    c0, u = comp(rho)
+    g = s.compliance_gradient(fr.from_dof(u))
+    h = 1e-6
+    for e in range(16):
+        rp, rm = rho.copy(), rho.copy()
+        rp[e] += h; rm[e] -= h
+        fd = (comp(rp)[0] - comp(rm)[0]) / (2 * h)
+        assert g[e] == pytest.approx(fd, rel=2e-6, abs=1e-10), e
+
+
 # ---------------------------------------------------------------- filters (TopologyOptimizationFilter.hh)
 def _dense_filter(dims, rmin, linear):
     n = int(np.prod(dims))
@@ -409,6 +472,20 @@ def test_oc_update_bounds():
     assert np.allclose(out, ref, atol=1e-15)
     dJ0 = dJ.copy(); dJ0[3] = 0.0; dc0 = dc.copy(); dc0[3] = 0.0  # 0/0 -> keep x0
     assert orc.oc_update(x0, dJ0, dc0, 5.0)[3] == x0[3]
+
+
+def test_oc_update_non_finite_ratios():
+    """OptimalityCriterion.hh:64-75: a NaN ratio (0/0, or a positive sensitivity under p = 0.5) keeps x0;
+    an infinite one is clamped to x0 + m like any large step."""
+    x0 = np.array([0.3, 0.3, 0.3, 0.0, 0.95])
+    dJ = np.array([0.0, 1.0, -1.0, -1.0, -1.0])
+    dc = np.array([0.0, -1.0, 0.0, 0.0, -1e-300])
+    out = orc.oc_update(x0, dJ, dc, 1.0, m=0.2, p=0.5)
+    assert out[0] == 0.3                 # 0/0
+    assert out[1] == 0.3                 # (-1)^0.5
+    assert out[2] == pytest.approx(0.5)  # +inf -> x0 + m
+    assert out[3] == 0.0                 # 0 * inf = NaN -> x0
+    assert out[4] == 1.0                 # huge -> x0 + m, then clipped to 1
 
 
 # ---------------------------------------------------------------- layer-by-layer (LayerByLayer.hh:223-300)

@@ -94,7 +94,7 @@ class CompliancePipeline:
                                    self.mg_iterations, self.smoothing, self.full_multigrid, self.precision)
         c = ctypes.c_double()
         capi.check(L.vf_sim_dot(self.sim_h, _ptr(self.f), _ptr(self.u), 0, capi.VF_DEVICE, ctypes.byref(c)))
-        return c.value
+        return 0.5 * c.value
 
     def compliance_gradient(self, rho, u):
         """dc/drho at densities rho for the equilibrium displacement u (both device tensors)."""

@@ -39,6 +39,7 @@ def lib():
         L.vf_projection.argtypes = [dbl, i64, vp, vp, i32, vp]
         L.vf_projection_backprop.argtypes = [dbl, i64, vp, vp, vp, i32, vp]
         L.vf_mg_update_stiffness.argtypes = [vp]
+        L.vf_mg_finish_update.argtypes = [vp]
         _lib = L
     return _lib
 

@@ -413,7 +413,7 @@ struct ComplianceObjective : Objective {
     double compliance() {
         double c;
         check(vf_sim_dot(tps->sim, f.p, u.p, 0, VF_DEVICE, &c));
-        return c;
+        return 0.5 * c;
     }
     double evaluate() override { return compliance(); }
     void gradient(DevVec &out) override {
@@ -772,10 +772,11 @@ void copyMaterial(const TPS &src, TPS &dst) {
     dst.young = src.young;
     dst.poisson = src.poisson;
     check(vf_sim_set_material(dst.sim, dst.young, dst.poisson));
+    // E_0, E_min and the SIMP exponent carry over; the RAMP factor and the law keep the new simulator's
+    // defaults (TensorProductSimulator.hh:1876-1884, 1940-1943)
     dst.E0 = src.E0;
     dst.Emin = src.Emin;
     dst.gamma = src.gamma;
-    dst.q = src.q;
 }
 
 // getIntermediateFabricationShape (TensorProductSimulator.hh:1864-1923)
@@ -1101,7 +1102,10 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 check(vf_mg_zero_dirichlet(mg.mg, l, us.data(), VF_HOST));
                 return soa_to_vfield(us.data(), nn);
             }, py::arg("l"), py::arg("u"))
-        .def("updateStiffnessMatrices", [](MG &mg) { check(vf_mg_update_stiffness(mg.mg)); })
+        .def("updateStiffnessMatrices", [](MG &mg) {
+            check(vf_mg_update_stiffness(mg.mg));
+            check(vf_mg_finish_update(mg.mg));  // failures reported here, as the reference does
+        })
         .def("setSymmetricGaussSeidel", [](MG &mg, bool s) { check(vf_mg_set_symmetric_gs(mg.mg, s ? 1 : 0)); }, py::arg("symmetric"))
         .def("solve", [](MG &mg, const Arr &u, const Arr &f, size_t numSteps, size_t nu, bool stiffnessUpdated,
                          bool zeroDirichlet, std::function<void(size_t, py::array_t<double>)> cb, bool fmg) {

@@ -1017,15 +1017,25 @@ struct vf_mg {
     }
     // the main stream waits for the coarsest factorization launched by the last update; a non-positive
     // pivot is reported here
+    // A failure leaves the hierarchy marked stale, so every later solve rebuilds and refactors (and reports
+    // the failure again) instead of running with the failed factor.
     void finish_factor() {
         if (!factorPending) return;
         factorPending = false;
-        join_factor();
+        try {
+            join_factor();
+        } catch (...) {
+            stiffnessReady = false;
+            throw;
+        }
         VF_HIP(hipStreamWaitEvent(st(), evFactor, 0));
         int flag = 0;
         d2h(&flag, cholFlag.data(), 1, fstream);
         sync(fstream);
-        if (flag) throw std::runtime_error("coarsest-level operator is not positive definite");
+        if (flag) {
+            stiffnessReady = false;
+            throw std::runtime_error("coarsest-level operator is not positive definite");
+        }
     }
     void ensure_factor() {
         if (!stiffnessReady) update_stiffness();
@@ -1724,7 +1734,7 @@ int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int wh
         *out = sim->read_scalar(sc + 9);
     } else {
         launch_dot_full(n, da, db, sim->red_partials(reduction_partials_full(n)), sc + 9, sim->stream);
-        *out = 0.5 * sim->read_scalar(sc + 9);
+        *out = sim->read_scalar(sc + 9);
     }
     VF_API_END
 }
@@ -1781,6 +1791,13 @@ int vf_mg_update_stiffness(vf_mg *mg) {
     // asynchronous on the solver's stream: later calls are ordered after it, and a failed factorisation
     // is reported by the first coarse solve
     mg->update_stiffness();
+    VF_API_END
+}
+
+int vf_mg_finish_update(vf_mg *mg) {
+    VF_API_BEGIN
+    need(mg, "null mg");
+    mg->finish_factor();
     VF_API_END
 }
 

@@ -469,8 +469,11 @@ __global__ void __launch_bounds__(TPB) k_oc_update(size_t n, const double *__res
     if (i >= n) return;
     const double x = x0[i];
     double r = x * pow(dJ[i] / (dc[i] * lambda), p);
-    r = fmin(fmax(fmin(fmax(r, x - m), x + m), 0.0), 1.0);
-    if (!isfinite(r)) r = x;
+    // The reference's clamps propagate NaN (std::max/std::min with the ratio as first operand, Eigen's
+    // cwiseMax/cwiseMin likewise) and a NaN ratio then keeps x0; fmax/fmin would drop it, so test it first.
+    // An infinite ratio is clamped to x0 +- m, as in the reference.
+    if (isnan(r)) r = x;
+    else r = fmin(fmax(fmin(fmax(r, x - m), x + m), 0.0), 1.0);
     out[i] = r;
 }
 
