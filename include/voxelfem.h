@@ -104,6 +104,20 @@ int vf_sim_compliance_gradient(vf_sim *sim, const double *u, double *g, int wher
 int vf_sim_accumulate_compliance_gradient(vf_sim *sim, const double *u, double *g, int where);
 /* elementEnergyDensity (TensorProductSimulator.hh:1057-1073) */
 int vf_sim_energy_density(vf_sim *sim, const double *u, double *out, int where);
+/* constantStrainLoad (TensorProductSimulator.hh:1119-1145, element load 84-113): the load of the constant strain
+ * eps (3x3 row-major, symmetric) through the material's isotropic tensor, each element weighted by its density
+ * (not its interpolated modulus). F: SoA 3*nn. */
+int vf_sim_constant_strain_load(vf_sim *sim, const double eps[9], double *F, int where);
+/* sampleNodalField (TensorProductSimulator.hh:1161-1179): u holds ncomp nodal planes of nn values, p the np query
+ * points (np x 3, row-major), out ncomp planes of np values. Each point is clamped to the domain and located in
+ * its element, whose trilinear interpolant is evaluated at the clamped point's coordinates themselves, as the
+ * reference does (it passes the point, not its reference coordinates in the element). */
+int vf_sim_sample_nodal_field(vf_sim *sim, const double *u, int64_t ncomp, const double *p, int64_t np, double *out,
+                              int where);
+/* setStretchings / getStretchings (TensorProductSimulator.hh:54-59,819-823,1779): element size per axis; the
+ * element stiffness, element volume and element lookup follow it, node positions do not. */
+int vf_sim_set_stretchings(vf_sim *sim, const double h[3]);
+int vf_sim_get_stretchings(vf_sim *sim, double h[3]);
 /* dot product a.b of two nodal fields (3*nn values): masked = 0 over every node (the compliance
  * 1/2 f.u of TopologyOptimizationObjective.hh:41-43 is half of it), masked = 1 over the non-detached
  * nodes only (maskedNodalDotProduct, TensorProductSimulator.hh:445-447) */
@@ -130,6 +144,13 @@ int vf_mg_update_stiffness(vf_mg *mg);
 int vf_mg_finish_update(vf_mg *mg);
 /* setFabricationMaskHeightByLayer (MultigridSolver.hh:1022-1028) */
 int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer);
+/* Partial stiffness updates after layers are removed from the top of the mask (m_partialStiffnessMatrixUpdate,
+ * MultigridSolver.hh:907-1017; on by default): the element matrices of the coarse element rows holding the newly
+ * masked fine layers are recomputed and the block rows of the nodes they touch reassembled, instead of the whole
+ * hierarchy; the coarsest level is refactored. enable = 0 rebuilds the whole hierarchy every time. The count is
+ * the number of partial updates made so far. */
+int vf_mg_set_partial_updates(vf_mg *mg, int enable);
+int vf_mg_partial_update_count(vf_mg *mg, int64_t *count);
 /* applyK(l, u) (MultigridSolver.hh:471-504) */
 int vf_mg_apply_K(vf_mg *mg, int64_t l, const double *u, double *y, int where);
 /* computeResidual (MultigridSolver.hh:527-541) */
@@ -142,6 +163,9 @@ int vf_mg_restrict(vf_mg *mg, int64_t l, const double *fine, double *coarse, int
 int vf_mg_interpolate(vf_mg *mg, int64_t l, const double *coarse, double *fine, int accumulate, int where);
 /* coarsest direct solve (TensorProductSimulator::solve, TensorProductSimulator.hh:1198-1230) */
 int vf_mg_coarse_solve(vf_mg *mg, const double *b, double *x, int where);
+/* debug_get_x / debug_get_b (MultigridSolver.hh:1158-1159): level l's iterate (which = 0) or right-hand side
+ * (which = 1) of the last multigrid cycle, 3*nn_l doubles (SoA; an FP32 cycle's buffers are widened). */
+int vf_mg_debug_get(vf_mg *mg, int64_t l, int which, double *out, int where);
 /* zeroOutDirichletComponents(l, u) (MultigridSolver.hh:507-513), in place */
 int vf_mg_zero_dirichlet(vf_mg *mg, int64_t l, double *u, int where);
 /* MultigridSolver::solve (MultigridSolver.hh:546-575): numSteps V-cycles (the first one a full
@@ -190,10 +214,17 @@ int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
 #define VF_LBL_INIT_ZERO 0
 #define VF_LBL_INIT_FD 1
 #define VF_LBL_INIT_SUBSPACE 2
+/* lblCallback(l, compliance, grad_compliance, u) after every layer solve (LayerByLayer.hh:222,277-279): l is the
+ * layer height, compliance = f_l.u_l, grad the layer's compliance gradient (numElements host doubles) and u
+ * its solution (3*numNodes host doubles, SoA), valid during the call. Return nonzero to abort the run. */
+typedef int (*vf_lbl_callback)(void *user, int64_t l, double compliance, const double *grad, const double *u);
+/* itcb (optional) is the PCG it_callback of every layer solve (as in vf_mg_pcg); lblcb (optional) the per-layer
+ * callback above. */
 int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations, int64_t smoothing,
                int fullMultigrid, int precision, int initMethod, int64_t history, int zeroInit, double *uFull,
                int uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
-               double *perLayerCompliance, int64_t *perLayerIterations);
+               double *perLayerCompliance, int64_t *perLayerIterations, vf_pcg_callback itcb, void *ituser,
+               vf_lbl_callback lblcb, void *lbluser);
 /* The same evaluation with the layer solves batched: the layers are spread over `pipelines`
  * concurrent device pipelines (each a copy of the simulator and hierarchy on its own stream, driven by
  * its own host thread; 0 = 8), and with useGraphs the steady-state PCG iteration of each layer solve

@@ -287,7 +287,7 @@ class MG:
     def lbl_run_zero(self, layer_increment, max_iter, tol, mg_iter=1, nu=1, fmg=False):
         s = self.sim
         grad = np.zeros(s.nelems)
-        nl = (s.ne[1] + layer_increment - 1) // layer_increment + 1
+        nl = (s.ne[1] + layer_increment - 1) // layer_increment  # layer solves: heights ne1, ne1 - inc, ...
         comp = np.zeros(nl)
         its = np.zeros(nl, dtype=np.int64)
         obj = ctypes.c_double()
@@ -396,3 +396,99 @@ class OC:
                 self.lmin = mid
             if v > 0:
                 self.lmax = mid
+
+
+# ---------------------------------------------------------------------------------------------------------
+# Boundary utilities of the simulator, restated in numpy (small grids only).
+
+def _node_grid(ne):
+    nn = [n + 1 for n in ne]
+    ix, iy, iz = np.meshgrid(np.arange(nn[0]), np.arange(nn[1]), np.arange(nn[2]), indexing="ij")
+    return nn, ix.ravel(), iy.ravel(), iz.ravel()
+
+
+def _elem_nodes(ne):
+    nn = [n + 1 for n in ne]
+    ex, ey, ez = np.meshgrid(np.arange(ne[0]), np.arange(ne[1]), np.arange(ne[2]), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    cols = []
+    for l in range(8):
+        i, j, k = (l >> 2) & 1, (l >> 1) & 1, l & 1
+        cols.append(((ex + i) * nn[1] + ey + j) * nn[2] + ez + k)
+    return np.stack(cols, axis=1)
+
+
+def constant_strain_load(ne, h, young, nu, rho, eps):
+    """TensorProductSimulator::constantStrainLoad (TensorProductSimulator.hh:1119-1145) with the element load of
+    constantStrainLoad / constantStressLoad (84-113): sigma = C : eps for the isotropic material tensor, element
+    e adds rho_e * V * int eps(phi_l e_c) : sigma to component c of its node l, i.e.
+    rho_e * sum_d sigma[c][d] * s_d(l) * V / (4 h_d). Returns the (nn, 3) load."""
+    eps = np.asarray(eps, dtype=float)
+    lam = young * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = young / (2 * (1 + nu))
+    sigma = lam * np.trace(eps) * np.eye(3) + 2 * mu * eps
+    V = h[0] * h[1] * h[2]
+    en = _elem_nodes(ne)
+    nn = int(np.prod([n + 1 for n in ne]))
+    F = np.zeros((nn, 3))
+    rho = np.asarray(rho, dtype=float)
+    for l in range(8):
+        s = np.array([1.0 if (l >> (2 - d)) & 1 else -1.0 for d in range(3)])
+        load_l = sigma @ (s * V / (4.0 * np.asarray(h, dtype=float)))  # (3,)
+        np.add.at(F, en[:, l], rho[:, None] * load_l[None, :])
+    return F
+
+
+def sample_nodal_field(ne, lo, hi, u, p):
+    """TensorProductSimulator::sampleNodalField (TensorProductSimulator.hh:1161-1179): clamp each point to the
+    domain, find its element (getElementNDIndex, 1731-1752) and evaluate the element's trilinear interpolant at
+    the clamped point's own coordinates q (the reference passes q, not the reference coordinates of q):
+    sum_l u[node(e, l)] prod_d (bit_d(l) ? q_d : 1 - q_d)."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    u = np.asarray(u, float)
+    if u.ndim == 1:
+        u = u[:, None]
+    nn = [n + 1 for n in ne]
+    h = (hi - lo) / np.asarray(ne, float)
+    out = np.zeros((len(p), u.shape[1]))
+    for i, pt in enumerate(np.asarray(p, float)):
+        q = np.minimum(np.maximum(pt, lo), hi)
+        e = []
+        for d in range(3):
+            f = (q[d] - lo[d]) / h[d]
+            e.append(ne[d] - 1 if abs(f - ne[d]) < 1e-10 else int(f))
+        val = 0.0
+        for l in range(8):
+            a, b, c = (l >> 2) & 1, (l >> 1) & 1, l & 1
+            w = (q[2] if c else 1 - q[2]) * (q[1] if b else 1 - q[1]) * (q[0] if a else 1 - q[0])
+            n = ((e[0] + a) * nn[1] + e[1] + b) * nn[2] + e[2] + c
+            val = val + u[n] * w
+        out[i] = val
+    return out
+
+
+def stiffness_upper(ne, K0, E):
+    """TensorProductSimulator::getK (TensorProductSimulator.hh:1506-1513, 833-865): the upper triangle (row <=
+    column, DoF 3 node + component) of sum_e E_e K0, as a scipy CSC matrix."""
+    import scipy.sparse as sp
+
+    en = _elem_nodes(ne)
+    dofs = (3 * en[:, :, None] + np.arange(3)[None, None, :]).reshape(len(en), 24)
+    rows = np.repeat(dofs, 24, axis=1).ravel()
+    cols = np.tile(dofs, (1, 24)).ravel()
+    vals = (np.asarray(E)[:, None, None] * np.asarray(K0)[None]).reshape(-1)
+    keep = rows <= cols
+    nn = int(np.prod([n + 1 for n in ne]))
+    return sp.csc_matrix((vals[keep], (rows[keep], cols[keep])), shape=(3 * nn, 3 * nn))
+
+
+def mesh(ne, lo, hi):
+    """TensorProductSimulator::getMesh (TensorProductSimulator.hh:747-776): node positions and the hexahedra with
+    the local node pairs (0,1), (3,2), (4,5), (7,6) (Gmsh order)."""
+    lo, hi = np.asarray(lo, float), np.asarray(hi, float)
+    nn, ix, iy, iz = _node_grid(ne)
+    h = (hi - lo) / np.asarray(ne, float)
+    V = np.stack([lo[0] + ix * h[0], lo[1] + iy * h[1], lo[2] + iz * h[2]], axis=1)
+    en = _elem_nodes(ne)
+    F = en[:, [0, 1, 3, 2, 4, 5, 7, 6]]
+    return V, F

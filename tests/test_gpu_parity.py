@@ -419,8 +419,23 @@ def test_oc_update_non_finite_ratios_match_oracle():
     torch.cuda.synchronize()
     got = out.cpu().numpy()
     ref = orc.oc_update(x0, dJ, dc, 1.7, 0.2, 0.5)
-    assert np.array_equal(got, ref)
-    assert np.all(got[::7] == x0[::7])
+    with np.errstate(all="ignore"):
+        raw = x0 * (dJ / (dc * 1.7)) ** 0.5
+    special = ~np.isfinite(raw)
+    # non-finite ratios: the kept x0 (NaN) or the clamped bound (inf), exactly
+    bad = np.flatnonzero(got[special] != ref[special])
+    print(f"{special.sum()} non-finite ratios, {bad.size} differ; finite entries max rel diff "
+          f"{np.max(np.abs(got[~special] - ref[~special]) / np.abs(ref[~special])):.2e}")
+    assert bad.size == 0, (x0[special][bad[:5]], got[special][bad[:5]], ref[special][bad[:5]])
+    # NaN ratios in C semantics (numpy's power gives NaN for (-inf)^0.5 where C's pow gives +inf): 0/0, a
+    # negative finite base, and 0 * inf; each keeps x0
+    with np.errstate(all="ignore"):
+        q = dJ / (dc * 1.7)
+    nan_ratio = np.isnan(q) | (np.isfinite(q) & (q < 0)) | ((x0 == 0) & np.isinf(q))
+    assert nan_ratio.sum() > 100
+    assert np.all(got[nan_ratio] == x0[nan_ratio])
+    # finite ratios: device pow and the host's may differ in the last bit
+    np.testing.assert_allclose(got[~special], ref[~special], rtol=1e-14, atol=0)
 
 
 def test_oc_step_self_weight_matches_oracle(pv):

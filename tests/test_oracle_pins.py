@@ -516,3 +516,87 @@ def test_masked_layer_solve_equals_truncated_grid():
         comps.append(fr.to_dof(f) @ u)
         assert comp[k] == pytest.approx(comps[-1], rel=1e-9)
     assert obj == pytest.approx(0.5 * sum(comps) / 3, rel=1e-9)
+
+
+# ---------------------------------------------------------------------------------------- boundary utilities
+def _node_positions(ne, lo, hi):
+    nn = [n + 1 for n in ne]
+    h = (np.asarray(hi, float) - np.asarray(lo, float)) / np.asarray(ne, float)
+    ix, iy, iz = np.meshgrid(*[np.arange(n) for n in nn], indexing="ij")
+    return np.stack([lo[0] + ix.ravel() * h[0], lo[1] + iy.ravel() * h[1], lo[2] + iz.ravel() * h[2]], axis=1), h
+
+
+@pytest.mark.parametrize("uniform", [True, False])
+def test_constant_strain_load_is_K_times_the_linear_field(uniform):
+    """For the linear displacement u(x) = eps x, sum_e rho_e K0_e u_e = sum_e rho_e int B^T C eps: the constant-strain
+    load with densities rho equals the stiffness assembled with moduli rho applied to u (brute-force scipy
+    assembly), which pins the element load's signs, scaling and node placement."""
+    rng = np.random.default_rng(5)
+    ne, lo, hi = (3, 2, 4), np.array([0.0, -1.0, 0.5]), np.array([3.0, 0.0, 2.5])
+    X, h = _node_positions(ne, lo, hi)
+    young, nu = 1.7, 0.3
+    K0 = orc.element_stiffness(young, nu, h)
+    rho = np.ones(int(np.prod(ne))) if uniform else rng.uniform(0.1, 1.0, int(np.prod(ne)))
+    A = rng.normal(size=(3, 3))
+    eps = 0.5 * (A + A.T)
+    u = X @ eps.T
+    K = fr.assemble_K(ne, K0, rho)
+    ref = (K @ u.reshape(-1)).reshape(-1, 3)
+    got = orc.constant_strain_load(ne, h, young, nu, rho, eps)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def test_sample_nodal_field_closed_form_for_linear_fields():
+    """The element interpolant of a linear field u(x) = a + B x in reference coordinates xi is u(x_e0 + h xi); the
+    reference evaluates it at the clamped point q itself, so the sample is a + B (x_e0 + h q), x_e0 the first
+    node of the element that holds q (points on a far face belong to the last element)."""
+    rng = np.random.default_rng(6)
+    ne, lo, hi = (4, 3, 2), np.array([0.5, -1.0, 0.0]), np.array([2.5, 2.0, 1.0])
+    X, h = _node_positions(ne, lo, hi)
+    a, B = rng.normal(size=3), rng.normal(size=(3, 3))
+    u = a[None, :] + X @ B.T
+    pts = np.concatenate([rng.uniform(lo - 0.3, hi + 0.3, size=(40, 3)), hi[None, :], lo[None, :],
+                          np.array([[2.5, 0.25, 0.5], [1.0, 2.0, 1.0]])])
+    got = orc.sample_nodal_field(ne, lo, hi, u, pts)
+    for p, g in zip(pts, got):
+        q = np.clip(p, lo, hi)
+        e = np.minimum(np.floor((q - lo) / h + 1e-12), np.asarray(ne) - 1)
+        x0 = lo + e * h
+        assert np.allclose(g, a + B @ (x0 + h * q), rtol=0, atol=1e-12)
+
+
+def test_sample_nodal_field_is_interpolation_on_a_unit_cell():
+    """On the unit cube with one element the reference coordinates equal the point: trilinear interpolation."""
+    rng = np.random.default_rng(7)
+    u = rng.normal(size=(8, 2))
+    pts = rng.uniform(0, 1, size=(10, 3))
+    got = orc.sample_nodal_field((1, 1, 1), [0, 0, 0], [1, 1, 1], u, pts)
+    for p, g in zip(pts, got):
+        w = [(p[0] if (l >> 2) & 1 else 1 - p[0]) * (p[1] if (l >> 1) & 1 else 1 - p[1]) * (p[2] if l & 1 else 1 - p[2])
+             for l in range(8)]
+        assert np.allclose(g, np.asarray(w) @ u, atol=1e-14)
+
+
+def test_stiffness_upper_triangle():
+    rng = np.random.default_rng(8)
+    ne = (3, 2, 2)
+    K0 = orc.element_stiffness(1.0, 0.3, np.ones(3))
+    E = rng.uniform(0.1, 1, int(np.prod(ne)))
+    full = fr.assemble_K(ne, K0, E).toarray()
+    up = orc.stiffness_upper(ne, K0, E).toarray()
+    assert np.allclose(up, np.triu(full), atol=1e-14, rtol=0)
+    assert np.allclose(up + up.T - np.diag(np.diag(up)), full, atol=1e-14, rtol=0)
+
+
+def test_mesh_hexahedra_in_gmsh_order():
+    """Gmsh hexahedron: nodes 0-3 one face in cyclic order, 4-7 the opposite face, 4+k above k."""
+    ne, lo, hi = (2, 3, 2), [0, 0, 0], [2.0, 1.5, 1.0]
+    V, F = orc.mesh(ne, lo, hi)
+    h = np.array([1.0, 0.5, 0.5])
+    for f in F:
+        P = V[f]
+        for k in range(4):
+            assert np.isclose(np.abs(P[(k + 1) % 4] - P[k]).sum(), h[1] if k % 2 else h[2]) or \
+                np.isclose(np.abs(P[(k + 1) % 4] - P[k]).sum(), h[2] if k % 2 else h[1])
+            assert np.allclose(P[4 + k] - P[k], [h[0], 0, 0])
+        assert np.allclose(P.min(0), P[0]) and np.allclose(P.max(0), P[6])

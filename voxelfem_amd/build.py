@@ -4,6 +4,7 @@
                           of ``include/voxelfem.h`` (hipcc).
 * ``pyVoxelFEM*.so``   -- pybind11 module mirroring the reference's ``pyVoxelFEM`` API on top of the
                           C ABI (g++; no HIP headers, no torch types).
+* ``pyOptimizer*.so``  -- pybind11 module with the reference's ``pyOptimizer.MMA`` (host C++, OpenMP).
 
 Run ``python -m voxelfem_amd.build`` (or ``__graft_entry__.build()``). Every call recompiles.
 """
@@ -80,10 +81,29 @@ def build_pybind(verbose=False):
     return out
 
 
+def build_optimizer(verbose=False):
+    """pyOptimizer: the host-side MMA optimiser (no GPU code)."""
+    import pybind11
+
+    src = os.path.join(CSRC, "pyoptimizer.cpp")
+    out = os.path.join(HERE, "pyOptimizer" + ext_suffix())
+    os.makedirs(BUILD, exist_ok=True)
+    tmp = os.path.join(BUILD, "pyOptimizer.so.tmp")
+    py_inc = sysconfig.get_paths()["include"]
+    cmd = ["g++", "-O3", "-shared", "-fPIC", "-std=c++17", "-fopenmp", "-fvisibility=hidden", "-I", py_inc, "-I",
+           pybind11.get_include(), src, "-o", tmp]
+    o = _run(cmd)
+    if verbose and o.strip():
+        print(o)
+    shutil.move(tmp, out)
+    return out
+
+
 def build(verbose=False):
     lib = build_library(verbose=verbose)
     mod = build_pybind(verbose=verbose)
-    return lib, mod
+    opt = build_optimizer(verbose=verbose)
+    return lib, mod, opt
 
 
 if __name__ == "__main__":

@@ -143,6 +143,12 @@ struct TPS : std::enable_shared_from_this<TPS> {
 
     std::array<int64_t, 3> nnodes3() const { return {ne[0] + 1, ne[1] + 1, ne[2] + 1}; }
     std::array<double, 3> h() const { return {(hi[0] - lo[0]) / ne[0], (hi[1] - lo[1]) / ne[1], (hi[2] - lo[2]) / ne[2]}; }
+    // element stretchings (the node spacing h() unless set through the dx property)
+    std::array<double, 3> stretch() const {
+        std::array<double, 3> s;
+        check(vf_sim_get_stretchings(sim, s.data()));
+        return s;
+    }
     std::array<double, 3> nodePos(size_t n) const {
         const auto N = nnodes3();
         const size_t iz = n % N[2], iy = (n / N[2]) % N[1], ix = n / (N[1] * N[2]);
@@ -215,6 +221,7 @@ struct MG {
     vf_mg *mg = nullptr;
     int64_t levels = 0;
     bool useGraphs = false;
+    bool partialUpdates = true;
     MG(TPSPtr t, int64_t L) : tps(std::move(t)), levels(L) { check(vf_mg_create(&mg, tps->sim, L)); }
     ~MG() { if (mg) vf_mg_destroy(mg); }
     size_t nodes(int64_t l) const {
@@ -656,20 +663,20 @@ struct LBL {
     void run(MG &solver, bool zeroInit, size_t layerIncrement, size_t maxIter, double tol,
              std::function<void(size_t, py::array_t<double>, py::array_t<double>)> it_cb, size_t mgIterations,
              size_t mgSmoothingIterations, bool fmg, bool verbose, py::object lblCallback) {
-        if (it_cb) throw std::runtime_error("per-iteration callbacks are not supported by the batched layer evaluator");
-        if (!lblCallback.is_none()) throw std::runtime_error("lblCallback is not supported by the batched layer evaluator");
+        const bool callbacks = (bool)it_cb || !lblCallback.is_none();
         const size_t numLayers = (size_t)tps->ne[1];
         const size_t nl = (numLayers + layerIncrement - 1) / layerIncrement;
         grad.assign(tps->nel, 0.0);
-        perLayer.assign(nl + 1, 0.0);
-        perLayerIters.assign(nl + 1, 0);
+        perLayer.assign(nl, 0.0);
+        perLayerIters.assign(nl, 0);
         double obj = 0;
         const auto t0 = std::chrono::steady_clock::now();
         // the zero guess makes the layer solves independent, so they can run batched; the other guesses
         // chain each layer to the previous solutions and run one after the other
         const size_t nDof = 3 * (size_t)tps->nn;
         const bool fullValid = uFullDesign.size() == nDof;
-        if (batched && initMethod == VF_LBL_INIT_ZERO && (zeroInit || !fullValid)) {
+        // callbacks see every iterate and layer in order: they run with the sequential evaluator
+        if (batched && !callbacks && initMethod == VF_LBL_INIT_ZERO && (zeroInit || !fullValid)) {
             check(vf_lbl_run_zero_batched(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol,
                                           (int64_t)mgIterations, (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision,
                                           pipelines, useGraphs ? 1 : 0, &obj, grad.data(), VF_HOST, perLayer.data(),
@@ -677,10 +684,26 @@ struct LBL {
         } else {
             std::vector<double> uf(nDof);
             if (fullValid) uf = uFullDesign;
+            struct Ctx {
+                std::function<void(size_t, py::array_t<double>, py::array_t<double>)> *it;
+                py::object *lbl;
+                size_t nn, ne;
+            } ctx{&it_cb, &lblCallback, (size_t)tps->nn, (size_t)tps->nel};
+            auto itr = [](void *user, int64_t it, const double *xx, const double *rr) -> int {
+                auto *c = (Ctx *)user;
+                (*c->it)((size_t)it, soa_to_vfield(xx, c->nn), soa_to_vfield(rr, c->nn));
+                return 0;
+            };
+            auto lbr = [](void *user, int64_t l, double compliance, const double *g, const double *u) -> int {
+                auto *c = (Ctx *)user;
+                (*c->lbl)((size_t)l, compliance, from_vec(std::vector<double>(g, g + c->ne)), soa_to_vfield(u, c->nn));
+                return 0;
+            };
             check(vf_lbl_run(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol, (int64_t)mgIterations,
                              (int64_t)mgSmoothingIterations, fmg ? 1 : 0, precision, initMethod, history,
                              zeroInit ? 1 : 0, uf.data(), fullValid ? 1 : 0, VF_HOST, &obj, grad.data(), VF_HOST,
-                             perLayer.data(), perLayerIters.data()));
+                             perLayer.data(), perLayerIters.data(), it_cb ? +itr : nullptr, &ctx,
+                             lblCallback.is_none() ? nullptr : +lbr, &ctx));
             uFullDesign.swap(uf);
         }
         seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -780,6 +803,20 @@ void copyMaterial(const TPS &src, TPS &dst) {
 }
 
 // getIntermediateFabricationShape (TensorProductSimulator.hh:1864-1923)
+// validatedIntermediateFabricationShapeSize (TensorProductSimulator.hh:1990-1999): an intermediate shape has
+// this grid's element counts except along the build direction (axis 1), where it has at most as many
+void checkIntermediateSize(const TPS &full, const TPS &part) {
+    if (part.ne[0] != full.ne[0] || part.ne[2] != full.ne[2] || part.ne[1] > full.ne[1])
+        throw std::runtime_error("Intermediate shape is of unexpected size");
+}
+
+// isotropic ElasticityTensor value of the ETensor property (MeshFEM's tensor, restricted to the isotropic
+// materials this build supports)
+struct IsoETensor {
+    double E = 1, nu = 0;
+    void setIsotropic(double e, double n) { E = e; nu = n; }
+};
+
 TPSPtr intermediateShape(const TPS &t, double hfrac, bool validate, InterpolationLaw law) {
     if (hfrac < 0 || hfrac > 1) throw std::runtime_error("hfrac is out of bounds");
     const double fh = hfrac * (double)t.ne[1];
@@ -828,6 +865,22 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
     py::enum_<NumberType>(m, "NumberType").value("DOUBLE", NumberType::DOUBLE).value("FLOAT", NumberType::FLOAT);
 
     // ------------------------------------------------------------------ TensorProductSimulator
+    py::class_<IsoETensor>(m, "ElasticityTensor")
+        .def(py::init<>())
+        .def(py::init([](double E, double nu) { IsoETensor e; e.setIsotropic(E, nu); return e; }), py::arg("E"), py::arg("nu"))
+        .def("setIsotropic", &IsoETensor::setIsotropic, py::arg("E"), py::arg("nu"))
+        .def_readonly("E", &IsoETensor::E)
+        .def_readonly("nu", &IsoETensor::nu)
+        .def("__len__", [](const IsoETensor &) { return 2; })
+        .def("__getitem__", [](const IsoETensor &e, int i) {
+                if (i == 0 || i == -2) return e.E;
+                if (i == 1 || i == -1) return e.nu;
+                throw py::index_error();
+            })
+        .def("__repr__", [](const IsoETensor &e) {
+                return "ElasticityTensor(isotropic, E=" + std::to_string(e.E) + ", nu=" + std::to_string(e.nu) + ")";
+            });
+
     py::class_<TPS, TPSPtr>(detail, "TensorProductSimulator1_1_1")
         .def("numNodes", [](const TPS &t) { return t.nn; })
         .def("numElements", [](const TPS &t) { return t.nel; })
@@ -987,19 +1040,169 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def_property("q", [](const TPS &t) { return t.q; }, [](TPS &t, double v) { t.q = v; t.push_law(); })
         .def_property("gravity", [](const TPS &t) { return t.grav; },
                       [](TPS &t, const std::array<double, 3> &g) { t.grav = g; check(vf_sim_set_gravity(t.sim, g.data())); })
-        .def_property("ETensor", [](const TPS &t) { return std::make_pair(t.young, t.poisson); },
-                      [](TPS &t, const std::pair<double, double> &ep) {
-                          t.young = ep.first; t.poisson = ep.second;
+        .def_property("ETensor", [](const TPS &t) { IsoETensor e; e.E = t.young; e.nu = t.poisson; return e; },
+                      [](TPS &t, py::object o) {
+                          if (py::isinstance<IsoETensor>(o)) {
+                              auto e = o.cast<IsoETensor>();
+                              t.young = e.E; t.poisson = e.nu;
+                          } else {
+                              auto ep = o.cast<std::pair<double, double>>();
+                              t.young = ep.first; t.poisson = ep.second;
+                          }
                           check(vf_sim_set_material(t.sim, t.young, t.poisson));
-                      }, "Isotropic elasticity tensor as (Young's modulus, Poisson ratio)")
-        .def_property_readonly("dx", [](const TPS &t) { return t.h(); })
-        .def_property_readonly("elementVolume", [](const TPS &t) { auto h = t.h(); return h[0] * h[1] * h[2]; })
+                      }, "Isotropic elasticity tensor (an ElasticityTensor, or a (Young's modulus, Poisson ratio) pair)")
+        .def_property("dx", [](const TPS &t) { return t.stretch(); },
+                      [](TPS &t, const std::array<double, 3> &h) { check(vf_sim_set_stretchings(t.sim, h.data())); },
+                      "Get the dimensions of a grid cell")
+        .def_property_readonly("elementVolume", [](const TPS &t) { auto h = t.stretch(); return h[0] * h[1] * h[2]; })
         .def("elementEnergyDensity", [](const TPS &t, const Arr &u) {
                 auto us = vfield_to_soa(u, t.nn);
                 std::vector<double> out(t.nel);
                 check(vf_sim_energy_density(t.sim, us.data(), out.data(), VF_HOST));
                 return from_vec(out);
             }, py::arg("u"))
+        .def("getK", [](const TPS &t) {
+                // getK (TensorProductSimulator.hh:1506-1513, assembly 833-865): E_e K0 summed into the upper
+                // triangle (row <= column, DoF 3 node + component) of the stiffness matrix, as a scipy CSC matrix
+                const auto K = t.K0();
+                const auto E = t.getE();
+                const size_t nt = t.nel * 300;
+                py::array_t<double> vals((py::ssize_t)nt);
+                py::array_t<int64_t> rows((py::ssize_t)nt), cols((py::ssize_t)nt);
+                double *v = vals.mutable_data();
+                int64_t *ri = rows.mutable_data(), *ci = cols.mutable_data();
+                size_t k = 0;
+                for (size_t e = 0; e < t.nel; ++e) {
+                    size_t dofs[24];
+                    for (int l = 0; l < 8; ++l)
+                        for (int c = 0; c < 3; ++c) dofs[3 * l + c] = 3 * t.elemNode(e, l) + c;
+                    for (int a = 0; a < 24; ++a)
+                        for (int b = 0; b < 24; ++b) {
+                            if (dofs[a] > dofs[b]) continue;
+                            v[k] = E[e] * K[a * 24 + b];
+                            ri[k] = (int64_t)dofs[a];
+                            ci[k] = (int64_t)dofs[b];
+                            ++k;
+                        }
+                }
+                py::module_ sp = py::module_::import("scipy.sparse");
+                const py::ssize_t n = (py::ssize_t)(3 * t.nn);
+                return sp.attr("csc_matrix")(py::make_tuple(vals, py::make_tuple(rows, cols)), py::make_tuple(n, n));
+            })
+        .def("constantStrainLoad", [](const TPS &t, const Arr &eps) {
+                // the strain as a 3x3 symmetric matrix or as the 6 values (xx, yy, zz, yz, xz, xy)
+                double e[9];
+                if (eps.ndim() == 2 && eps.shape(0) == 3 && eps.shape(1) == 3) {
+                    auto r = eps.unchecked<2>();
+                    for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) e[3 * a + b] = r(a, b);
+                } else if (eps.size() == 6) {
+                    const double *q = eps.data();
+                    e[0] = q[0]; e[4] = q[1]; e[8] = q[2];
+                    e[5] = e[7] = q[3];
+                    e[2] = e[6] = q[4];
+                    e[1] = e[3] = q[5];
+                } else {
+                    throw std::runtime_error("constantStrainLoad: eps must be a 3x3 symmetric matrix or 6 values");
+                }
+                std::vector<double> F(3 * t.nn);
+                check(vf_sim_constant_strain_load(t.sim, e, F.data(), VF_HOST));
+                return soa_to_vfield(F.data(), t.nn);
+            }, py::arg("eps"))
+        .def("sampleNodalField", [](TPS &t, const Arr &u, const Arr &p) {
+                if (u.ndim() < 1 || (size_t)u.shape(0) != t.nn) throw std::runtime_error("u must have one row per node");
+                if (p.ndim() != 2 || p.shape(1) != 3) throw std::runtime_error("p must hold one 3D point per row");
+                const int64_t k = u.ndim() == 1 ? 1 : (int64_t)u.shape(1);
+                const int64_t np = (int64_t)p.shape(0);
+                std::vector<double> us((size_t)k * t.nn), ps(3 * (size_t)np), out((size_t)k * np);
+                auto uu = u.unchecked();
+                for (size_t n = 0; n < t.nn; ++n)
+                    for (int64_t c = 0; c < k; ++c)
+                        us[(size_t)c * t.nn + n] = u.ndim() == 1 ? u.data()[n] : u.at(n, c);
+                (void)uu;
+                std::memcpy(ps.data(), p.data(), ps.size() * sizeof(double));
+                check(vf_sim_sample_nodal_field(t.sim, us.data(), k, ps.data(), np, out.data(), VF_HOST));
+                py::array_t<double> res({(py::ssize_t)np, (py::ssize_t)k});
+                auto w = res.mutable_unchecked<2>();
+                for (int64_t i = 0; i < np; ++i)
+                    for (int64_t c = 0; c < k; ++c) w(i, c) = out[(size_t)c * np + i];
+                return res;
+            }, py::arg("u"), py::arg("p"), "Sample the nodal FEM field `u` at the query points specified by the rows of `p`")
+        .def("transferDensitiesToIntermediateFabricationShape", [](const TPS &t, TPS &part) {
+                checkIntermediateSize(t, part);
+                const auto rho = t.getDensities();
+                std::vector<double> r(part.nel);
+                for (size_t e = 0; e < part.nel; ++e) {
+                    const size_t ez = e % part.ne[2], ey = (e / part.ne[2]) % part.ne[1], ex = e / (part.ne[1] * part.ne[2]);
+                    r[e] = rho[(ex * t.ne[1] + ey) * t.ne[2] + ez];
+                }
+                part.setDensities(r);
+            }, py::arg("intermediateTPS"))
+        .def("transferVFieldToIntermediateFabricationShape", [](const TPS &t, const TPS &part, const Arr &u) {
+                checkIntermediateSize(t, part);
+                if (u.ndim() != 2 || (size_t)u.shape(0) != t.nn || u.shape(1) != 3)
+                    throw std::runtime_error("expected a nodal field of shape (" + std::to_string(t.nn) + ", 3)");
+                const auto N = t.nnodes3(), P = part.nnodes3();
+                py::array_t<double> res({(py::ssize_t)part.nn, (py::ssize_t)3});
+                auto w = res.mutable_unchecked<2>();
+                auto r = u.unchecked<2>();
+                for (int64_t ix = 0; ix < P[0]; ++ix)
+                    for (int64_t iy = 0; iy < P[1]; ++iy)
+                        for (int64_t iz = 0; iz < P[2]; ++iz) {
+                            const size_t src = (ix * N[1] + iy) * N[2] + iz, dst = (ix * P[1] + iy) * P[2] + iz;
+                            for (int c = 0; c < 3; ++c) w(dst, c) = r(src, c);
+                        }
+                return res;
+            }, py::arg("intermediateTPS"), py::arg("u"))
+        .def("accumElementScalarFieldFromIntermediateFabricationShape",
+             [](const TPS &t, const TPS &part, const Arr &rho_in, py::array_t<double> rho_accum) {
+                 // rho_accum is updated in place (the reference binds it as an Eigen::Ref)
+                 checkIntermediateSize(t, part);
+                 if ((size_t)rho_in.size() != part.nel) throw std::runtime_error("rho_in size mismatch");
+                 if ((size_t)rho_accum.size() != t.nel || rho_accum.ndim() != 1 || !rho_accum.writeable() ||
+                     rho_accum.strides(0) != (py::ssize_t)sizeof(double))
+                     throw std::runtime_error("rho_accum must be a writable contiguous float64 vector of numElements values");
+                 double *acc = rho_accum.mutable_data();
+                 const double *in = rho_in.data();
+                 for (size_t e = 0; e < part.nel; ++e) {
+                     const size_t ez = e % part.ne[2], ey = (e / part.ne[2]) % part.ne[1], ex = e / (part.ne[1] * part.ne[2]);
+                     acc[(ex * t.ne[1] + ey) * t.ne[2] + ez] += in[e];
+                 }
+             }, py::arg("intermediateTPS"), py::arg("rho_in"), py::arg("rho_accum"))
+        .def("getMesh", [](const TPS &t) {
+                // getMesh (TensorProductSimulator.hh:747-776): node positions and hexahedra in Gmsh node order
+                py::array_t<double> V({(py::ssize_t)t.nn, (py::ssize_t)3});
+                auto v = V.mutable_unchecked<2>();
+                for (size_t n = 0; n < t.nn; ++n) {
+                    auto pos = t.nodePos(n);
+                    for (int c = 0; c < 3; ++c) v(n, c) = pos[c];
+                }
+                py::array_t<int64_t> F({(py::ssize_t)t.nel, (py::ssize_t)8});
+                auto f = F.mutable_unchecked<2>();
+                for (size_t e = 0; e < t.nel; ++e)
+                    for (int pair = 0; pair < 4; ++pair) {
+                        const int64_t v1 = (int64_t)t.elemNode(e, 2 * pair), v2 = (int64_t)t.elemNode(e, 2 * pair + 1);
+                        f(e, 2 * pair) = pair % 2 == 0 ? v1 : v2;
+                        f(e, 2 * pair + 1) = pair % 2 == 0 ? v2 : v1;
+                    }
+                return py::make_tuple(V, F);
+            })
+        .def("mesh", [](TPSPtr t) { return t; }, "The simulator itself (as the reference, for mesh helpers)")
+        .def("clearCachedElementStiffness", [](TPS &) {
+                // no custom per-element stiffness is cached by this build: every element uses E_e K0
+            })
+        .def("debugMulticolorElementVisit", [](const TPS &t) {
+                // visitElementsMulticolored (TensorProductSimulator.hh:1444-1457) order, serial: colour offsets
+                // (x, y, z parities, x slowest), then the elements of a colour in C order
+                std::vector<double> r(t.nel, 0.0);
+                double i = 0;
+                for (int colour = 0; colour < 8; ++colour) {
+                    const int cx = (colour >> 2) & 1, cy = (colour >> 1) & 1, cz = colour & 1;
+                    for (int64_t x = cx; x < t.ne[0]; x += 2)
+                        for (int64_t y = cy; y < t.ne[1]; y += 2)
+                            for (int64_t z = cz; z < t.ne[2]; z += 2) r[(x * t.ne[1] + y) * t.ne[2] + z] = i++;
+                }
+                return from_vec(r);
+            })
         .def("getIntermediateFabricationShape", [](const TPS &t, double yfrac, bool validate, InterpolationLaw law) {
                 return intermediateShape(t, yfrac, validate, law);
             }, py::arg("yfrac"), py::arg("validateBoundaryConditions") = true, py::arg("law") = InterpolationLaw::SIMP)
@@ -1126,6 +1329,14 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def_property("useGraphs", [](const MG &mg) { return mg.useGraphs; },
                       [](MG &mg, bool v) { check(vf_mg_set_use_graphs(mg.mg, v ? 1 : 0)); mg.useGraphs = v; })
         .def("setFabricationMaskHeightByLayer", [](MG &mg, size_t h) { check(vf_mg_set_mask_layer(mg.mg, (int64_t)h)); }, py::arg("h"))
+        .def_property("partialStiffnessUpdates", [](const MG &mg) { return mg.partialUpdates; },
+                      [](MG &mg, bool on) { check(vf_mg_set_partial_updates(mg.mg, on ? 1 : 0)); mg.partialUpdates = on; },
+                      "Recompute only the band of coarse rows touched by removed layers (the reference's partial update)")
+        .def_property_readonly("partialUpdateCount", [](const MG &mg) {
+                int64_t c = 0;
+                check(vf_mg_partial_update_count(mg.mg, &c));
+                return c;
+            })
         .def("preconditionedConjugateGradient",
              [](MG &mg, const Arr &u, const Arr &b, size_t maxIter, double tol,
                 std::function<void(size_t, py::array_t<double>, py::array_t<double>)> cb, size_t mgIterations,
@@ -1187,6 +1398,18 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 return soa_to_vfield(out.data(), nn);
             }, py::arg("r"), py::arg("mgIterations") = 1, py::arg("nu") = 1, py::arg("fullMultigrid") = false,
             py::arg("precision") = "fp64")
+        .def("debug_get_x", [](MG &mg, int l) {
+                const size_t nn = mg.nodes(l);
+                std::vector<double> v(3 * nn);
+                check(vf_mg_debug_get(mg.mg, l, 0, v.data(), VF_HOST));
+                return soa_to_vfield(v.data(), nn);
+            }, py::arg("l"))
+        .def("debug_get_b", [](MG &mg, int l) {
+                const size_t nn = mg.nodes(l);
+                std::vector<double> v(3 * nn);
+                check(vf_mg_debug_get(mg.mg, l, 1, v.data(), VF_HOST));
+                return soa_to_vfield(v.data(), nn);
+            }, py::arg("l"))
         .def("debugMulticolorVisit", [](MG &mg) {
                 auto &t = *mg.tps;
                 const auto N = t.nnodes3();

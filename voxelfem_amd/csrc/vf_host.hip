@@ -11,6 +11,7 @@
 #include "../../include/voxelfem.h"
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <climits>
 #include <cmath>
@@ -160,7 +161,8 @@ struct vf_sim {
     double maskHeight = std::numeric_limits<double>::infinity();
     DBuf<double> s_in, s_out, s_aux, partials, scal;
     DBuf<float> s32a, s32b;
-    uint64_t version = 0;  // bumped whenever E changes
+    uint64_t version = 0;      // bumped whenever E or K0 changes
+    uint64_t baseVersion = 0;  // the same, except when layers are removed from the top of the mask
 
     size_t nn() const { return (size_t)g.nnodes; }
     size_t nel() const { return (size_t)g.nelems; }
@@ -241,6 +243,7 @@ struct vf_sim {
         refresh_grid();
         launch_young(g, rho.data(), E.data(), E32.data(), law, E0, Emin, gamma, q, maskHeight < hi[1] ? 1 : 0, stream);
         ++version;
+        ++baseVersion;
     }
     // setFabricationMaskHeight (TensorProductSimulator.hh:286-310)
     void set_mask_height(double H, bool updateYoung) {
@@ -297,6 +300,7 @@ struct vf_sim {
         c->firstDetached = firstDetached;
         c->maskHeight = maskHeight;
         c->version = version;
+        c->baseVersion = baseVersion;
         auto dup = [&](auto &dst, const auto &src) {
             dst.resize(src.size());
             d2d(dst.data(), src.data(), src.size(), stream);
@@ -432,10 +436,24 @@ struct vf_mg {
     int nred = 0, ldA = 0;
     bool stiffnessReady = false;
     uint64_t stiffnessVersion = 0;
+    // Partial Galerkin updates after layers were removed from the top of the fabrication mask
+    // (MultigridSolver.hh:907-1017): only the coarse element rows holding the newly masked fine layers are
+    // recomputed, only the block rows of the nodes they touch reassembled; the coarsest level is refactored.
+    // The recomputed rows come from the same kernels and inputs as in a full rebuild, so the operators are
+    // identical to a full rebuild's.
+    bool partialUpdates = true;
+    uint64_t stiffBase = 0;
+    int64_t stiffLayer = 0;    // first masked fine element layer at the last update
+    int64_t partialCount = 0;  // partial updates made
+    std::vector<std::array<int, 2>> dirtyRows[2];  // per precision and level: node rows [lo, hi] to reassemble
     // PCG work
     DBuf<double> pd, pAd, ps, pr, px, pb;
     // level-0 views used by the cycles (set per preconditioner call)
     double *x0v = nullptr, *b0v = nullptr;
+    // what debug_get_x/b (MultigridSolver.hh:1158-1159) read: the level buffers of the last cycle, in the
+    // precision it ran in (-1: no cycle yet); level 0 is the last cycle's fine iterate and right-hand side
+    int dbgPrec = -1;
+    const void *dbgX0 = nullptr, *dbgB0 = nullptr;
 
     hipStream_t st() const { return sim->stream; }
 
@@ -903,6 +921,9 @@ struct vf_mg {
                 if (cb(user, i, hx.data())) { b0v = x0v = nullptr; throw std::runtime_error("solve aborted by callback"); }
             }
         }
+        dbgPrec = VF_PREC_FP64;
+        dbgX0 = x;
+        dbgB0 = f;
         b0v = x0v = nullptr;
     }
 
@@ -932,6 +953,9 @@ struct vf_mg {
                 if (fullMG && i == 0) fmg<double>(0, nu);
                 else vcycle<double>(0, nu, true, i == 0);
             }
+            dbgPrec = VF_PREC_FP64;
+            dbgX0 = s;
+            dbgB0 = r;
         } else {
             ensure_buffers<float>(0);
             tune<float>();
@@ -941,14 +965,69 @@ struct vf_mg {
                 if (fullMG && i == 0) fmg<float>(0, nu);
                 else vcycle<float>(0, nu, true, i == 0);
             }
+            dbgPrec = VF_PREC_FP32;
+            dbgX0 = lv[0]->x32.data();
+            dbgB0 = lv[0]->b32.data();
             if (!resident) launch_convert<double, float>(n0, lv[0]->x32.data(), s, st());
         }
         b0v = x0v = nullptr;
     }
 
+    // the level-0 vectors of an API-level cycle live in caller or staging memory: keep a copy for debug_get
+    void dbg_keep_level0(size_t n0) {
+        if (dbgPrec != VF_PREC_FP64 || (dbgX0 == lv[0]->x.data() && dbgB0 == lv[0]->b.data())) return;
+        lv[0]->x.resize(n0);
+        lv[0]->b.resize(n0);
+        d2d(lv[0]->x.data(), (const double *)dbgX0, n0, st());
+        d2d(lv[0]->b.data(), (const double *)dbgB0, n0, st());
+        dbgX0 = lv[0]->x.data();
+        dbgB0 = lv[0]->b.data();
+    }
+
     // ---------------------------------------------------------------- Galerkin setup
     void ensure_stiffness() {
-        if (!stiffnessReady || stiffnessVersion != sim->version) update_stiffness();
+        if (stiffnessReady && stiffnessVersion == sim->version) return;
+        const int64_t layer = std::min<int64_t>(sim->firstMasked, sim->ne[1]);
+        if (partialUpdates && stiffnessReady && L >= 2 && sim->baseVersion == stiffBase && layer < stiffLayer)
+            partial_update(layer, stiffLayer);
+        else
+            update_stiffness();
+    }
+    // the fine element layers [m, c) were masked since the last update
+    void partial_update(int64_t m, int64_t c) {
+        sync_masks();
+        hipStream_t s = st();
+        prepare_coarsest();
+        // element rows of level l that hold fine layers of [m, c) (the coarsened bands, MultigridSolver.hh:913-922)
+        std::vector<std::array<int, 2>> band(L + 1);
+        int64_t b0 = m, b1 = c;
+        band[0] = {(int)b0, (int)b1};
+        for (int l = 1; l <= L; ++l) {
+            b0 /= 2;
+            b1 = (b1 + 1) / 2;
+            band[l] = {(int)b0, (int)b1};
+        }
+        launch_galerkin_l2_elems(lv[2]->g, lv[0]->g, dM2.data(), sim->E.data(), lv[2]->Ke.data(), s, band[2][0],
+                                 band[2][1]);
+        for (int l = 3; l <= L; ++l)
+            launch_galerkin_recurse(lv[l]->g, lv[l - 1]->g, dphi.data(), lv[l - 1]->Ke.data(), lv[l]->Ke.data(), s,
+                                    band[l][0], band[l][1]);
+        // the block rows of the nodes on those element rows are reassembled by the next cycle that uses them
+        for (int p = 0; p < 2; ++p) {
+            if (!blocksReady[p]) continue;
+            auto &d = dirtyRows[p];
+            if ((int)d.size() < L + 1) d.assign(L + 1, {1, 0});
+            for (int l = 1; l < L; ++l) {
+                const int lo = band[l][0], hi = band[l][1];
+                if (d[l][0] > d[l][1]) d[l] = {lo, hi};
+                else d[l] = {std::min(d[l][0], lo), std::max(d[l][1], hi)};
+            }
+        }
+        launch_factor();
+        stiffnessReady = true;
+        stiffnessVersion = sim->version;
+        stiffLayer = m;
+        ++partialCount;
     }
     // updateStiffnessMatrices (MultigridSolver.hh:846-905)
     void update_stiffness() {
@@ -987,9 +1066,13 @@ struct vf_mg {
             }
         }
         blocksReady[0] = blocksReady[1] = false;
+        dirtyRows[0].clear();
+        dirtyRows[1].clear();
         launch_factor();
         stiffnessReady = true;
         stiffnessVersion = sim->version;
+        stiffBase = sim->baseVersion;
+        stiffLayer = std::min<int64_t>(sim->firstMasked, sim->ne[1]);
     }
     template <typename T>
     DBuf<T> &block_buf(int l);
@@ -999,8 +1082,23 @@ struct vf_mg {
     void ensure_blocks() {
         ensure_stiffness();
         const int p = pidx<T>();
-        if (blocksReady[p]) return;
         hipStream_t s = st();
+        if (blocksReady[p]) {
+            // after partial updates: only the stale node rows
+            auto &d = dirtyRows[p];
+            for (int l = 1; l < (int)d.size() && l < L; ++l) {
+                if (d[l][0] > d[l][1]) continue;
+                if (l == 1) {
+                    if (l1Blocks && !l1_blocks_unused(p) && block_buf<T>(1).size())
+                        launch_assemble_l1_blocks<T>(lv[1]->g, lv[0]->g, dChat.data(), sim->E.data(), block_buf<T>(1).data(),
+                                                     sim->structured, s, d[l][0], d[l][1]);
+                } else {
+                    launch_assemble_blocks<T>(lv[l]->g, lv[l]->Ke.data(), block_buf<T>(l).data(), s, d[l][0], d[l][1]);
+                }
+            }
+            d.clear();
+            return;
+        }
         if (L >= 2 && l1Blocks && !l1_blocks_unused(p)) {
             LevelData &c1 = *lv[1];
             DBuf<T> &buf = block_buf<T>(1);
@@ -1014,6 +1112,7 @@ struct vf_mg {
             launch_assemble_blocks<T>(c.g, c.Ke.data(), buf.data(), s);
         }
         blocksReady[p] = true;
+        dirtyRows[p].clear();
     }
     // the main stream waits for the coarsest factorization launched by the last update; a non-positive
     // pivot is reported here
@@ -1556,6 +1655,7 @@ int vf_sim_set_material(vf_sim *sim, double young, double poisson) {
     sim->poisson = poisson;
     sim->update_K0();
     ++sim->version;
+    ++sim->baseVersion;
     VF_API_END
 }
 
@@ -1739,6 +1839,78 @@ int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int wh
     VF_API_END
 }
 
+int vf_sim_constant_strain_load(vf_sim *sim, const double eps[9], double *F, int where) {
+    VF_API_BEGIN
+    need(sim && eps && F, "null argument");
+    for (int a = 0; a < 3; ++a)
+        for (int b = a + 1; b < 3; ++b)
+            need(std::fabs(eps[3 * a + b] - eps[3 * b + a]) <= 1e-12 * (std::fabs(eps[3 * a + b]) + std::fabs(eps[3 * b + a]) + 1e-300),
+                 "constantStrainLoad: the strain must be a symmetric 3x3 matrix");
+    // isotropic elasticity tensor of the material (setETensor): sigma = lambda tr(eps) I + 2 mu eps
+    const double E = sim->young, nu = sim->poisson;
+    const double lam = E * nu / ((1 + nu) * (1 - 2 * nu)), mu = E / (2 * (1 + nu));
+    const double tr = eps[0] + eps[4] + eps[8];
+    const double V = sim->h[0] * sim->h[1] * sim->h[2];
+    StressCoef sc;
+    for (int c = 0; c < 3; ++c)
+        for (int d = 0; d < 3; ++d) {
+            const double sigma = (c == d ? lam * tr : 0.0) + 2 * mu * eps[3 * c + d];
+            sc.w[c][d] = sigma * V / (4.0 * sim->h[d]);
+        }
+    const size_t n = 3 * sim->nn();
+    Staged st{sim, where};
+    double *dF = st.out(F, sim->stage_out(n));
+    launch_constant_strain_load(sim->g, sim->rho.data(), sc, dF, sim->stream);
+    st.finish(F, dF, n);
+    VF_API_END
+}
+
+int vf_sim_sample_nodal_field(vf_sim *sim, const double *u, int64_t ncomp, const double *p, int64_t np, double *out,
+                              int where) {
+    VF_API_BEGIN
+    need(sim && u && p && out, "null argument");
+    need(ncomp >= 1 && np >= 0 && np < INT_MAX / 3, "invalid sizes");
+    if (np == 0) return VF_OK;
+    hipStream_t s = sim->stream;
+    const size_t nu = (size_t)ncomp * sim->nn(), npo = (size_t)ncomp * np;
+    DBuf<double> hu, hp, ho;
+    const double *du = u, *dp = p;
+    double *dout = out;
+    if (where == VF_HOST) {
+        hu.resize(nu); hp.resize(3 * (size_t)np); ho.resize(npo);
+        h2d(hu.data(), u, nu, s);
+        h2d(hp.data(), p, 3 * (size_t)np, s);
+        du = hu.data(); dp = hp.data(); dout = ho.data();
+    }
+    DBuf<int> bad(1);
+    launch_fill<int>(1, bad.data(), 0, s);
+    launch_sample_nodal(sim->g, sim->lo, sim->hi, sim->h, du, (int)ncomp, dp, (int)np, dout, bad.data(), s);
+    int flag = 0;
+    d2h(&flag, bad.data(), 1, s);
+    if (where == VF_HOST) d2h(out, dout, npo, s);
+    sync(s);
+    if (flag) throw std::runtime_error("Point out of bounds");
+    VF_API_END
+}
+
+int vf_sim_set_stretchings(vf_sim *sim, const double h[3]) {
+    VF_API_BEGIN
+    need(sim && h, "null argument");
+    for (int d = 0; d < 3; ++d) need(h[d] > 0, "stretchings must be positive");
+    std::copy(h, h + 3, sim->h);
+    sim->update_K0();
+    ++sim->version;
+    ++sim->baseVersion;
+    VF_API_END
+}
+
+int vf_sim_get_stretchings(vf_sim *sim, double h[3]) {
+    VF_API_BEGIN
+    need(sim && h, "null argument");
+    std::copy(sim->h, sim->h + 3, h);
+    VF_API_END
+}
+
 // ---------------------------------------------------------------------------------------------- MG
 int vf_mg_create(vf_mg **out, vf_sim *fine, int64_t levels) {
     VF_API_BEGIN
@@ -1916,6 +2088,47 @@ int vf_mg_zero_dirichlet(vf_mg *mg, int64_t l, double *u, int where) {
     VF_API_END
 }
 
+int vf_mg_set_partial_updates(vf_mg *mg, int enable) {
+    VF_API_BEGIN
+    need(mg, "null argument");
+    mg->partialUpdates = enable != 0;
+    VF_API_END
+}
+
+int vf_mg_partial_update_count(vf_mg *mg, int64_t *count) {
+    VF_API_BEGIN
+    need(mg && count, "null argument");
+    *count = mg->partialCount;
+    VF_API_END
+}
+
+int vf_mg_debug_get(vf_mg *mg, int64_t l, int which, double *out, int where) {
+    VF_API_BEGIN
+    need(mg && out, "null argument");
+    need(l >= 0 && l <= mg->L, "level out of range");
+    need(which == 0 || which == 1, "which must be 0 (x) or 1 (b)");
+    if (mg->dbgPrec < 0) throw std::runtime_error("no multigrid cycle has run yet");
+    const size_t n = level_n(mg, l);
+    hipStream_t s = mg->st();
+    double *dst = where == VF_DEVICE ? out : mg->sim->stage_out(n);
+    if (mg->dbgPrec == VF_PREC_FP64) {
+        const double *src = l == 0 ? (const double *)(which ? mg->dbgB0 : mg->dbgX0)
+                                   : (which ? mg->lv[l]->b.data() : mg->lv[l]->x.data());
+        need(src != nullptr && (l == 0 || (which ? mg->lv[l]->b.size() : mg->lv[l]->x.size()) >= n),
+             "the last cycle did not use this level's buffer");
+        d2d(dst, src, n, s);
+    } else {
+        const float *src = l == 0 ? (const float *)(which ? mg->dbgB0 : mg->dbgX0)
+                                  : (which ? mg->lv[l]->b32.data() : mg->lv[l]->x32.data());
+        need(src != nullptr && (l == 0 || (which ? mg->lv[l]->b32.size() : mg->lv[l]->x32.size()) >= n),
+             "the last cycle did not use this level's buffer");
+        launch_convert<double, float>(n, src, dst, s);
+    }
+    if (where == VF_HOST) d2h(out, dst, n, s);
+    sync(s);
+    VF_API_END
+}
+
 int vf_mg_solve(vf_mg *mg, double *x, const double *f, int64_t numSteps, int64_t smoothing, int stiffnessUpdated,
                 int zeroDirichlet, int fullMultigrid, vf_mg_callback cb, void *user, int where) {
     VF_API_BEGIN
@@ -1934,6 +2147,7 @@ int vf_mg_solve(vf_mg *mg, double *x, const double *f, int64_t numSteps, int64_t
     }
     mg->solve(dx, df, (int)numSteps, (int)smoothing, stiffnessUpdated != 0, zeroDirichlet != 0, fullMultigrid != 0, cb,
               user);
+    mg->dbg_keep_level0(n);
     if (where == VF_HOST) d2h(x, dx, n, s);
     sync(s);
     VF_API_END
@@ -1949,6 +2163,7 @@ int vf_mg_precondition(vf_mg *mg, const double *r, double *s, int64_t mgIteratio
     const double *dr = st.in(r, n, mg->sim->stage_in(n));
     double *ds = st.out(s, mg->sim->stage_out(n));
     mg->precondition(dr, ds, (int)mgIterations, (int)smoothing, fullMultigrid != 0, precision);
+    mg->dbg_keep_level0(n);
     st.finish(s, ds, n);
     VF_API_END
 }
@@ -2038,7 +2253,8 @@ static std::vector<double> sym_pinv_solve(std::vector<double> A, const std::vect
 static void lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
                     int64_t smoothing, int fullMultigrid, int precision, int method, int64_t history, bool zeroInit,
                     double *uFull, bool uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
-                    double *perLayerCompliance, int64_t *perLayerIterations) {
+                    double *perLayerCompliance, int64_t *perLayerIterations, vf_pcg_callback itcb = nullptr,
+                    void *ituser = nullptr, vf_lbl_callback lblcb = nullptr, void *lbluser = nullptr) {
     vf_sim *sim = mg->sim;
     hipStream_t s = mg->st();
     const int64_t numLayers = sim->ne[1];
@@ -2100,8 +2316,19 @@ static void lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
         }
         launch_zero_detached<double>(sim->g, u.data(), s);
         const int64_t its = mg->pcg(u.data(), f.data(), maxIter, tol, (int)mgIterations, (int)smoothing,
-                                    fullMultigrid != 0, precision, true, nullptr, nullptr, nullptr);
+                                    fullMultigrid != 0, precision, true, itcb, ituser, nullptr);
         const double comp = dot(f.data(), u.data());
+        if (lblcb) {
+            // lblCallback(l, compliance, complianceGradientFlattened(u), u) (LayerByLayer.hh:277-279)
+            DBuf<double> gl(ne);
+            launch_compliance_gradient(sim->g, sim->K0.data(), sim->rho.data(), u.data(), gl.data(), sim->law, sim->E0,
+                                       sim->Emin, sim->gamma, sim->q, sim->grav, sim->h[0] * sim->h[1] * sim->h[2], 0, s);
+            std::vector<double> hgl(ne), hu(n);
+            d2h(hgl.data(), gl.data(), ne, s);
+            d2h(hu.data(), u.data(), n, s);
+            sync(s);
+            if (lblcb(lbluser, l, comp, hgl.data(), hu.data())) throw std::runtime_error("layer-by-layer run aborted by callback");
+        }
         if (perLayerCompliance) perLayerCompliance[count] = comp;
         if (perLayerIterations) perLayerIterations[count] = its;
         total += comp;
@@ -2148,7 +2375,8 @@ int vf_lbl_run_zero(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
 int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations, int64_t smoothing,
                int fullMultigrid, int precision, int initMethod, int64_t history, int zeroInit, double *uFull,
                int uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
-               double *perLayerCompliance, int64_t *perLayerIterations) {
+               double *perLayerCompliance, int64_t *perLayerIterations, vf_pcg_callback itcb, void *ituser,
+               vf_lbl_callback lblcb, void *lbluser) {
     VF_API_BEGIN
     need(mg && objective && grad, "null argument");
     need(layerIncrement >= 1, "layerIncrement must be positive");
@@ -2158,7 +2386,7 @@ int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, i
     need(initMethod != VF_LBL_INIT_SUBSPACE || (history >= 1 && history <= 64), "subspace history must be 1..64");
     lbl_run(mg, layerIncrement, maxIter, tol, mgIterations, smoothing, fullMultigrid, precision, initMethod, history,
             zeroInit != 0, uFull, uFullValid != 0, uFullWhere, objective, grad, gradWhere, perLayerCompliance,
-            perLayerIterations);
+            perLayerIterations, itcb, ituser, lblcb, lbluser);
     VF_API_END
 }
 

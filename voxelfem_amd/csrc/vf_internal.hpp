@@ -219,6 +219,13 @@ void launch_dot_active(const Grid &g, const double *a, const double *b, double *
 void launch_dot2_active(const Grid &g, const double *a, const double *b, const double *c, double *partials,
                         double *out2, hipStream_t s);  // out2[0] = a.b, out2[1] = c.b
 void launch_dot_full(size_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s);
+// constant-stress element load coefficients: w[c][d] = sigma[c][d] * V / (4 h_d)
+struct StressCoef {
+    double w[3][3];
+};
+void launch_constant_strain_load(const Grid &g, const double *rho, const StressCoef &sc, double *f, hipStream_t s);
+void launch_sample_nodal(const Grid &g, const double lo[3], const double hi[3], const double h[3], const double *u,
+                         int ncomp, const double *p, int np, double *out, int *bad, hipStream_t s);
 int reduction_partials(const Grid &g);
 int reduction_partials_full(size_t n);
 
@@ -244,18 +251,19 @@ void launch_pcg_scalars(double *sc, int op, hipStream_t st);
 // setup-time Galerkin coarsening and assembly (MultigridSolver.hh:704-905)
 void launch_galerkin_l1_elems(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, double *Ke,
                               hipStream_t s);
+// element matrices of the element rows [ylo, yhi) (default: all rows)
 void launch_galerkin_l2_elems(const Grid &g2, const Grid &gf, const double *M2, const double *Ef, double *Ke,
-                              hipStream_t s);
+                              hipStream_t s, int ylo = 0, int yhi = -1);
 void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, const double *Kf, double *Kc,
-                             hipStream_t s);
-// block rows (FP64 sums stored as OutT = double or float)
+                             hipStream_t s, int ylo = 0, int yhi = -1);
+// block rows (FP64 sums stored as OutT = double or float) of the node rows [nylo, nyhi] (default: all)
 template <typename OutT>
-void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s);
+void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s, int nylo = 0, int nyhi = -1);
 // level-1 block rows assembled from the fine moduli and the eight coarsened element matrices
 // (reflect: the element matrix is reflection-symmetric, so child 0's matrix gives all eight)
 template <typename OutT>
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
-                               bool reflect, hipStream_t s);
+                               bool reflect, hipStream_t s, int nylo = 0, int nyhi = -1);
 // coarsest level: band Cholesky of the reduced operator (half-bandwidth kb blocks of 64) and the explicit
 // X = L^-1 (dense lower triangular, leading dimension n rounded up to 64) and its transpose XT (filled on
 // and above the diagonal); flag != 0 if not positive definite

@@ -4,6 +4,8 @@
 // factorisation lives in vf_kernels_chol.hip.
 #include "vf_internal.hpp"
 
+#include <algorithm>
+
 namespace vf {
 namespace {
 
@@ -32,16 +34,24 @@ __global__ void __launch_bounds__(576) k_galerkin_l1(Grid gc, Grid gf, const dou
 // One workgroup of 576 work-items (one per matrix entry) handles EPB elements, so each M2 entry
 // fetched from L2 is reused EPB times.
 constexpr int EPB = 16;
+// Only the element rows ey in [ylo, yhi) are computed (all rows: [0, ne[1])); the workgroups enumerate
+// that band's elements in C order.
 __global__ void __launch_bounds__(576) k_galerkin_l2(Grid g2, Grid gf, const double *__restrict__ M2,
-                                                     const double *__restrict__ Ef, double *__restrict__ Ke) {
+                                                     const double *__restrict__ Ef, double *__restrict__ Ke, int ylo,
+                                                     int yhi) {
     __shared__ double sE[EPB][64];
     const int e0 = blockIdx.x * EPB;
     const int t = threadIdx.x;
+    const int rows = yhi - ylo, nband = g2.ne[0] * rows * g2.ne[2];
+    auto band_elem = [&](int b) {
+        const int ez = b % g2.ne[2], r = (b / g2.ne[2]) % rows, ex = b / (rows * g2.ne[2]);
+        return (ex * g2.ne[1] + ylo + r) * g2.ne[2] + ez;
+    };
     for (int w = t; w < EPB * 64; w += 576) {
         const int j = w / 64, k = w % 64;
-        const int e = e0 + j;
         double v = 0;
-        if (e < g2.nelems) {
+        if (e0 + j < nband) {
+            const int e = band_elem(e0 + j);
             const int ez = e % g2.ne[2], ey = (e / g2.ne[2]) % g2.ne[1], ex = e / (g2.ne[1] * g2.ne[2]);
             if (ey < g2.ymask) {
                 const int f2 = k >> 3, f1 = k & 7;
@@ -64,7 +74,7 @@ __global__ void __launch_bounds__(576) k_galerkin_l2(Grid g2, Grid gf, const dou
     }
 #pragma unroll
     for (int j = 0; j < EPB; ++j)
-        if (e0 + j < g2.nelems) Ke[(size_t)(e0 + j) * 576 + t] = acc[j];
+        if (e0 + j < nband) Ke[(size_t)band_elem(e0 + j) * 576 + t] = acc[j];
 }
 
 // ---- one more level of Galerkin coarsening: Kc(e) = sum_fi P_fi^T Kf(child fi) P_fi
@@ -72,14 +82,18 @@ __global__ void __launch_bounds__(576) k_galerkin_l2(Grid g2, Grid gf, const dou
 // The eight child matrices are fetched together (eight independent loads per work-item) and staged in LDS,
 // so the workgroup waits on global memory once; T = Kf P goes through a double-buffered LDS tile, one
 // barrier per child.
+// One workgroup per element of the element rows [ylo, yhi).
 __global__ void __launch_bounds__(576) k_galerkin_recurse(Grid gc, Grid gf, const double *__restrict__ phi,
-                                                          const double *__restrict__ Kf, double *__restrict__ Kc) {
+                                                          const double *__restrict__ Kf, double *__restrict__ Kc,
+                                                          int ylo, int yhi) {
     __shared__ double sK[8][576];
     __shared__ double sT[2][576];
     __shared__ double sphi[512];
-    const int e = blockIdx.x;
+    const int rows = yhi - ylo;
+    const int b = blockIdx.x;
     const int t = threadIdx.x;
-    const int ez = e % gc.ne[2], ey = (e / gc.ne[2]) % gc.ne[1], ex = e / (gc.ne[1] * gc.ne[2]);
+    const int ez = b % gc.ne[2], ey = ylo + (b / gc.ne[2]) % rows, ex = b / (rows * gc.ne[2]);
+    const int e = (ex * gc.ne[1] + ey) * gc.ne[2] + ez;
     if (ey >= gc.ymask) {  // uniform over the workgroup
         Kc[(size_t)e * 576 + t] = 0;
         return;
@@ -145,11 +159,13 @@ __device__ inline bool class_node(const Grid &g, int colour, ClassNode &r) {
 }
 
 // group sums of an intermediate level from its element matrices Ke (576 FP64 per element, row-major 24x24)
+// Only the nodes of rows iy in [nylo, nyhi] are assembled.
 template <typename OutT>
 __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *__restrict__ Ke,
-                                                         OutT *__restrict__ blocks) {
+                                                         OutT *__restrict__ blocks, int nylo, int nyhi) {
     ClassNode n;
     if (!class_node(g, blockIdx.y, n)) return;
+    if (n.iy < nylo || n.iy > nyhi) return;
     OutT *out = blocks + n.before * 243 + n.j;
     for (int gx = 0; gx < 3; ++gx) {
         const int ox = gx - 1;
@@ -209,9 +225,10 @@ __global__ void __launch_bounds__(TPB) k_assemble_blocks(Grid g, const double *_
 template <typename OutT, bool REFL>
 __global__ void __launch_bounds__(TPB) k_assemble_l1_blocks(Grid gc, Grid gf, const double *__restrict__ Chat,
                                                             const double *__restrict__ Ef,
-                                                            OutT *__restrict__ blocks) {
+                                                            OutT *__restrict__ blocks, int nylo, int nyhi) {
     ClassNode n;
     if (!class_node(gc, blockIdx.y, n)) return;
+    if (n.iy < nylo || n.iy > nyhi) return;
     const int eyEnd = min(gc.ne[1], gc.ymask);
     const int fz = gf.ne[2], fyz = gf.ne[1] * gf.ne[2];
     OutT *out = blocks + n.before * 243 + n.j;
@@ -308,42 +325,58 @@ void launch_galerkin_l1_elems(const Grid &gc, const Grid &gf, const double *Chat
 }
 
 void launch_galerkin_l2_elems(const Grid &g2, const Grid &gf, const double *M2, const double *Ef, double *Ke,
-                              hipStream_t s) {
-    hipLaunchKernelGGL(k_galerkin_l2, dim3((g2.nelems + EPB - 1) / EPB), dim3(576), 0, s, g2, gf, M2, Ef, Ke);
+                              hipStream_t s, int ylo, int yhi) {
+    if (yhi < 0) yhi = g2.ne[1];
+    ylo = std::max(ylo, 0);
+    yhi = std::min(yhi, g2.ne[1]);
+    if (yhi <= ylo) return;
+    const long nband = (long)g2.ne[0] * (yhi - ylo) * g2.ne[2];
+    hipLaunchKernelGGL(k_galerkin_l2, dim3((unsigned)((nband + EPB - 1) / EPB)), dim3(576), 0, s, g2, gf, M2, Ef, Ke,
+                       ylo, yhi);
     VF_HIP(hipGetLastError());
 }
 
 void launch_galerkin_recurse(const Grid &gc, const Grid &gf, const double *phi, const double *Kf, double *Kc,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(k_galerkin_recurse, dim3(gc.nelems), dim3(576), 0, s, gc, gf, phi, Kf, Kc);
+                             hipStream_t s, int ylo, int yhi) {
+    if (yhi < 0) yhi = gc.ne[1];
+    ylo = std::max(ylo, 0);
+    yhi = std::min(yhi, gc.ne[1]);
+    if (yhi <= ylo) return;
+    const long nband = (long)gc.ne[0] * (yhi - ylo) * gc.ne[2];
+    hipLaunchKernelGGL(k_galerkin_recurse, dim3((unsigned)nband), dim3(576), 0, s, gc, gf, phi, Kf, Kc, ylo, yhi);
     VF_HIP(hipGetLastError());
 }
 
 template <typename OutT>
-void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s) {
+void launch_assemble_blocks(const Grid &g, const double *Ke, OutT *blocks, hipStream_t s, int nylo, int nyhi) {
+    if (nyhi < 0) nyhi = g.nn[1] - 1;
+    if (nyhi < nylo) return;
     const long maxClass = (long)((g.nn[0] + 1) / 2) * ((g.nn[1] + 1) / 2) * ((g.nn[2] + 1) / 2);
-    hipLaunchKernelGGL(k_assemble_blocks<OutT>, dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, g, Ke, blocks);
+    hipLaunchKernelGGL(k_assemble_blocks<OutT>, dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, g, Ke, blocks, nylo,
+                       nyhi);
     VF_HIP(hipGetLastError());
 }
 
 template <typename OutT>
 void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Chat, const double *Ef, OutT *blocks,
-                               bool reflect, hipStream_t s) {
+                               bool reflect, hipStream_t s, int nylo, int nyhi) {
+    if (nyhi < 0) nyhi = gc.nn[1] - 1;
+    if (nyhi < nylo) return;
     const long maxClass = (long)((gc.nn[0] + 1) / 2) * ((gc.nn[1] + 1) / 2) * ((gc.nn[2] + 1) / 2);
     if (reflect)
         hipLaunchKernelGGL((k_assemble_l1_blocks<OutT, true>), dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, gc, gf,
-                           Chat, Ef, blocks);
+                           Chat, Ef, blocks, nylo, nyhi);
     else
         hipLaunchKernelGGL((k_assemble_l1_blocks<OutT, false>), dim3(nblocks(maxClass), 8), dim3(TPB), 0, s, gc, gf,
-                           Chat, Ef, blocks);
+                           Chat, Ef, blocks, nylo, nyhi);
     VF_HIP(hipGetLastError());
 }
 
-template void launch_assemble_blocks<double>(const Grid &, const double *, double *, hipStream_t);
-template void launch_assemble_blocks<float>(const Grid &, const double *, float *, hipStream_t);
+template void launch_assemble_blocks<double>(const Grid &, const double *, double *, hipStream_t, int, int);
+template void launch_assemble_blocks<float>(const Grid &, const double *, float *, hipStream_t, int, int);
 template void launch_assemble_l1_blocks<double>(const Grid &, const Grid &, const double *, const double *, double *,
-                                                bool, hipStream_t);
+                                                bool, hipStream_t, int, int);
 template void launch_assemble_l1_blocks<float>(const Grid &, const Grid &, const double *, const double *, float *,
-                                               bool, hipStream_t);
+                                               bool, hipStream_t, int, int);
 
 }  // namespace vf

@@ -658,6 +658,86 @@ void launch_self_weight(const Grid &g, const double *rho, const double *grav, do
 }
 
 namespace {
+// ---- constantStrainLoad (TensorProductSimulator.hh:1119-1145 with the element load of 84-113): element e
+// contributes rho_e * V * int_ref eps(phi_l e_c) : sigma = rho_e * sum_d sigma[c][d] s_d(l) V / (4 h_d) to
+// component c of its local node l (s_d(l) = +-1 the side of l along axis d; the derivative of the Q1 shape
+// function integrates to s_d / (4 h_d) times the cell volume). Nodes sum their elements in increasing
+// element index order (local node 7 of the lowest element first), as the reference's serial accumulation.
+__global__ void __launch_bounds__(TPB) k_constant_strain_load(Grid g, const double *__restrict__ rho, StressCoef sc,
+                                                              double *__restrict__ f) {
+    const int n = blockIdx.x * TPB + threadIdx.x;
+    if (n >= g.nnodes) return;
+    const int nz = g.nn[2], ix = n / (g.nn[1] * nz), iy = (n / nz) % g.nn[1], iz = n % nz;
+    double a0 = 0, a1 = 0, a2 = 0;
+    for (int l = 7; l >= 0; --l) {
+        const int i = (l >> 2) & 1, j = (l >> 1) & 1, k = l & 1;
+        const int ex = ix - i, ey = iy - j, ez = iz - k;
+        if ((unsigned)ex >= (unsigned)g.ne[0] || (unsigned)ey >= (unsigned)g.ne[1] || (unsigned)ez >= (unsigned)g.ne[2])
+            continue;
+        const double r = rho[(ex * g.ne[1] + ey) * g.ne[2] + ez];
+        const double sx = i ? 1.0 : -1.0, sy = j ? 1.0 : -1.0, sz = k ? 1.0 : -1.0;
+        a0 += r * (sc.w[0][0] * sx + sc.w[0][1] * sy + sc.w[0][2] * sz);
+        a1 += r * (sc.w[1][0] * sx + sc.w[1][1] * sy + sc.w[1][2] * sz);
+        a2 += r * (sc.w[2][0] * sx + sc.w[2][1] * sy + sc.w[2][2] * sz);
+    }
+    f[n] = a0;
+    f[g.nnodes + n] = a1;
+    f[2 * g.nnodes + n] = a2;
+}
+
+// ---- sampleNodalField (TensorProductSimulator.hh:1161-1179): the query point is clamped to the domain
+// and located in its element (getElementNDIndex, 1731-1752: a coordinate within 1e-10 of the far face
+// maps to the last element); the element's trilinear interpolant is then evaluated, as in the reference,
+// at the clamped point's coordinates q themselves (Element::interpolate receives q, not the reference
+// coordinates of q in the element): value = sum_l u[node(e, l)] * prod_d (bit_d(l) ? q_d : 1 - q_d).
+__global__ void __launch_bounds__(TPB) k_sample_nodal(Grid g, double lo0, double lo1, double lo2, double hi0,
+                                                      double hi1, double hi2, double h0, double h1, double h2,
+                                                      const double *__restrict__ u, int ncomp,
+                                                      const double *__restrict__ p, int np, double *__restrict__ out,
+                                                      int *__restrict__ bad) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= np) return;
+    const double lo[3] = {lo0, lo1, lo2}, hi[3] = {hi0, hi1, hi2}, h[3] = {h0, h1, h2};
+    double q[3];
+    int e[3];
+    for (int d = 0; d < 3; ++d) {
+        q[d] = fmin(fmax(p[3 * i + d], lo[d]), hi[d]);
+        const double fi = (q[d] - lo[d]) / h[d];
+        if (fabs(fi - g.ne[d]) < 1e-10) e[d] = g.ne[d] - 1;
+        else e[d] = (int)fi;
+        if (e[d] < 0 || e[d] >= g.ne[d]) { atomicOr(bad, 1); e[d] = 0; }
+    }
+    double w[8];
+    for (int l = 0; l < 8; ++l) {
+        const int a = (l >> 2) & 1, b = (l >> 1) & 1, c = l & 1;
+        w[l] = ((c ? q[2] : 1.0 - q[2]) * (b ? q[1] : 1.0 - q[1])) * (a ? q[0] : 1.0 - q[0]);
+    }
+    for (int k = 0; k < ncomp; ++k) {
+        double v = 0;
+        for (int l = 0; l < 8; ++l) {
+            const int nx = e[0] + ((l >> 2) & 1), ny = e[1] + ((l >> 1) & 1), nzz = e[2] + (l & 1);
+            const double un = u[(size_t)k * g.nnodes + (nx * g.nn[1] + ny) * g.nn[2] + nzz];
+            v = l == 0 ? un * w[0] : v + un * w[l];
+        }
+        out[(size_t)k * np + i] = v;
+    }
+}
+}  // namespace
+
+void launch_constant_strain_load(const Grid &g, const double *rho, const StressCoef &sc, double *f, hipStream_t s) {
+    hipLaunchKernelGGL(k_constant_strain_load, dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g, rho, sc, f);
+    VF_HIP(hipGetLastError());
+}
+
+void launch_sample_nodal(const Grid &g, const double lo[3], const double hi[3], const double h[3], const double *u,
+                         int ncomp, const double *p, int np, double *out, int *bad, hipStream_t s) {
+    if (np <= 0) return;
+    hipLaunchKernelGGL(k_sample_nodal, dim3(nblocks(np)), dim3(TPB), 0, s, g, lo[0], lo[1], lo[2], hi[0], hi[1], hi[2],
+                       h[0], h[1], h[2], u, ncomp, p, np, out, bad);
+    VF_HIP(hipGetLastError());
+}
+
+namespace {
 __global__ void __launch_bounds__(TPB) k_axpy_plain(size_t n, double a, const double *__restrict__ x,
                                                     double *__restrict__ y) {
     const size_t i = (size_t)blockIdx.x * TPB + threadIdx.x;
