@@ -251,6 +251,14 @@ int vf_oc_update(int64_t n, const double *x0, const double *dJ, const double *dc
                  double p, double *out, int where, void *stream);
 /* mean of n values (TotalVolumeConstraint::evaluate, TopologyOptimizationConstraint.hh:30-32) */
 int vf_mean(int64_t n, const double *x, int where, void *stream, double *out);
+/* Allocation-free variant of vf_mean. No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). x must be device memory; out is a host double (outWhere = VF_HOST: the call synchronises
+ * `stream`) or a device double (VF_DEVICE: the result is stream-ordered and nothing is synchronised).
+ * The partial sums live in a fixed device array of the library (256 KiB per device, held until the
+ * process ends), so n is limited to 67,108,864 values; larger n fails with VF_ERR_ARG. Every call on a
+ * device shares that array: calls that overlap on one device (other streams, other threads) race. */
+int vf_mean_noalloc(int64_t n, const double *x, void *stream, double *out, int outWhere);
 
 /* ---- measurement helpers (no reference counterpart: used by bench.py and the tests) ---------- */
 /* Select the fine-level kernels process-wide: 0 (default) = per solver and precision, the fastest

@@ -40,6 +40,8 @@ def lib():
         L.vf_projection_backprop.argtypes = [dbl, i64, vp, vp, vp, i32, vp]
         L.vf_mg_update_stiffness.argtypes = [vp]
         L.vf_mg_finish_update.argtypes = [vp]
+        L.vf_mean.argtypes = [i64, vp, i32, vp, ctypes.POINTER(dbl)]
+        L.vf_mean_noalloc.argtypes = [i64, vp, vp, vp, i32]
         _lib = L
     return _lib
 

@@ -20,6 +20,7 @@
 #include <exception>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <thread>
 
 using namespace vf;
@@ -1525,6 +1526,37 @@ static void with_buffers(size_t nin, size_t nout, const double *in, double *out,
     sync(s);
 }
 
+// ---- allocation-free mean: its scratch lives in a file-scope device array -----------------------
+// vf_mean_noalloc keeps its partial sums and its sum in `scratch` (one copy per device, held until the
+// process ends) instead of allocating them, so it can run where hipMalloc cannot (graph capture, serving
+// loops). Layout: NOALLOC_PARTIALS partial sums (one per TPB*ITEMS = 2048 inputs), then the sum at
+// NOALLOC_PARTIALS. The array's name is a host-side stand-in; host code uses the address that
+// hipGetSymbolAddress returns for the current device.
+namespace {
+constexpr int NOALLOC_PARTIALS = 32768;
+constexpr int NOALLOC_SLOTS = NOALLOC_PARTIALS + 32;
+constexpr int64_t NOALLOC_MAX_N = (int64_t)NOALLOC_PARTIALS * 2048;
+}  // namespace
+static __device__ __align__(256) double scratch[NOALLOC_SLOTS];
+namespace {
+std::mutex g_scratch_mu;
+std::vector<double *> g_scratch_ptr;  // device address of `scratch`, per device id
+
+double *scratch_on_current_device() {
+    int dev = 0;
+    VF_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        VF_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        need(p != nullptr, "hipGetSymbolAddress returned no address for the scratch array");
+        g_scratch_ptr[dev] = (double *)p;
+    }
+    return g_scratch_ptr[dev];
+}
+}  // namespace
+
 extern "C" {
 
 const char *vf_last_error(void) { return g_err.c_str(); }
@@ -2579,6 +2611,28 @@ int vf_mean(int64_t n, const double *x, int where, void *stream, double *out) {
     d2h(&v, res.data(), 1, s);
     sync(s);
     *out = v / (double)n;
+    VF_API_END
+}
+
+int vf_mean_noalloc(int64_t n, const double *x, void *stream, double *out, int outWhere) {
+    VF_API_BEGIN
+    need(x && out && n > 0, "invalid argument");
+    if (n > NOALLOC_MAX_N)
+        throw std::invalid_argument("vf_mean_noalloc: n = " + std::to_string(n) + " exceeds the maximum of " +
+                                    std::to_string(NOALLOC_MAX_N) + " values its fixed scratch array covers");
+    need(reduction_partials_full((size_t)n) <= NOALLOC_PARTIALS, "vf_mean_noalloc: partial-sum count out of range");
+    hipStream_t s = as_stream(stream);
+    double *sc = scratch_on_current_device();
+    // every partial the final sum reads is written by the first pass: nothing needs clearing
+    launch_dot_full((size_t)n, x, nullptr, sc, sc + NOALLOC_PARTIALS, s);
+    if (outWhere == VF_DEVICE) {
+        launch_div_scalar(sc + NOALLOC_PARTIALS, (double)n, out, s);
+    } else {
+        double v;
+        d2h(&v, sc + NOALLOC_PARTIALS, 1, s);
+        sync(s);
+        *out = v / (double)n;
+    }
     VF_API_END
 }
 

@@ -219,6 +219,7 @@ void launch_dot_active(const Grid &g, const double *a, const double *b, double *
 void launch_dot2_active(const Grid &g, const double *a, const double *b, const double *c, double *partials,
                         double *out2, hipStream_t s);  // out2[0] = a.b, out2[1] = c.b
 void launch_dot_full(size_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s);
+void launch_div_scalar(const double *in, double d, double *out, hipStream_t s);
 // constant-stress element load coefficients: w[c][d] = sigma[c][d] * V / (4 h_d)
 struct StressCoef {
     double w[3][3];

@@ -539,6 +539,15 @@ void launch_final_sum(const double *partials, int count, double *out, hipStream_
     VF_HIP(hipGetLastError());
 }
 
+// out[0] = in[0] / d (IEEE division on the device, the same rounding as the host's).
+__global__ void k_div_scalar(const double *__restrict__ in, double d, double *__restrict__ out) {
+    if (threadIdx.x == 0) out[0] = in[0] / d;
+}
+void launch_div_scalar(const double *in, double d, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_div_scalar, dim3(1), dim3(64), 0, s, in, d, out);
+    VF_HIP(hipGetLastError());
+}
+
 void launch_dot_full(size_t n, const double *a, const double *b, double *partials, double *out, hipStream_t s) {
     if (!n) { launch_fill<double>(1, out, 0.0, s); return; }
     const int cnt = reduction_partials_full(n);
