@@ -150,6 +150,11 @@ int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer);
  * hierarchy; the coarsest level is refactored. enable = 0 rebuilds the whole hierarchy every time. The count is
  * the number of partial updates made so far. */
 int vf_mg_set_partial_updates(vf_mg *mg, int enable);
+/* Queue the Galerkin chain of levels >= 2 (and the block rows assembled from it) on a side stream of default
+ * priority, so the first cycle's fine-level and level-1 work, which needs only the moduli, overlaps it
+ * (enable = 1, the default); 0 queues it on the solver's stream. No reference counterpart (the reference's
+ * updateStiffnessMatrices, MultigridSolver.hh:846-905, is synchronous). */
+int vf_mg_set_overlap_setup(vf_mg *mg, int enable);
 int vf_mg_partial_update_count(vf_mg *mg, int64_t *count);
 /* applyK(l, u) (MultigridSolver.hh:471-504) */
 int vf_mg_apply_K(vf_mg *mg, int64_t l, const double *u, double *y, int where);

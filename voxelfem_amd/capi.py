@@ -42,6 +42,7 @@ def lib():
         L.vf_mg_finish_update.argtypes = [vp]
         L.vf_mean.argtypes = [i64, vp, i32, vp, ctypes.POINTER(dbl)]
         L.vf_mean_noalloc.argtypes = [i64, vp, vp, vp, i32]
+        L.vf_mg_set_overlap_setup.argtypes = [vp, i32]
         _lib = L
     return _lib
 

@@ -257,6 +257,17 @@ struct vf_sim {
         refresh_grid();
         if (updateYoung) update_E();
     }
+    // decrementFabricationMaskHeightByLayer (TensorProductSimulator.hh:311-324): mask `count` more element layers
+    // from the top, zeroing their moduli in place (the moduli update_E would give at the lower height). Only
+    // `version` changes, so a hierarchy built before can be updated by bands.
+    void remove_top_layers(int64_t count) {
+        maskHeight -= (double)count * h[1];
+        firstMasked -= count;
+        firstDetached = firstMasked + 1;
+        refresh_grid();
+        launch_zero_young_layers(g, (int)firstMasked, (int)(firstMasked + count), E.data(), E32.data(), stream);
+        ++version;
+    }
     double *stage_in(size_t n) {
         s_in.resize(std::max(s_in.size(), n));
         return s_in.data();
@@ -414,6 +425,25 @@ struct vf_mg {
     hipStream_t fstream = nullptr;
     hipEvent_t evGalerkin = nullptr, evFactor = nullptr;
     bool factorPending = false;
+    // The Galerkin chain of levels >= 2 and the block rows assembled from it run on a third stream (default
+    // priority), after the moduli (evModuli): the first cycle's fine-level and level-1 work, which needs only
+    // the moduli, overlaps it. The solver's stream waits for evHier before its first use of level >= 2 data
+    // (wait_hier); hierPending marks that wait as due.
+    hipStream_t gstream = nullptr;
+    hipEvent_t evModuli = nullptr, evHier = nullptr;
+    bool hierPending = false;
+    bool overlapSetup = true;  // queue the Galerkin chain on gstream (false: on the solver's stream)
+    void ensure_gstream() {
+        if (gstream) return;
+        VF_HIP(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
+        VF_HIP(hipEventCreateWithFlags(&evModuli, hipEventDisableTiming));
+        VF_HIP(hipEventCreateWithFlags(&evHier, hipEventDisableTiming));
+    }
+    void wait_hier() {
+        if (!hierPending) return;
+        VF_HIP(hipStreamWaitEvent(st(), evHier, 0));
+        hierPending = false;
+    }
     // With useGraphs the factorization's ~90 small launches are replayed from a captured HIP graph: issued
     // one by one they keep the host busy for milliseconds while the main stream has nothing queued. The
     // graph is captured again when its operands (buffers, sizes, coarsest grid) change.
@@ -480,6 +510,12 @@ struct vf_mg {
             (void)hipStreamSynchronize(fstream);
             (void)hipStreamDestroy(fstream);
         }
+        if (gstream) {
+            (void)hipStreamSynchronize(gstream);
+            (void)hipStreamDestroy(gstream);
+        }
+        if (evModuli) (void)hipEventDestroy(evModuli);
+        if (evHier) (void)hipEventDestroy(evHier);
         if (evGalerkin) (void)hipEventDestroy(evGalerkin);
         if (evFactor) (void)hipEventDestroy(evFactor);
         for (hipEvent_t e : kdEv)
@@ -662,9 +698,10 @@ struct vf_mg {
     // best of two runs after a warm-up) and keep the fastest per operation and precision.
     template <typename T>
     void tune() {
-        ensure_blocks<T>();
+        ensure_blocks<T>(true);
         const int p = pidx<T>();
         if (g_kernel_variant != 0 || !sim->structured || (kuChoice[p] >= 0 && gsChoice[p] >= 0)) return;
+        wait_hier();
         const Grid &g = lv[0]->g;
         const size_t n = 3 * (size_t)g.nnodes;
         hipStream_t s = st();
@@ -860,6 +897,7 @@ struct vf_mg {
     // xZero: the caller left x at zero (the restriction zeroes the coarse iterate), so its Dirichlet
     // components need no clearing
     void vcycle(int l, int nu, bool residual = true, bool xZero = false) {
+        if (l >= 2 || (l == 1 && l1Blocks && !l1_blocks_unused(pidx<T>()))) wait_hier();
         if (l == L) { coarse_solve<T>(B<T>(l), X<T>(l)); return; }
         T *x = X<T>(l);
         const T *b = B<T>(l);
@@ -886,7 +924,7 @@ struct vf_mg {
     }
     template <typename T>
     void fmg(int l, int nu, bool residual = true) {
-        if (l == L) { coarse_solve<T>(B<T>(l), X<T>(l)); return; }
+        if (l == L) { wait_hier(); coarse_solve<T>(B<T>(l), X<T>(l)); return; }
         launch_restrict<T>(lv[l]->g, lv[l + 1]->g, B<T>(l), B<T>(l + 1), st());
         fmg<T>(l + 1, nu, residual);
         launch_interpolate<T>(lv[l]->g, lv[l + 1]->g, X<T>(l + 1), X<T>(l), 0, st());
@@ -998,6 +1036,7 @@ struct vf_mg {
     void partial_update(int64_t m, int64_t c) {
         sync_masks();
         hipStream_t s = st();
+        wait_hier();
         prepare_coarsest();
         // element rows of level l that hold fine layers of [m, c) (the coarsened bands, MultigridSolver.hh:913-922)
         std::vector<std::array<int, 2>> band(L + 1);
@@ -1024,7 +1063,7 @@ struct vf_mg {
                 else d[l] = {std::min(d[l][0], lo), std::max(d[l][1], hi)};
             }
         }
-        launch_factor();
+        launch_factor(s);
         stiffnessReady = true;
         stiffnessVersion = sim->version;
         stiffLayer = m;
@@ -1057,19 +1096,30 @@ struct vf_mg {
                 lv[1]->blocks.release();
                 lv[1]->blocks32.release();
             }
+            // the buffers are (re)allocated before the chain is queued on the side stream
+            for (int l = 2; l <= L; ++l) lv[l]->Ke.resize((size_t)lv[l]->g.nelems * 576);
+            hipStream_t hs = s;
+            if (overlapSetup) {
+                ensure_gstream();
+                VF_HIP(hipEventRecord(evModuli, s));
+                VF_HIP(hipStreamWaitEvent(gstream, evModuli, 0));
+                hs = gstream;
+            }
             LevelData &c2 = *lv[2];
-            c2.Ke.resize((size_t)c2.g.nelems * 576);
-            launch_galerkin_l2_elems(c2.g, lv[0]->g, dM2.data(), sim->E.data(), c2.Ke.data(), s);
+            launch_galerkin_l2_elems(c2.g, lv[0]->g, dM2.data(), sim->E.data(), c2.Ke.data(), hs);
             for (int l = 3; l <= L; ++l) {
                 LevelData &c = *lv[l];
-                c.Ke.resize((size_t)c.g.nelems * 576);
-                launch_galerkin_recurse(c.g, lv[l - 1]->g, dphi.data(), lv[l - 1]->Ke.data(), c.Ke.data(), s);
+                launch_galerkin_recurse(c.g, lv[l - 1]->g, dphi.data(), lv[l - 1]->Ke.data(), c.Ke.data(), hs);
+            }
+            if (overlapSetup) {
+                VF_HIP(hipEventRecord(evHier, gstream));
+                hierPending = true;
             }
         }
         blocksReady[0] = blocksReady[1] = false;
         dirtyRows[0].clear();
         dirtyRows[1].clear();
-        launch_factor();
+        launch_factor(hierPending ? gstream : s);
         stiffnessReady = true;
         stiffnessVersion = sim->version;
         stiffBase = sim->baseVersion;
@@ -1079,11 +1129,14 @@ struct vf_mg {
     DBuf<T> &block_buf(int l);
     // block rows of levels 1 (when assembled) .. L-1 in the precision T of the cycle about to use them,
     // summed in FP64 from the current element matrices
+    // lazy: leave the solver's stream unordered with a pending hierarchy (the cycle waits where it first needs
+    // level >= 2 data); otherwise the stream is ordered after the hierarchy on return
     template <typename T>
-    void ensure_blocks() {
+    void ensure_blocks(bool lazy = false) {
         ensure_stiffness();
         const int p = pidx<T>();
-        hipStream_t s = st();
+        // with the hierarchy still pending the assembly follows the Galerkin chain on its stream
+        hipStream_t s = hierPending ? gstream : st();
         if (blocksReady[p]) {
             // after partial updates: only the stale node rows
             auto &d = dirtyRows[p];
@@ -1098,6 +1151,7 @@ struct vf_mg {
                 }
             }
             d.clear();
+            if (!lazy) wait_hier();
             return;
         }
         if (L >= 2 && l1Blocks && !l1_blocks_unused(p)) {
@@ -1114,6 +1168,8 @@ struct vf_mg {
         }
         blocksReady[p] = true;
         dirtyRows[p].clear();
+        if (hierPending) VF_HIP(hipEventRecord(evHier, gstream));
+        if (!lazy) wait_hier();
     }
     // the main stream waits for the coarsest factorization launched by the last update; a non-positive
     // pivot is reported here
@@ -1202,10 +1258,10 @@ struct vf_mg {
         ctmp_out.resize(ldA);
         cholFlag.resize(1);
     }
-    void launch_factor() {
+    // `s`: the stream that queued the coarsest element matrices
+    void launch_factor(hipStream_t s) {
         if (nred == 0) return;  // nothing to solve: the coarse correction is zero
         join_factor();
-        hipStream_t s = st();
         if (!fstream) {
             VF_HIP(hipStreamCreateWithFlags(&fstream, hipStreamNonBlocking));
             VF_HIP(hipEventCreateWithFlags(&evGalerkin, hipEventDisableTiming));
@@ -2120,6 +2176,13 @@ int vf_mg_zero_dirichlet(vf_mg *mg, int64_t l, double *u, int where) {
     VF_API_END
 }
 
+int vf_mg_set_overlap_setup(vf_mg *mg, int enable) {
+    VF_API_BEGIN
+    need(mg, "null argument");
+    mg->overlapSetup = enable != 0;
+    VF_API_END
+}
+
 int vf_mg_set_partial_updates(vf_mg *mg, int enable) {
     VF_API_BEGIN
     need(mg, "null argument");
@@ -2307,16 +2370,7 @@ static void lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
     double total = 0;
     int64_t count = 0;
     for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) {
-        if (l < numLayers) {
-            // decrementFabricationMaskHeightByLayer (TensorProductSimulator.hh:311-324)
-            sim->maskHeight -= (double)layerIncrement * sim->h[1];
-            sim->firstMasked -= layerIncrement;
-            sim->firstDetached = sim->firstMasked + 1;
-            sim->refresh_grid();
-            launch_zero_young_layers(sim->g, (int)sim->firstMasked, (int)(sim->firstMasked + layerIncrement),
-                                     sim->E.data(), sim->E32.data(), s);
-            ++sim->version;
-        }
+        if (l < numLayers) sim->remove_top_layers(layerIncrement);
         mg->sync_masks();
         if (sim->hasForce) d2d(f.data(), sim->force.data(), n, s);
         else launch_fill<double>(n, f.data(), 0.0, s);
@@ -2474,6 +2528,7 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
             vf_mg &m = *pipes[p].mg;
             m.symmetricGS = mg->symmetricGS;
             m.useGraphs = useGraphs != 0;
+            m.partialUpdates = mg->partialUpdates;
             std::copy(mg->kuChoice, mg->kuChoice + 2, m.kuChoice);
             std::copy(mg->gsChoice, mg->gsChoice + 2, m.gsChoice);
             std::copy(&mg->l1Tuned[0][0], &mg->l1Tuned[0][0] + 4, &m.l1Tuned[0][0]);
@@ -2495,7 +2550,10 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
                     vf_mg &M = *pp.mg;
                     hipStream_t st = pp.stream;
                     for (int k = p; k < nl; k += P) {
-                        S.set_mask_height(S.h[1] * (double)heights[k], true);
+                        // the pipeline's layers come top down: after its first layer the hierarchy is updated
+                        // by bands (partial stiffness updates)
+                        if (k == p) S.set_mask_height(S.h[1] * (double)heights[k], true);
+                        else S.remove_top_layers(heights[k - P] - heights[k]);
                         M.sync_masks();
                         if (S.hasForce) d2d(pp.f.data(), S.force.data(), n, st);
                         else launch_fill<double>(n, pp.f.data(), 0.0, st);

@@ -34,7 +34,19 @@ constexpr int NTW = TE * TE; // work-items
 
 __device__ __forceinline__ constexpr int bitof(int a) { return 1 << (2 - a); }
 
-// y = W^T diag/offdiag(Khat/64) W u for one element, u and y as [8 nodes][3 components] (y may alias u)
+// Rows of Khat coupled by its off-diagonal entries: (k, d) with (k ^ e_d ^ e_c, c) for c != d, k_d != k_c. They
+// form 14 groups of one to three rows (listed as k * 3 + d), closed under the coupling, so the product can be
+// formed one group at a time in place.
+struct WalshGroup {
+    int n;
+    int r[3];
+};
+__device__ constexpr WalshGroup kWalshGroups[14] = {
+    {1, {0, 0, 0}},   {1, {1, 0, 0}},   {1, {2, 0, 0}},    {2, {3, 14, 0}},  {2, {4, 8, 0}},
+    {3, {5, 7, 12}},  {2, {6, 13, 0}},  {3, {9, 16, 20}},  {2, {10, 15, 0}}, {2, {11, 18, 0}},
+    {2, {17, 19, 0}}, {1, {21, 0, 0}},  {1, {22, 0, 0}},   {1, {23, 0, 0}}};
+
+// y = W^T diag/offdiag(Khat/64) W u for one element, u and y as [8 nodes][3 components], in place
 template <typename T>
 __device__ __forceinline__ void walsh_apply(const T *__restrict__ C, T Ee, T v[8][3]) {
     // forward butterflies: v[m][d] -> z[k][d]
@@ -52,17 +64,22 @@ __device__ __forceinline__ void walsh_apply(const T *__restrict__ C, T Ee, T v[8
             }
         }
     }
-    // zh[k][d] = E (Khat[(k,d),(k,d)] z[k][d] + sum_{c != d, k_d != k_c} Khat[(k,d),(k^e_d^e_c,c)] z[..][c]) / 64
-    T zh[8][3];
+    // zh[k][d] = E (Khat[(k,d),(k,d)] z[k][d] + sum_{c != d, k_d != k_c} Khat[(k,d),(k^e_d^e_c,c)] z[..][c]) / 64,
+    // group by group: a group's rows only read the group's own entries, so they are overwritten after the group
+    // (the temporaries are one group, not a second 24-entry array)
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        // coefficients of row group k (nine values, loaded through an opaque pointer so they are not
-        // hoisted out of the element loop and spilled)
-        const T *Cg = C + 16 * k;
-        asm volatile("" : "+s"(Cg));
-        const CT<T> *Ck = (const CT<T> *)Cg;
+    for (int gi = 0; gi < 14; ++gi) {
+        const WalshGroup grp = kWalshGroups[gi];
+        T out[3];
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
+        for (int q = 0; q < 3; ++q) {
+            if (q >= grp.n) continue;
+            const int k = grp.r[q] / 3, d = grp.r[q] % 3;
+            // coefficients of row group k (nine values, loaded through an opaque pointer so they are not
+            // hoisted out of the element loop and spilled)
+            const T *Cg = C + 16 * k;
+            asm volatile("" : "+s"(Cg));
+            const CT<T> *Ck = (const CT<T> *)Cg;
             T acc = Ck[3 * d + d] * v[k][d];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -71,7 +88,12 @@ __device__ __forceinline__ void walsh_apply(const T *__restrict__ C, T Ee, T v[8
                 if (kd == kc) continue;
                 acc += Ck[3 * d + c] * v[k ^ bitof(d) ^ bitof(c)][c];
             }
-            zh[k][d] = Ee * acc;
+            out[q] = Ee * acc;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (q >= grp.n) continue;
+            v[grp.r[q] / 3][grp.r[q] % 3] = out[q];
         }
     }
     // backward butterflies: y[m][d] = sum_k W[k][m] zh[k][d]
@@ -83,21 +105,17 @@ __device__ __forceinline__ void walsh_apply(const T *__restrict__ C, T Ee, T v[8
             if (k & bb) continue;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                const T z0 = zh[k][d], z1 = zh[k | bb][d];
-                zh[k][d] = z0 - z1;
-                zh[k | bb][d] = z0 + z1;
+                const T z0 = v[k][d], z1 = v[k | bb][d];
+                v[k][d] = z0 - z1;
+                v[k | bb][d] = z0 + z1;
             }
         }
     }
-#pragma unroll
-    for (int m = 0; m < 8; ++m)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) v[m][d] = zh[m][d];
 }
 
 // dotp (SET mode, optional): per-workgroup partial sums of <u, y> over the owned nodes, in a fixed order.
 template <typename T, int MODE>
-__global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ C, const T *__restrict__ E,
+__global__ void __launch_bounds__(NTW, sizeof(T) == 8 ? 3 : 5) k_ku_walsh(Grid g, const T *__restrict__ C, const T *__restrict__ E,
                                                   const T *__restrict__ u, T *__restrict__ y, const T *__restrict__ b,
                                                   const uint8_t *__restrict__ dmask, int ntZ, int xchunk,
                                                   double *__restrict__ dotp) {
@@ -157,7 +175,6 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
         }
     };
 
-    T uf[4][3];       // u at the element's face on the current node plane
     T carry[4][3];    // contributions of the previous element to the current node plane
     double dacc = 0;  // <u, y> over the owned nodes (SET mode with dotp)
 #pragma unroll
@@ -166,8 +183,6 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
     store_plane((x0 + 1) & 1);
     fetch_plane(x0);
     T Enext = fetch_E(x0 - 1);
-    __syncthreads();
-    read_face((x0 + 1) & 1, uf);
     for (int ex = x0 - 1; ex < x1; ++ex) {
         // element plane ex: node planes ex (face i = 0) and ex + 1 (face i = 1)
         const int buf = (ex + 1) & 1;
@@ -189,19 +204,13 @@ __global__ void __launch_bounds__(NTW) k_ku_walsh(Grid g, const T *__restrict__ 
             }
         }
         __syncthreads();
-        T ub[4][3];
-        read_face(buf, ub);
-        // u at the owned node (Y, Z) of plane ex: corner (1,1) of the element's face i = 0
-        const T un0 = uf[3][0], un1 = uf[3][1], un2 = uf[3][2];
+        // both node planes of the element are in LDS: plane ex (stored during the previous step) and plane
+        // ex + 1; the slot of plane ex is next overwritten after this step's second barrier
         T v[8][3];
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                v[q][c] = uf[q][c];      // local node 0jk
-                v[4 + q][c] = ub[q][c];  // local node 1jk
-                uf[q][c] = ub[q][c];
-            }
+        read_face(buf ^ 1, v);       // local nodes 0jk (face i = 0)
+        read_face(buf, v + 4);       // local nodes 1jk (face i = 1)
+        // u at the owned node (Y, Z) of plane ex: corner (1,1) of the element's face i = 0
+        const T un0 = v[3][0], un1 = v[3][1], un2 = v[3][2];
         walsh_apply<T>(C, Ee, v);
         if (ex >= x0) {
 #pragma unroll
