@@ -432,7 +432,7 @@ struct vf_mg {
     hipStream_t gstream = nullptr;
     hipEvent_t evModuli = nullptr, evHier = nullptr;
     bool hierPending = false;
-    bool overlapSetup = true;  // queue the Galerkin chain on gstream (false: on the solver's stream)
+    bool overlapSetup = false;  // queue the Galerkin chain on gstream (measured no faster on the full-size cantilever)
     void ensure_gstream() {
         if (gstream) return;
         VF_HIP(hipStreamCreateWithFlags(&gstream, hipStreamNonBlocking));
@@ -1364,8 +1364,17 @@ struct vf_mg {
         if (!dirichletSatisfied)
             launch_enforce_dirichlet(sim->ndir, sim->dnodes.data(), sim->dnmask.data(), g.nnodes, sim->dvals.data(), x, s);
         launch_dot_active(g, b, b, part, sc + 7, s);
+        // r = b - K x; a zero initial guess (every design iteration that does not warm-start) skips the product:
+        // r = b on the active rows with the constrained components zeroed, exactly what the product gives
+        launch_dot_active(g, x, x, part, sc + 8, s);
+        const bool xZero = sim->read_scalar(sc + 8) == 0.0;
         launch_fill<double>(n0, r, 0.0, s);
-        apply<double>(0, x, r, b, AM_RESIDUAL, true);
+        if (xZero) {
+            launch_copy_active<double, double>(g, 0, b, r, s);
+            zero_dirichlet<double>(0, r);
+        } else {
+            apply<double>(0, x, r, b, AM_RESIDUAL, true);
+        }
         launch_dot_active(g, r, r, part, sc + 4, s);
         const double bb = sim->read_scalar(sc + 7);
         double rr = sim->read_scalar(sc + 4);
@@ -1378,9 +1387,9 @@ struct vf_mg {
         // direction update (and once after the loop), and the residual update also writes the FP32
         // right-hand side of the next cycle.
         const bool fused = !cb && mgIter > 0 && nu > 0 && L > 0;
-        // the direction updates cover the active rows only; the element products also read the detached
-        // rows (with zero moduli), which must therefore hold finite values
-        if (fused) launch_fill<double>(n0, d, 0.0, s);
+        // the direction updates cover the active rows only (the first one writes them without reading d); the
+        // element products also read the detached rows (with zero moduli), which must therefore hold finite values
+        if (fused) launch_zero_detached<double>(g, d, s);
         float *r32 = nullptr;
         if (fused && prec == VF_PREC_FP32) {
             ensure_buffers<float>(0);
@@ -2529,6 +2538,9 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
             m.symmetricGS = mg->symmetricGS;
             m.useGraphs = useGraphs != 0;
             m.partialUpdates = mg->partialUpdates;
+            // one stream per pipeline: the concurrency comes from the pipelines (the device has few hardware
+            // queues; a side stream per pipeline would share them)
+            m.overlapSetup = false;
             std::copy(mg->kuChoice, mg->kuChoice + 2, m.kuChoice);
             std::copy(mg->gsChoice, mg->gsChoice + 2, m.gsChoice);
             std::copy(&mg->l1Tuned[0][0], &mg->l1Tuned[0][0] + 4, &m.l1Tuned[0][0]);
