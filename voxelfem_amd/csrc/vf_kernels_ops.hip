@@ -750,91 +750,107 @@ __global__ void __launch_bounds__(TPB) k_elem_apply(Grid g, const T *__restrict_
 
 // ------------------------------------------------------------------ transfers
 // r_c(n) = sum_{o in {-1,0,1}^3} prod_d (1 - |o_d|/2) r_f(2n + o) over active fine nodes
-// (MultigridSolver.hh:214-275); every detached coarse row is zeroed.
-template <typename T>
+// (MultigridSolver.hh:214-275); every detached coarse row is zeroed. The weights are separable, so a
+// work-item owns one coarse (y, z) column and marches along x through its chunk of coarse planes: each
+// fine plane's 3x3 in-plane sum is formed once and serves the coarse planes on both sides of it.
 // xc (optional): the coarse iterate, zeroed in the same pass (the cycle starts the coarse level from zero)
+template <typename T>
 __global__ void __launch_bounds__(TPB) k_restrict(Grid gf, Grid gc, const T *__restrict__ rf, T *__restrict__ rc,
-                                                  T *__restrict__ xc) {
+                                                  T *__restrict__ xc, int xchunk) {
+    const int ncy = gc.nn[1], ncz = gc.nn[2];
     const int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= gc.nnodes) return;
-    const int nz = gc.nn[2], ix = t / (gc.nn[1] * nz), iy = (t / nz) % gc.nn[1], iz = t % nz;
-    T v0 = 0, v1 = 0, v2 = 0;
-    if (iy < gc.ydet) {
-        const int Nf = gf.nnodes;
+    if (t >= ncy * ncz) return;
+    const int cy = t / ncz, cz = t - cy * ncz;
+    const int cx0 = blockIdx.y * xchunk, cx1 = min(cx0 + xchunk, gc.nn[0]);
+    const bool act = cy < gc.ydet;
+    const int Nf = gf.nnodes, Nc = gc.nnodes, nfy = gf.nn[1], nfz = gf.nn[2];
+    auto plane = [&](int fx, T out[3]) {
+        out[0] = out[1] = out[2] = T(0);
+        if (!act || (unsigned)fx >= (unsigned)gf.nn[0]) return;
 #pragma unroll
-        for (int ox = -1; ox <= 1; ++ox) {
-            const int fx = 2 * ix + ox;
-            if ((unsigned)fx >= (unsigned)gf.nn[0]) continue;
+        for (int oy = -1; oy <= 1; ++oy) {
+            const int fy = 2 * cy + oy;
+            if ((unsigned)fy >= (unsigned)gf.ydet) continue;
 #pragma unroll
-            for (int oy = -1; oy <= 1; ++oy) {
-                const int fy = 2 * iy + oy;
-                if ((unsigned)fy >= (unsigned)gf.ydet) continue;
-#pragma unroll
-                for (int oz = -1; oz <= 1; ++oz) {
-                    const int fz = 2 * iz + oz;
-                    if ((unsigned)fz >= (unsigned)gf.nn[2]) continue;
-                    const T w = T((ox ? 0.5 : 1.0) * (oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
-                    const int m = (fx * gf.nn[1] + fy) * gf.nn[2] + fz;
-                    v0 += w * rf[m];
-                    v1 += w * rf[Nf + m];
-                    v2 += w * rf[2 * Nf + m];
-                }
+            for (int oz = -1; oz <= 1; ++oz) {
+                const int fz = 2 * cz + oz;
+                if ((unsigned)fz >= (unsigned)nfz) continue;
+                const T w = T((oy ? 0.5 : 1.0) * (oz ? 0.5 : 1.0));
+                const int m = (fx * nfy + fy) * nfz + fz;
+                out[0] += w * rf[m];
+                out[1] += w * rf[Nf + m];
+                out[2] += w * rf[2 * Nf + m];
             }
         }
-    }
-    rc[t] = v0;
-    rc[gc.nnodes + t] = v1;
-    rc[2 * gc.nnodes + t] = v2;
-    if (xc) {
-        xc[t] = T(0);
-        xc[gc.nnodes + t] = T(0);
-        xc[2 * gc.nnodes + t] = T(0);
+    };
+    T prev[3];
+    plane(2 * cx0 - 1, prev);
+    for (int cx = cx0; cx < cx1; ++cx) {
+        T a[3], b[3];
+        plane(2 * cx, a);
+        plane(2 * cx + 1, b);
+        const int n = (cx * ncy + cy) * ncz + cz;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            rc[(size_t)c * Nc + n] = T(0.5) * prev[c] + a[c] + T(0.5) * b[c];
+            if (xc) xc[(size_t)c * Nc + n] = T(0);
+            prev[c] = b[c];
+        }
     }
 }
 
 // x_f (+)= P x_c, trilinear (MultigridSolver.hh:162-212). accumulate: active fine rows only
-// (accum_interpolation); otherwise every fine row is overwritten (interpolation).
+// (accum_interpolation); otherwise every fine row is overwritten (interpolation). A work-item owns one
+// fine (y, z) column and marches along x through its chunk of coarse elements: the bilinear (y, z)
+// interpolant of each coarse plane is formed once and serves the fine planes on both sides of it.
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_interpolate(Grid gf, Grid gc, const T *__restrict__ xc, T *__restrict__ xf,
-                                                     int accumulate) {
-    const int nz = gf.nn[2];
-    const int rows = accumulate ? gf.ydet : gf.nn[1];
-    const int slab = rows * nz;
+                                                     int accumulate, int xchunk) {
+    const int nfy = gf.nn[1], nfz = gf.nn[2];
+    const int rows = accumulate ? gf.ydet : nfy;
     const int t = blockIdx.x * TPB + threadIdx.x;
-    if (t >= gf.nn[0] * slab) return;
-    const int ix = t / slab, r = t - ix * slab, iy = r / nz, iz = r - iy * nz;
-    const int id[3] = {ix, iy, iz};
-    int e[3];
-    T w0[3], w1[3];
+    if (t >= rows * nfz) return;
+    const int fy = t / nfz, fz = t - fy * nfz;
+    const int ncy = gc.nn[1], ncz = gc.nn[2], Nc = gc.nnodes, Nf = gf.nnodes;
+    // coarse rows / columns of the (y, z) interpolant and their weights (the second index is clamped
+    // where its weight is zero)
+    const int cy = min(fy >> 1, gc.ne[1]), cz = min(fz >> 1, gc.ne[2]);
+    const bool oy = fy & 1, oz = fz & 1;
+    const T wy1 = oy ? T(0.5) : T(0), wy0 = oy ? T(0.5) : T(1);
+    const T wz1 = oz ? T(0.5) : T(0), wz0 = oz ? T(0.5) : T(1);
+    const int cy1 = oy ? cy + 1 : cy, cz1 = oz ? cz + 1 : cz;
+    const int o00 = cy * ncz + cz, o01 = cy * ncz + cz1, o10 = cy1 * ncz + cz, o11 = cy1 * ncz + cz1;
+    auto inplane = [&](int cx, T out[3]) {
+        const int pb = cx * ncy * ncz;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        e[d] = min(id[d] >> 1, gc.ne[d]);
-        const int lf = id[d] - 2 * e[d];
-        w0[d] = lf ? T(0.5) : T(1);
-        w1[d] = lf ? T(0.5) : T(0);
-    }
-    const int Nc = gc.nnodes;
-    T v0 = 0, v1 = 0, v2 = 0;
+        for (int c = 0; c < 3; ++c) {
+            const T *p = xc + (size_t)c * Nc + pb;
+            out[c] = wy0 * (wz0 * p[o00] + wz1 * p[o01]) + wy1 * (wz0 * p[o10] + wz1 * p[o11]);
+        }
+    };
+    auto put = [&](int fx, const T v[3]) {
+        const size_t n = ((size_t)fx * nfy + fy) * nfz + fz;
 #pragma unroll
-    for (int lc = 0; lc < 8; ++lc) {
-        const int a = (lc >> 2) & 1, bb = (lc >> 1) & 1, cc = lc & 1;
-        const T coeff = (a ? w1[0] : w0[0]) * (bb ? w1[1] : w0[1]) * (cc ? w1[2] : w0[2]);
-        if (coeff == T(0)) continue;
-        const int m = ((e[0] + a) * gc.nn[1] + e[1] + bb) * gc.nn[2] + e[2] + cc;
-        v0 += coeff * xc[m];
-        v1 += coeff * xc[Nc + m];
-        v2 += coeff * xc[2 * Nc + m];
+        for (int c = 0; c < 3; ++c) {
+            if (accumulate) xf[(size_t)c * Nf + n] += v[c];
+            else xf[(size_t)c * Nf + n] = v[c];
+        }
+    };
+    const int ne0 = gc.ne[0];
+    const int e0 = blockIdx.y * xchunk, e1 = min(e0 + xchunk, ne0);
+    T a[3];
+    inplane(e0, a);
+    for (int e = e0; e < e1; ++e) {
+        T b[3];
+        inplane(e + 1, b);
+        put(2 * e, a);
+        const T h[3] = {T(0.5) * (a[0] + b[0]), T(0.5) * (a[1] + b[1]), T(0.5) * (a[2] + b[2])};
+        put(2 * e + 1, h);
+        a[0] = b[0];
+        a[1] = b[1];
+        a[2] = b[2];
     }
-    const int n = (ix * gf.nn[1] + iy) * nz + iz, Nf = gf.nnodes;
-    if (accumulate) {
-        xf[n] += v0;
-        xf[Nf + n] += v1;
-        xf[2 * Nf + n] += v2;
-    } else {
-        xf[n] = v0;
-        xf[Nf + n] = v1;
-        xf[2 * Nf + n] = v2;
-    }
+    if (e1 == ne0) put(2 * ne0, a);  // the last fine plane, on the last coarse plane
 }
 
 inline unsigned nblocks(long count) { return (unsigned)((count + TPB - 1) / TPB); }
@@ -972,17 +988,39 @@ void launch_block_gs(const Grid &g, const T *blocks, T *u, const T *b, const uin
     VF_HIP(hipGetLastError());
 }
 
+// x-chunks of a marching transfer: about TARGET workgroups in all, at least MINLEN planes per chunk; a
+// small level (fewer than TARGET workgroups even at one plane each), whose launch is latency-bound, gets
+// one plane per workgroup
+static inline int transfer_chunks(unsigned planeBlocks, int len, int minLen) {
+    constexpr unsigned TARGET = 2048;
+    if ((long)planeBlocks * len <= (long)TARGET) return std::max(1, std::min(len, 65535));
+    int c = (int)std::max(1u, TARGET / std::max(1u, planeBlocks));
+    c = std::min(c, std::max(1, len / minLen));
+    return std::max(1, std::min(c, 65535));
+}
 template <typename T>
 void launch_restrict(const Grid &gf, const Grid &gc, const T *rf, T *rc, hipStream_t s, T *xzero) {
-    hipLaunchKernelGGL(k_restrict<T>, dim3(nblocks(gc.nnodes)), dim3(TPB), 0, s, gf, gc, rf, rc, xzero);
+    if (gf.nn[0] != 2 * gc.nn[0] - 1 || gf.nn[1] != 2 * gc.nn[1] - 1 || gf.nn[2] != 2 * gc.nn[2] - 1)
+        throw std::runtime_error("restriction: the fine grid is not the refinement of the coarse grid");
+    const unsigned pb = nblocks((long)gc.nn[1] * gc.nn[2]);
+    const int chunks = transfer_chunks(pb, gc.nn[0], 4);
+    const int xchunk = (gc.nn[0] + chunks - 1) / chunks;
+    const int nch = (gc.nn[0] + xchunk - 1) / xchunk;
+    hipLaunchKernelGGL(k_restrict<T>, dim3(pb, nch), dim3(TPB), 0, s, gf, gc, rf, rc, xzero, xchunk);
     VF_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_interpolate(const Grid &gf, const Grid &gc, const T *xc, T *xf, int accumulate, hipStream_t s) {
-    const long count = (long)gf.nn[0] * (accumulate ? gf.ydet : gf.nn[1]) * gf.nn[2];
-    if (!count) return;
-    hipLaunchKernelGGL(k_interpolate<T>, dim3(nblocks(count)), dim3(TPB), 0, s, gf, gc, xc, xf, accumulate);
+    if (gf.nn[0] != 2 * gc.nn[0] - 1 || gf.nn[1] != 2 * gc.nn[1] - 1 || gf.nn[2] != 2 * gc.nn[2] - 1)
+        throw std::runtime_error("interpolation: the fine grid is not the refinement of the coarse grid");
+    const long count = (long)(accumulate ? gf.ydet : gf.nn[1]) * gf.nn[2];
+    if (!count || gc.ne[0] < 1) return;
+    const unsigned pb = nblocks(count);
+    const int chunks = transfer_chunks(pb, gc.ne[0], 2);
+    const int xchunk = (gc.ne[0] + chunks - 1) / chunks;
+    const int nch = (gc.ne[0] + xchunk - 1) / xchunk;
+    hipLaunchKernelGGL(k_interpolate<T>, dim3(pb, nch), dim3(TPB), 0, s, gf, gc, xc, xf, accumulate, xchunk);
     VF_HIP(hipGetLastError());
 }
 
