@@ -231,12 +231,16 @@ int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, i
                int uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
                double *perLayerCompliance, int64_t *perLayerIterations, vf_pcg_callback itcb, void *ituser,
                vf_lbl_callback lblcb, void *lbluser);
-/* The same evaluation with the layer solves batched: the layers are spread over `pipelines`
- * concurrent device pipelines (each a copy of the simulator and hierarchy on its own stream, driven by
- * its own host thread; 0 = 8), and with useGraphs the steady-state PCG iteration of each layer solve
- * is captured once into a HIP graph and replayed. Results are summed in layer order, so objective and
- * gradient equal the sequential evaluator's (LayerByLayer.hh:223-300). No reference counterpart for
- * the two extra arguments. */
+/* The same evaluation with the layer solves batched (LayerByLayer.hh:223-300 with the zero guess): the
+ * partial structures of a batch are laid side by side along x in one stacked grid, separated by one
+ * coarsest-level element of void, and solved as one block-diagonal problem: one hierarchy, one Galerkin
+ * chain, one V-cycle launch sequence per iteration for the whole batch, with per-structure PCG scalars
+ * and stopping tests and a batched block-diagonal coarsest solve. `pipelines` caps the structures per
+ * batch (0: as many as fit 2^25 stacked nodes); with useGraphs the steady-state iteration is captured once
+ * into a HIP graph and replayed. Each structure's iterates are those of its own solve up to the rounding of
+ * the PCG reductions; results are summed in layer order. Needs a multigrid preconditioner and box elements
+ * (otherwise the layers are solved one after the other, with the same results). No reference counterpart
+ * for the two extra arguments. */
 int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
                             int64_t smoothing, int fullMultigrid, int precision, int64_t pipelines, int useGraphs,
                             double *objective, double *grad, int gradWhere, double *perLayerCompliance,

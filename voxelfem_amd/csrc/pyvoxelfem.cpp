@@ -633,8 +633,9 @@ struct LBL {
     std::vector<double> perLayer;
     std::vector<int64_t> perLayerIters;
     int precision = VF_PREC_FP64;
-    // batched evaluation (extension): concurrent pipelines (0 = default) and graph-replayed PCG iterations;
-    // batched = false runs the layer solves one after the other on the caller's solver
+    // batched evaluation (extension): the layer solves as stacked batches of at most `pipelines` structures
+    // (0 = as many as fit) with graph-replayed PCG iterations; batched = false runs the layer solves one
+    // after the other on the caller's solver
     bool batched = true;
     int64_t pipelines = 0;
     bool useGraphs = true;

@@ -162,6 +162,7 @@ struct vf_sim {
     double maskHeight = std::numeric_limits<double>::infinity();
     DBuf<double> s_in, s_out, s_aux, partials, scal;
     DBuf<float> s32a, s32b;
+    uint64_t bcVersion = 0;    // bumped whenever the constraint masks change
     uint64_t version = 0;      // bumped whenever E or K0 changes
     uint64_t baseVersion = 0;  // the same, except when layers are removed from the top of the mask
 
@@ -395,6 +396,8 @@ struct GraphGuard {
 };
 }  // namespace
 
+struct StackedBatch;  // a stacked layer-by-layer batch (see lbl_stacked_batch)
+
 struct vf_mg {
     vf_sim *sim;
     int L;
@@ -479,6 +482,37 @@ struct vf_mg {
     std::vector<std::array<int, 2>> dirtyRows[2];  // per precision and level: node rows [lo, hi] to reassemble
     // PCG work
     DBuf<double> pd, pAd, ps, pr, px, pb;
+    // Block-diagonal coarsest level (a stacked layer-by-layer batch): segment k owns the coarsest node planes
+    // [k Pc, k Pc + Wc]; the planes between segments border only void elements and are constrained, so the
+    // reduced operator splits into one block per segment. Each block is factored and solved on its own, all
+    // blocks in one batched launch sequence (set `seg` before the first stiffness update).
+    struct SegCoarse {
+        int nseg = 0, Pc = 0, Wc = 0;
+        Grid gsub{};
+        std::vector<int> n;
+        std::vector<int> ydet;  // first detached coarsest row of each segment
+        int nb = 1, kb = 1, ld = 64;
+        DBuf<int> dn, red, redToFull, flags;
+        DBuf<double> B, Linv, X, XT, tin, tout;
+        bool pending = false;
+    };
+    std::unique_ptr<SegCoarse> seg;
+    // the stacked batches of the last batched layer-by-layer evaluation on this solver: their grids, masks,
+    // hierarchy buffers and work vectors are reused by the next evaluation with the same layers and
+    // constraints (the moduli, hierarchy and factors are recomputed every time)
+    std::vector<std::shared_ptr<StackedBatch>> stackCache;
+    // the coarsened element matrices (Chat, the level-2 path matrices and the Walsh form of Chat_0) of
+    // another solver, so that both build identical level operators from the same moduli
+    void copy_coarsening(const vf_mg &o) {
+        std::copy(o.Chat, o.Chat + 8 * 576, Chat);
+        std::copy(o.phi, o.phi + 512, phi);
+        auto dup = [&](auto &dst, const auto &src) {
+            dst.resize(src.size());
+            d2d(dst.data(), src.data(), src.size(), st());
+        };
+        dup(dChat, o.dChat); dup(dChat32, o.dChat32); dup(dphi, o.dphi); dup(dM2, o.dM2);
+        dup(dCw, o.dCw); dup(dCw32, o.dCw32);
+    }
     // level-0 views used by the cycles (set per preconditioner call)
     double *x0v = nullptr, *b0v = nullptr;
     // what debug_get_x/b (MultigridSolver.hh:1158-1159) read: the level buffers of the last cycle, in the
@@ -873,6 +907,16 @@ struct vf_mg {
     }
     template <typename T>
     void coarse_solve(const T *b, T *x) {
+        if (seg) {
+            ensure_factor();
+            seg_finish();
+            const SegCoarse &S = *seg;
+            const Grid &g = lv[L]->g;
+            launch_coarse_chol_apply_batched<T>(S.nseg, S.dn.data(), S.ld, S.X.data(), S.XT.data(), S.redToFull.data(),
+                                                (size_t)S.ld, g.nnodes, (size_t)S.Pc * g.nn[1] * g.nn[2], b, x,
+                                                S.tin.data(), S.tout.data(), st());
+            return;
+        }
         ensure_factor();
         finish_factor();
         launch_coarse_chol_apply<T>(nred, Ainv.data(), AinvT.data(), dRedToFull.data(), lv[L]->g.nnodes, b, x, ctmp_in.data(),
@@ -1198,7 +1242,101 @@ struct vf_mg {
     }
     // reduced coarsest operator: Dirichlet components and detached nodes removed
     // (findFixedVars, TensorProductSimulator.hh:1181-1195), then its FP64 inverse.
+    // per-segment reduced maps (the same DoF order as a single structure's prepare_coarsest) and buffers
+    void seg_prepare() {
+        SegCoarse &S = *seg;
+        const Grid &g = lv[L]->g;
+        const std::vector<uint8_t> &mask = L == 0 ? sim->hmask : lv[L]->hmask;
+        const int64_t nes[3] = {S.Wc, g.ne[1], g.ne[2]};
+        S.gsub = make_grid(nes);
+        const Grid &gs = S.gsub;
+        const size_t rs = 3 * (size_t)gs.nnodes;
+        std::vector<int> red(rs * S.nseg, -1);
+        std::vector<std::vector<int>> r2f(S.nseg);
+        S.n.assign(S.nseg, 0);
+        int bw = 0, nmax = 0;
+        for (int k = 0; k < S.nseg; ++k) {
+            int *rk = red.data() + k * rs;
+            for (int n = 0; n < gs.nnodes; ++n) {
+                const int lx = n / (gs.nn[1] * gs.nn[2]), iy = (n / gs.nn[2]) % gs.nn[1], iz = n % gs.nn[2];
+                const int m = ((k * S.Pc + lx) * g.nn[1] + iy) * g.nn[2] + iz;
+                for (int a = 0; a < 3; ++a) {
+                    const bool fixed = ((mask[m] >> a) & 1) || iy >= S.ydet[k];
+                    if (!fixed) { rk[3 * n + a] = (int)r2f[k].size(); r2f[k].push_back(3 * n + a); }
+                }
+            }
+            S.n[k] = (int)r2f[k].size();
+            nmax = std::max(nmax, S.n[k]);
+            for (int n = 0; n < gs.nnodes; ++n) {
+                const int ix = n / (gs.nn[1] * gs.nn[2]), iy = (n / gs.nn[2]) % gs.nn[1], iz = n % gs.nn[2];
+                for (int a = 0; a < 3; ++a) {
+                    const int rr = rk[3 * n + a];
+                    if (rr < 0) continue;
+                    for (int dx = -1; dx <= 1; ++dx)
+                        for (int dy = -1; dy <= 1; ++dy)
+                            for (int dz = -1; dz <= 1; ++dz) {
+                                const int jx = ix + dx, jy = iy + dy, jz = iz + dz;
+                                if (jx < 0 || jy < 0 || jz < 0 || jx >= gs.nn[0] || jy >= gs.nn[1] || jz >= gs.nn[2])
+                                    continue;
+                                const int mm = (jx * gs.nn[1] + jy) * gs.nn[2] + jz;
+                                for (int c2 = 0; c2 < 3; ++c2) {
+                                    const int cc = rk[3 * mm + c2];
+                                    if (cc >= 0 && cc < rr) bw = std::max(bw, rr - cc);
+                                }
+                            }
+                }
+            }
+        }
+        S.kb = std::max(1, (bw + 63) / 64);
+        S.nb = std::max(1, (nmax + 63) / 64);
+        S.ld = S.nb * 64;
+        const double bytes = (double)S.nseg * (2.0 * S.ld * S.ld + (double)band_storage(S.ld, S.kb)) * 8.0;
+        if (S.ld > 24576 || bytes > 32e9)
+            throw std::runtime_error("stacked coarsest level: " + std::to_string(nmax) +
+                                     " unknowns per segment exceed the coarse-solver limit; use more coarsening levels");
+        std::vector<int> r2fAll((size_t)S.nseg * S.ld, 0);
+        for (int k = 0; k < S.nseg; ++k) std::copy(r2f[k].begin(), r2f[k].end(), r2fAll.begin() + (size_t)k * S.ld);
+        hipStream_t s = st();
+        S.red.resize(red.size());
+        S.redToFull.resize(r2fAll.size());
+        S.dn.resize(S.nseg);
+        h2d(S.red.data(), red.data(), red.size(), s);
+        h2d(S.redToFull.data(), r2fAll.data(), r2fAll.size(), s);
+        h2d(S.dn.data(), S.n.data(), S.n.size(), s);
+        S.B.resize((size_t)S.nseg * band_storage(S.ld, S.kb));
+        S.Linv.resize((size_t)S.nseg * S.nb * 4096);
+        S.X.resize((size_t)S.nseg * S.ld * S.ld);
+        S.XT.resize((size_t)S.nseg * S.ld * S.ld);
+        S.tin.resize((size_t)S.nseg * S.ld);
+        S.tout.resize((size_t)S.nseg * S.ld);
+        S.flags.resize(S.nseg);
+        sync(s);
+    }
+    void seg_factor(hipStream_t s) {
+        SegCoarse &S = *seg;
+        const Grid &g = lv[L]->g;
+        band_cholesky_inverse_batched(S.gsub, lv[L]->Ke.data(), (size_t)S.Pc * g.ne[1] * g.ne[2] * 576, S.red.data(),
+                                      3 * (size_t)S.gsub.nnodes, S.nseg, S.dn.data(), S.nb, S.kb, S.B.data(),
+                                      S.Linv.data(), S.X.data(), S.XT.data(), S.flags.data(), s);
+        S.pending = true;
+    }
+    // a non-positive pivot in any block is reported at the first coarse solve after the factorization
+    void seg_finish() {
+        SegCoarse &S = *seg;
+        if (!S.pending) return;
+        S.pending = false;
+        std::vector<int> f(S.nseg);
+        d2h(f.data(), S.flags.data(), f.size(), st());
+        sync(st());
+        for (int k = 0; k < S.nseg; ++k)
+            if (f[k]) {
+                stiffnessReady = false;
+                throw std::runtime_error("coarsest-level operator is not positive definite (stacked segment " +
+                                         std::to_string(k) + ")");
+            }
+    }
     void prepare_coarsest() {
+        if (seg) { seg_prepare(); return; }
         finish_factor();  // buffers below may be reallocated
         LevelData &c = *lv[L];
         hipStream_t s = st();
@@ -1260,6 +1398,7 @@ struct vf_mg {
     }
     // `s`: the stream that queued the coarsest element matrices
     void launch_factor(hipStream_t s) {
+        if (seg) { seg_factor(s); return; }
         if (nred == 0) return;  // nothing to solve: the coarse correction is zero
         join_factor();
         if (!fstream) {
@@ -1808,6 +1947,7 @@ int vf_sim_set_dirichlet(vf_sim *sim, const uint8_t *mask, const double *values)
     const size_t nn = sim->nn();
     for (size_t n = 0; n < nn; ++n) sim->hmask[n] = mask[n] & 7;
     for (size_t i = 0; i < 3 * nn; ++i) sim->hvals[i] = values ? values[i] : 0.0;
+    ++sim->bcVersion;
     upload_dirichlet(sim->hmask, sim->dmask, sim->dnodes, sim->dnmask, sim->ndir, sim->stream);
     std::vector<double> v(3 * std::max(sim->ndir, 1), 0.0);
     int k = 0;
@@ -2485,6 +2625,380 @@ int vf_lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, i
     VF_API_END
 }
 
+// ---------------------------------------------------------------- stacked layer-by-layer batches
+// The zero-initial-guess layer-by-layer evaluation (LayerByLayer.hh:223-296) solves one partial structure
+// per layer. A batch of them is solved here as one problem: the structures lie side by side along x in
+// segments of P = W + 2^L element columns (W the design's width), separated by 2^L columns of void, one
+// element of the coarsest level. Every level's operator is then block-diagonal over the segments (the
+// void columns couple nothing, their interior nodes and each segment's detached rows are constrained), so
+// one hierarchy, one Galerkin chain and one V-cycle launch sequence serve the whole batch and reproduce,
+// segment by segment, the V-cycle of each structure on its own. The outer PCG keeps one set of scalars per
+// segment and stops each segment at its own tolerance.
+namespace {
+constexpr int kSegActive = 0, kSegFinishing = 1, kSegDone = 2;
+}  // namespace
+
+// A stacked batch: its grid, masks, hierarchy and work buffers (kept between evaluations with the same
+// layers, constraints and element size; everything numeric is refreshed each time)
+struct StackedBatch {
+    std::vector<int> top;  // first masked element row of each segment's structure, tallest first
+    int rows = 0, P = 0, L = 0;
+    uint64_t bcVersion = 0;
+    double h[3] = {0, 0, 0};
+    std::unique_ptr<vf_sim> sim;
+    std::unique_ptr<vf_mg> mg;
+    DBuf<int> dtop, dstate;
+    DBuf<double> f, x, sc, part, gst, xx;
+};
+
+namespace {
+// the stacked simulator's geometry and the design's constraints in every segment (their values are not
+// needed: the zero-guess evaluation does not impose them); the void columns and each segment's detached
+// rows are held through the per-node masks of every level (constrain_stacked)
+std::unique_ptr<vf_sim> make_stacked_sim(const vf_sim &base, const std::vector<int> &top, int P, int rows) {
+    const int nseg = (int)top.size();
+    auto c = std::make_unique<vf_sim>();
+    c->stream = base.stream;
+    const int64_t ne[3] = {(int64_t)nseg * P - (P - base.ne[0]), rows, base.ne[2]};
+    for (int d = 0; d < 3; ++d) {
+        c->ne[d] = ne[d];
+        c->h[d] = base.h[d];
+        c->lo[d] = base.lo[d];
+        c->hi[d] = base.lo[d] + base.h[d] * (double)ne[d];
+    }
+    if ((ne[0] + 1) * (ne[1] + 1) * (ne[2] + 1) * 3 >= INT_MAX) throw std::runtime_error("stacked grid too large");
+    c->g = make_grid(ne);
+    hipStream_t s = c->stream;
+    c->rho.resize(c->nel());
+    c->E.resize(c->nel());
+    c->E32.resize(c->nel());
+    const size_t nn = c->nn();
+    const Grid &g = c->g, &gb = base.g;
+    c->hmask.assign(nn, 0);
+    std::vector<size_t> held;  // the design's constrained nodes below the stacked grid's top
+    for (size_t m = 0; m < base.nn(); ++m)
+        if (base.hmask[m] && (int)((m / gb.nn[2]) % gb.nn[1]) < g.nn[1]) held.push_back(m);
+    std::vector<int> nodes;
+    std::vector<uint8_t> cm;
+    nodes.reserve(held.size() * nseg);
+    cm.reserve(held.size() * nseg);
+    for (int k = 0; k < nseg; ++k)
+        for (size_t m : held) {
+            const uint8_t v = base.hmask[m];
+            const int lx = (int)(m / ((size_t)gb.nn[1] * gb.nn[2])), iy = (int)((m / gb.nn[2]) % gb.nn[1]),
+                      iz = (int)(m % gb.nn[2]);
+            const size_t n = ((size_t)(k * P + lx) * g.nn[1] + iy) * g.nn[2] + iz;
+            c->hmask[n] = v;
+            nodes.push_back((int)n);
+            cm.push_back(v);
+        }
+    c->ndir = (int)nodes.size();
+    c->dmask.resize(nn);
+    h2d(c->dmask.data(), c->hmask.data(), nn, s);
+    c->dnodes.resize(std::max<size_t>(nodes.size(), 1));
+    c->dnmask.resize(std::max<size_t>(cm.size(), 1));
+    h2d(c->dnodes.data(), nodes.data(), nodes.size(), s);
+    h2d(c->dnmask.data(), cm.data(), cm.size(), s);
+    c->dvals.resize(3 * std::max<size_t>(nodes.size(), 1));
+    launch_fill<double>(c->dvals.size(), c->dvals.data(), 0.0, s);
+    // every segment's structure is below the tallest one's mask height
+    c->maskHeight = base.h[1] * (double)top[0];
+    c->firstMasked = top[0];
+    c->firstDetached = top[0] + 1;
+    c->refresh_grid();
+    sync(s);
+    return c;
+}
+
+// the numeric state of the stacked simulator from the design's: material, law, element tables, moduli,
+// densities, forces and gravity
+void refresh_stacked_sim(vf_sim &c, const vf_sim &base, int P, const int *dtop, int nseg, const double *Efull,
+                         const float *E32full) {
+    c.young = base.young; c.poisson = base.poisson; c.law = base.law;
+    c.E0 = base.E0; c.Emin = base.Emin; c.gamma = base.gamma; c.q = base.q;
+    std::copy(base.grav, base.grav + 3, c.grav);
+    std::copy(base.K0h, base.K0h + 576, c.K0h);
+    c.structured = base.structured;
+    c.walsh = base.walsh;
+    hipStream_t s = c.stream;
+    auto dup = [&](auto &dst, const auto &src) {
+        dst.resize(src.size());
+        d2d(dst.data(), src.data(), src.size(), s);
+    };
+    dup(c.K0, base.K0); dup(c.K032, base.K032); dup(c.Ftab, base.Ftab); dup(c.Ftab32, base.Ftab32);
+    dup(c.Wtab, base.Wtab); dup(c.Wtab32, base.Wtab32);
+    launch_stack_elems<double>(c.g, base.g, P, nseg, dtop, base.rho.data(), c.rho.data(), s);
+    launch_stack_elems<double>(c.g, base.g, P, nseg, dtop, Efull, c.E.data(), s);
+    launch_stack_elems<float>(c.g, base.g, P, nseg, dtop, E32full, c.E32.data(), s);
+    c.hasForce = base.hasForce;
+    if (base.hasForce) {
+        c.force.resize(3 * c.nn());
+        launch_stack_nodes(c.g, base.g, P, nseg, dtop, base.force.data(), c.force.data(), s);
+    }
+    ++c.version;
+    ++c.baseVersion;
+}
+}  // namespace
+
+// first detached node row of each segment's structure on every level (vf_mg::sync_masks for a structure of
+// that height): ydet[l][k]
+static std::vector<std::vector<int>> stacked_ydet(const vf_mg &m, const std::vector<int> &top, double hy) {
+    std::vector<std::vector<int>> y(m.L + 1, std::vector<int>(top.size()));
+    for (int l = 0; l <= m.L; ++l)
+        for (size_t k = 0; k < top.size(); ++k) {
+            const Grid &g = m.lv[l]->g;
+            if (l == 0) {
+                y[l][k] = std::min(top[k] + 1, g.nn[1]);
+            } else {
+                const int64_t fm = (int64_t)std::ceil(hy * (double)top[k] / m.lv[l]->h[1] - 1e-10);
+                y[l][k] = (int)std::min<int64_t>(fm + 1, g.nn[1]);
+            }
+        }
+    return y;
+}
+
+// hold, on every level, the nodes of the void columns and each segment's rows from its first detached row
+// on: their per-node masks become 7, so the smoothers skip them and the products and residuals are zero
+// there; the constraint lists keep the design's constraints only
+static void constrain_stacked(vf_mg &m, const std::vector<std::vector<int>> &ydet, int P) {
+    const int nseg = (int)ydet[0].size();
+    DBuf<int> dy(nseg);
+    for (int l = 0; l <= m.L; ++l) {
+        const Grid &g = m.lv[l]->g;
+        h2d(dy.data(), ydet[l].data(), nseg, m.st());
+        uint8_t *mask = l == 0 ? m.sim->dmask.data() : m.lv[l]->dmask.data();
+        launch_stack_mask(g, P >> l, (P - (1 << m.L)) >> l, nseg, dy.data(), mask, m.st());
+    }
+    sync(m.st());
+}
+
+// the batch for these layers from the solver's cache, or a new one
+static StackedBatch &stacked_batch_for(vf_mg *base, std::vector<std::shared_ptr<StackedBatch>> &next,
+                                       const std::vector<int> &top, int rows) {
+    vf_sim *bs = base->sim;
+    const int L = base->L, P = (int)bs->ne[0] + (1 << L);
+    for (auto &b : base->stackCache)
+        if (b && b->top == top && b->rows == rows && b->P == P && b->L == L && b->bcVersion == bs->bcVersion &&
+            std::equal(b->h, b->h + 3, bs->h)) {
+            next.push_back(b);
+            b.reset();
+            return *next.back();
+        }
+    auto b = std::make_shared<StackedBatch>();
+    const int nseg = (int)top.size(), W = (int)bs->ne[0];
+    b->top = top;
+    b->rows = rows;
+    b->P = P;
+    b->L = L;
+    b->bcVersion = bs->bcVersion;
+    std::copy(bs->h, bs->h + 3, b->h);
+    hipStream_t s = bs->stream;
+    b->dtop.resize(nseg);
+    h2d(b->dtop.data(), top.data(), top.size(), s);
+    b->sim = make_stacked_sim(*bs, top, P, rows);
+    std::copy(bs->K0h, bs->K0h + 576, b->sim->K0h);
+    b->mg = std::make_unique<vf_mg>(b->sim.get(), L);
+    vf_mg &m = *b->mg;
+    // the coarse levels take the design solver's constraint masks (coarsened when that solver was built,
+    // MultigridSolver.hh:58-103), replicated into every segment, so each segment's hierarchy is that
+    // solver's own
+    for (int l = 1; l <= L; ++l) {
+        LevelData &c = *m.lv[l];
+        const LevelData &bl = *base->lv[l];
+        const Grid &gc = c.g, &gb = bl.g;
+        std::fill(c.hmask.begin(), c.hmask.end(), uint8_t(0));
+        const int Pl = P >> l;
+        for (size_t mm = 0; mm < bl.hmask.size(); ++mm) {
+            const uint8_t v = bl.hmask[mm];
+            if (!v) continue;
+            const int lx = (int)(mm / ((size_t)gb.nn[1] * gb.nn[2])), iy = (int)((mm / gb.nn[2]) % gb.nn[1]),
+                      iz = (int)(mm % gb.nn[2]);
+            if (iy >= gc.nn[1]) continue;
+            for (int k = 0; k < nseg; ++k) c.hmask[((size_t)(k * Pl + lx) * gc.nn[1] + iy) * gc.nn[2] + iz] = v;
+        }
+        upload_dirichlet(c.hmask, c.dmask, c.dnodes, c.dnmask, c.ndir, m.st());
+    }
+    m.partialUpdates = false;
+    m.overlapSetup = false;
+    const std::vector<std::vector<int>> ydet = stacked_ydet(m, top, bs->h[1]);
+    constrain_stacked(m, ydet, P);
+    m.seg = std::make_unique<vf_mg::SegCoarse>();
+    m.seg->nseg = nseg;
+    m.seg->Pc = P >> L;
+    m.seg->Wc = W >> L;
+    m.seg->ydet = ydet[L];
+    const Grid &g = b->sim->g;
+    const size_t n0 = 3 * (size_t)g.nnodes;
+    b->f.resize(n0);
+    b->x.resize(n0);
+    b->sc.resize((size_t)16 * nseg);
+    b->dstate.resize(nseg);
+    b->gst.resize(b->sim->nel());
+    b->xx.resize(1);
+    const int xc = walsh_xchunk(g, AM_SET, P);
+    const int tiles = ((g.ydet + 14) / 15) * ((g.nn[2] + 14) / 15), nch = (g.nn[0] + xc - 1) / xc;
+    b->part.resize(std::max<size_t>(2 * (size_t)active_partials(g), (size_t)tiles * nch) + 64);
+    m.pr.resize(n0); m.ps.resize(n0); m.pd.resize(n0); m.pAd.resize(n0);
+    next.push_back(b);
+    return *next.back();
+}
+
+// One stacked batch: heights top[k] (element rows, tallest first); adds each segment's compliance <f, u>
+// and PCG iteration count to comp / its and its gradient (in segment order) to gsum.
+static void lbl_stacked_batch(vf_mg *base, std::vector<std::shared_ptr<StackedBatch>> &next,
+                              const std::vector<int> &top, int rows, int64_t maxIter, double tol, int mgIter, int nu,
+                              bool fullMG, int prec, bool useGraphs, const double *Efull, const float *E32full,
+                              double *comp, int64_t *its, double *gsum) {
+    vf_sim *bs = base->sim;
+    hipStream_t s = bs->stream;
+    const int nseg = (int)top.size();
+    StackedBatch &B = stacked_batch_for(base, next, top, rows);
+    const int P = B.P;
+    vf_sim *S = B.sim.get();
+    refresh_stacked_sim(*S, *bs, P, B.dtop.data(), nseg, Efull, E32full);
+    vf_mg &m = *B.mg;
+    m.copy_coarsening(*base);
+    m.symmetricGS = base->symmetricGS;
+    m.useGraphs = useGraphs;
+    std::copy(base->kuChoice, base->kuChoice + 2, m.kuChoice);
+    std::copy(base->gsChoice, base->gsChoice + 2, m.gsChoice);
+    std::copy(&base->l1Tuned[0][0], &base->l1Tuned[0][0] + 4, &m.l1Tuned[0][0]);
+    m.sync_masks();
+    const Grid &g = S->g;
+    const size_t n0 = 3 * (size_t)g.nnodes;
+    // load: the design's forces in every segment plus each structure's self-weight
+    DBuf<double> &f = B.f, &x = B.x, &sc = B.sc, &part = B.part, &xx = B.xx;
+    DBuf<int> &dstate = B.dstate;
+    if (S->hasForce) d2d(f.data(), S->force.data(), n0, s);
+    else launch_fill<double>(n0, f.data(), 0.0, s);
+    if (S->grav[0] != 0 || S->grav[1] != 0 || S->grav[2] != 0)
+        launch_self_weight(g, S->rho.data(), S->grav, S->h[0] * S->h[1] * S->h[2], f.data(), s);
+    // zero initial guess; like the sequential evaluator, the constrained values are not imposed on it
+    launch_fill<double>(n0, x.data(), 0.0, s);
+    // PCG (vf_mg::pcg, fused form) with per-segment scalars sc[16 k ..]
+    double *r = m.pr.data(), *sv = m.ps.data(), *d = m.pd.data(), *Ad = m.pAd.data();
+    launch_fill<double>((size_t)16 * nseg, sc.data(), 0.0, s);
+    const int xc = walsh_xchunk(g, AM_SET, P);
+    const int tiles = ((g.ydet + 14) / 15) * ((g.nn[2] + 14) / 15), nch = (g.nn[0] + xc - 1) / xc;
+    if (part.size() < (size_t)tiles * nch) throw std::runtime_error("stacked batch: partial-sum buffer too small");
+    std::vector<int> state(nseg, kSegActive);
+    std::vector<double> hsc((size_t)16 * nseg);
+    auto read_sc = [&] {
+        d2h(hsc.data(), sc.data(), hsc.size(), s);
+        sync(s);
+    };
+    launch_seg_dot(g, f.data(), f.data(), part.data(), P, nseg, sc.data(), 7, s);
+    launch_dot_active(g, x.data(), x.data(), part.data(), xx.data(), s);
+    double hx = 0;
+    d2h(&hx, xx.data(), 1, s);
+    sync(s);
+    launch_fill<double>(n0, r, 0.0, s);
+    if (hx == 0.0) {
+        launch_copy_active<double, double>(g, 0, f.data(), r, s);
+        m.zero_dirichlet<double>(0, r);
+        launch_zero_held<double>(g, S->dmask.data(), r, s);
+    } else {
+        m.apply<double>(0, x.data(), r, f.data(), AM_RESIDUAL, true);
+    }
+    launch_seg_dot(g, r, r, part.data(), P, nseg, sc.data(), 4, s);
+    read_sc();
+    const double tol2 = tol * tol;
+    for (int k = 0; k < nseg; ++k) {
+        const double rr = hsc[16 * k + 4], bb = hsc[16 * k + 7];
+        if (std::isnan(rr)) throw NanError("NaN encountered");
+        its[k] = 0;
+        if (!(maxIter > 0 && rr > tol2 * bb)) state[k] = kSegDone;
+    }
+    h2d(dstate.data(), state.data(), state.size(), s);
+    const bool flexible = prec == VF_PREC_FP32;
+    // the element products read d everywhere, also in the void columns (times zero moduli) and in segments
+    // that never iterate: it must hold finite values
+    launch_fill<double>(n0, d, 0.0, s);
+    float *r32 = nullptr;
+    if (prec == VF_PREC_FP32) {
+        m.ensure_buffers<float>(0);
+        r32 = m.lv[0]->b32.data();
+        launch_convert<float, double>(n0, r, r32, s);
+    }
+    auto body = [&](bool firstIt) {
+        m.precondition(r, sv, mgIter, nu, fullMG, prec, true);
+        const double *adp = (flexible && !firstIt) ? Ad : nullptr;
+        launch_seg_scalars(sc.data(), dstate.data(), nseg, 0, s);
+        if (prec == VF_PREC_FP32) {
+            float *s32 = m.lv[0]->x32.data();
+            m.zero_dirichlet<float>(0, s32);
+            launch_zero_held<float>(g, S->dmask.data(), s32, s);
+            launch_seg_dots_s<float>(g, r, s32, adp, part.data(), P, nseg, sc.data(), s);
+            launch_seg_direction_x<float>(g, s32, d, x.data(), sc.data(), dstate.data(), P, nseg, flexible ? 1 : 0,
+                                          firstIt ? 1 : 0, s);
+        } else {
+            m.zero_dirichlet<double>(0, sv);
+            launch_zero_held<double>(g, S->dmask.data(), sv, s);
+            launch_seg_dots_s<double>(g, r, sv, adp, part.data(), P, nseg, sc.data(), s);
+            launch_seg_direction_x<double>(g, sv, d, x.data(), sc.data(), dstate.data(), P, nseg, 0, firstIt ? 1 : 0,
+                                           s);
+        }
+        launch_fine_apply_walsh<double>(g, S->Wtab.data(), S->E.data(), d, Ad, nullptr, m.dm(0), AM_SET, s, part.data(),
+                                        xx.data(), P);
+        launch_seg_sum_chunks(part.data(), tiles, xc, nch, P, nseg, sc.data(), 3, s);
+        launch_seg_scalars(sc.data(), dstate.data(), nseg, 1, s);
+        launch_seg_residual(g, r, Ad, sc.data(), dstate.data(), P, nseg, part.data(), r32, s);
+    };
+    GraphGuard graph;
+    auto capture = [&] {
+        hipGraph_t gr = nullptr;
+        VF_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            body(false);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &gr);
+            if (gr) (void)hipGraphDestroy(gr);
+            throw;
+        }
+        VF_HIP(hipStreamEndCapture(s, &gr));
+        graph.graph = gr;
+        VF_HIP(hipGraphInstantiate(&graph.exec, gr, nullptr, nullptr, 0));
+    };
+    int64_t it = 0;
+    bool first = true, any = std::any_of(state.begin(), state.end(), [](int v) { return v == kSegActive; });
+    if (any) m.ensure_stiffness();
+    while (any) {
+        ++it;
+        if (useGraphs && !first && s != nullptr) {
+            if (!graph.exec) capture();
+            VF_HIP(hipGraphLaunch(graph.exec, s));
+        } else {
+            body(first);
+        }
+        read_sc();
+        bool changed = false;
+        any = false;
+        for (int k = 0; k < nseg; ++k) {
+            if (state[k] == kSegFinishing) { state[k] = kSegDone; changed = true; continue; }
+            if (state[k] != kSegActive) continue;
+            const double rr = hsc[16 * k + 4], bb = hsc[16 * k + 7];
+            if (std::isnan(rr)) throw NanError("NaN encountered at iteration " + std::to_string(it));
+            its[k] = it;
+            if (it >= maxIter || !(rr > tol2 * bb)) { state[k] = kSegFinishing; changed = true; }
+            else any = true;
+        }
+        if (changed) h2d(dstate.data(), state.data(), state.size(), s);
+        first = false;
+    }
+    // the deferred x update of the segments that stopped at the last iteration
+    if (std::any_of(state.begin(), state.end(), [](int v) { return v == kSegFinishing; }))
+        launch_seg_finish_x(g, x.data(), d, sc.data(), dstate.data(), P, nseg, s);
+    // compliance <f, u> per segment and the gradient, folded back onto the design in segment order
+    launch_seg_dot(g, f.data(), x.data(), part.data(), P, nseg, sc.data(), 8, s);
+    DBuf<double> &gst = B.gst;
+    launch_compliance_gradient(g, S->K0.data(), S->rho.data(), x.data(), gst.data(), S->law, S->E0, S->Emin, S->gamma,
+                               S->q, S->grav, S->h[0] * S->h[1] * S->h[2], 0, s);
+    launch_fold_gradient(bs->g, g, P, nseg, B.dtop.data(), gst.data(), gsum, s);
+    read_sc();
+    for (int k = 0; k < nseg; ++k) comp[k] = hsc[16 * k + 8];
+    sync(s);
+}
+
 int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
                             int64_t smoothing, int fullMultigrid, int precision, int64_t pipelines, int useGraphs,
                             double *objective, double *grad, int gradWhere, double *perLayerCompliance,
@@ -2493,128 +3007,83 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
     need(mg && objective && grad, "null argument");
     need(layerIncrement >= 1, "layerIncrement must be positive");
     vf_sim *sim = mg->sim;
-    const int64_t numLayers = sim->ne[1];
-    std::vector<int64_t> heights;
-    for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) heights.push_back(l);
-    const int nl = (int)heights.size();
-    int P = pipelines > 0 ? (int)pipelines : 8;
-    P = std::max(1, std::min(P, nl));
-    const size_t n = 3 * sim->nn(), ne = sim->nel();
-    // the base solver's kernel choices are reused by every pipeline (tuned once, on the full design)
-    sim->set_mask_height(sim->h[1] * (double)numLayers, true);
-    mg->sync_masks();
-    mg->ensure_stiffness();
-    mg->tune<double>();
-    if (precision == VF_PREC_FP32) mg->tune<float>();
-    sync(mg->st());
-
-    struct Pipe {
-        hipStream_t stream = nullptr;
-        std::unique_ptr<vf_sim> sim;
-        std::unique_ptr<vf_mg> mg;
-        DBuf<double> f, u;
-    };
-    std::vector<Pipe> pipes(P);
-    DBuf<double> grads((size_t)nl * ne);
-    std::vector<double> comp(nl, 0.0);
-    std::vector<int64_t> its(nl, 0);
-    auto cleanup = [&]() {
-        for (auto &p : pipes) {
-            if (p.stream) (void)hipStreamSynchronize(p.stream);
-            p.mg.reset();
-            p.sim.reset();
-            p.f.release();
-            p.u.release();
-            if (p.stream) (void)hipStreamDestroy(p.stream);
-            p.stream = nullptr;
+    const int L = mg->L;
+    if (L < 1 || mgIterations < 1 || smoothing < 1 || !sim->walsh) {
+        // the stacked form needs a multigrid preconditioner and the Walsh-basis element product: the
+        // sequential evaluator gives the same results
+        lbl_run(mg, layerIncrement, maxIter, tol, mgIterations, smoothing, fullMultigrid, precision, VF_LBL_INIT_ZERO,
+                0, true, nullptr, false, VF_HOST, objective, grad, gradWhere, perLayerCompliance, perLayerIterations);
+    } else {
+        const int64_t numLayers = sim->ne[1];
+        std::vector<int> heights;
+        for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) heights.push_back((int)l);
+        const int nl = (int)heights.size();
+        hipStream_t s = mg->st();
+        // the base solver's kernel choices serve every batch (tuned once, on the full design)
+        sim->set_mask_height(sim->h[1] * (double)numLayers, true);
+        mg->sync_masks();
+        mg->ensure_stiffness();
+        mg->tune<double>();
+        if (precision == VF_PREC_FP32) mg->tune<float>();
+        // the design's moduli without any mask
+        const size_t ne = sim->nel();
+        DBuf<double> Efull(ne);
+        DBuf<float> E32full(ne);
+        {
+            Grid gf = sim->g;
+            gf.ymask = gf.ne[1];
+            launch_young(gf, sim->rho.data(), Efull.data(), E32full.data(), sim->law, sim->E0, sim->Emin, sim->gamma,
+                         sim->q, 0, s);
         }
-    };
-    try {
-        for (int p = 0; p < P; ++p) {
-            VF_HIP(hipStreamCreateWithFlags(&pipes[p].stream, hipStreamNonBlocking));
-            pipes[p].sim = sim->clone(pipes[p].stream);
-            pipes[p].mg = std::make_unique<vf_mg>(pipes[p].sim.get(), mg->L);
-            vf_mg &m = *pipes[p].mg;
-            m.symmetricGS = mg->symmetricGS;
-            m.useGraphs = useGraphs != 0;
-            m.partialUpdates = mg->partialUpdates;
-            // one stream per pipeline: the concurrency comes from the pipelines (the device has few hardware
-            // queues; a side stream per pipeline would share them)
-            m.overlapSetup = false;
-            std::copy(mg->kuChoice, mg->kuChoice + 2, m.kuChoice);
-            std::copy(mg->gsChoice, mg->gsChoice + 2, m.gsChoice);
-            std::copy(&mg->l1Tuned[0][0], &mg->l1Tuned[0][0] + 4, &m.l1Tuned[0][0]);
-            pipes[p].f.resize(n);
-            pipes[p].u.resize(n);
+        // highest fine row at which any level carries a constraint: a batch grid is cut below the design's
+        // top only above it
+        int bcRow = -1;
+        for (int l = 0; l <= L; ++l) {
+            const std::vector<uint8_t> &hm = l == 0 ? sim->hmask : mg->lv[l]->hmask;
+            const Grid &gl = mg->lv[l]->g;
+            for (size_t n = 0; n < hm.size(); ++n)
+                if (hm[n]) bcRow = std::max(bcRow, (int)((n / gl.nn[2]) % gl.nn[1]) << l);
         }
-        // layer k (height heights[k], largest first) runs on pipeline k % P; each pipeline is driven by
-        // its own host thread, so the pipelines' kernels overlap on the device
-        std::vector<std::exception_ptr> errs(P);
-        std::vector<std::thread> threads;
-        int device = 0;
-        VF_HIP(hipGetDevice(&device));
-        for (int p = 0; p < P; ++p)
-            threads.emplace_back([&, p]() {
-                try {
-                    VF_HIP(hipSetDevice(device));
-                    Pipe &pp = pipes[p];
-                    vf_sim &S = *pp.sim;
-                    vf_mg &M = *pp.mg;
-                    hipStream_t st = pp.stream;
-                    for (int k = p; k < nl; k += P) {
-                        // the pipeline's layers come top down: after its first layer the hierarchy is updated
-                        // by bands (partial stiffness updates)
-                        if (k == p) S.set_mask_height(S.h[1] * (double)heights[k], true);
-                        else S.remove_top_layers(heights[k - P] - heights[k]);
-                        M.sync_masks();
-                        if (S.hasForce) d2d(pp.f.data(), S.force.data(), n, st);
-                        else launch_fill<double>(n, pp.f.data(), 0.0, st);
-                        if (S.grav[0] != 0 || S.grav[1] != 0 || S.grav[2] != 0)
-                            launch_self_weight(S.g, S.rho.data(), S.grav, S.h[0] * S.h[1] * S.h[2], pp.f.data(), st);
-                        launch_fill<double>(n, pp.u.data(), 0.0, st);
-                        its[k] = M.pcg(pp.u.data(), pp.f.data(), maxIter, tol, (int)mgIterations, (int)smoothing,
-                                       fullMultigrid != 0, precision, true, nullptr, nullptr, nullptr);
-                        double *sc = S.scalars();
-                        launch_dot_active(S.g, pp.f.data(), pp.u.data(), S.red_partials(reduction_partials(S.g)),
-                                          sc + 10, st);
-                        comp[k] = S.read_scalar(sc + 10);
-                        launch_compliance_gradient(S.g, S.K0.data(), S.rho.data(), pp.u.data(),
-                                                   grads.data() + (size_t)k * ne, S.law, S.E0, S.Emin, S.gamma, S.q,
-                                                   S.grav, S.h[0] * S.h[1] * S.h[2], 0, st);
-                    }
-                    sync(st);
-                } catch (...) {
-                    errs[p] = std::current_exception();
-                }
-            });
-        for (auto &t : threads) t.join();
-        for (auto &e : errs)
-            if (e) std::rethrow_exception(e);
-    } catch (...) {
-        cleanup();
-        throw;
+        const int G = 1 << L, P = (int)sim->ne[0] + G;
+        // batch size: at most `pipelines` structures (0: no limit) and 2^25 stacked nodes (so that one FP64
+        // vector stays below 1 GiB, the limit of the 32-bit buffer offsets of the plane-marching kernels)
+        const int64_t maxNodes = (int64_t)1 << 25;
+        std::vector<double> comp(nl, 0.0);
+        std::vector<int64_t> its(nl, 0);
+        std::vector<std::shared_ptr<StackedBatch>> next;  // this evaluation's batches (the cache afterwards)
+        DBuf<double> gsum(ne);
+        launch_fill<double>(ne, gsum.data(), 0.0, s);
+        for (int k0 = 0; k0 < nl;) {
+            const int rowsNeed = heights[k0];
+            int rows = std::min<int>((int)sim->ne[1], (rowsNeed + G - 1) / G * G);
+            if (bcRow > rows) rows = (int)sim->ne[1];
+            const int64_t perSeg = (int64_t)P * (rows + 1) * (sim->ne[2] + 1);
+            int cap = (int)std::max<int64_t>(1, maxNodes / perSeg);
+            if (pipelines > 0) cap = std::min<int>(cap, (int)pipelines);
+            const int k1 = std::min(nl, k0 + cap);
+            std::vector<int> top(heights.begin() + k0, heights.begin() + k1);
+            lbl_stacked_batch(mg, next, top, rows, maxIter, tol, (int)mgIterations, (int)smoothing, fullMultigrid != 0,
+                              precision, useGraphs != 0, Efull.data(), E32full.data(), comp.data() + k0,
+                              its.data() + k0, gsum.data());
+            k0 = k1;
+        }
+        mg->stackCache = std::move(next);
+        double total = 0;
+        for (int k = 0; k < nl; ++k) {
+            total += comp[k];
+            if (perLayerCompliance) perLayerCompliance[k] = comp[k];
+            if (perLayerIterations) perLayerIterations[k] = its[k];
+        }
+        *objective = 0.5 * total / (double)nl;
+        std::vector<double> hg(ne);
+        d2h(hg.data(), gsum.data(), ne, s);
+        sync(s);
+        for (auto &v : hg) v /= (double)nl;
+        if (gradWhere == VF_DEVICE) { h2d(grad, hg.data(), ne, s); sync(s); }
+        else std::copy(hg.begin(), hg.end(), grad);
+        // the simulator ends, like the sequential evaluator, at the last (lowest) layer's mask
+        sim->set_mask_height(sim->h[1] * (double)heights.back(), true);
     }
-    cleanup();
-    // objective and gradient summed in layer order (the order of the sequential evaluator)
-    double total = 0;
-    for (int k = 0; k < nl; ++k) {
-        total += comp[k];
-        if (perLayerCompliance) perLayerCompliance[k] = comp[k];
-        if (perLayerIterations) perLayerIterations[k] = its[k];
-    }
-    *objective = 0.5 * total / (double)nl;
-    hipStream_t s = mg->st();
-    DBuf<double> g(ne);
-    launch_fill<double>(ne, g.data(), 0.0, s);
-    for (int k = 0; k < nl; ++k) launch_axpy_plain(ne, 1.0, grads.data() + (size_t)k * ne, g.data(), s);
-    std::vector<double> hg(ne);
-    d2h(hg.data(), g.data(), ne, s);
-    sync(s);
-    for (auto &v : hg) v /= (double)nl;
-    if (gradWhere == VF_DEVICE) { h2d(grad, hg.data(), ne, s); sync(s); }
-    else std::copy(hg.begin(), hg.end(), grad);
-    // the base simulator ends, like the sequential evaluator, at the last (lowest) layer's mask
-    sim->set_mask_height(sim->h[1] * (double)heights.back(), true);
     VF_API_END
 }
 

@@ -139,11 +139,15 @@ int plane_chunks(int tiles, int n, int want, int resident);
 // Element-centric fine K u in the Walsh basis (AM_SET or AM_RESIDUAL); C = the 8 x 16 table of the
 // nonzero Walsh-basis element stiffness entries divided by 64 (see vf_kernels_walsh.hip).
 // With dotOut (SET mode only), *dotOut = <u, y> is also formed from per-workgroup partials (dotPartials
-// needs one entry per workgroup, at most 65536).
+// needs one entry per workgroup, at most tiles x nn[0]); the partials are stored chunk-major (entry
+// chunk * tiles + tile, tiles = ceil(ydet / 15) * ceil(nn[2] / 15), chunk c owning node planes
+// [c xchunk, (c + 1) xchunk)). alignX > 0: xchunk divides alignX.
 template <typename T>
 void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
                              const uint8_t *dmask, int mode, hipStream_t s, double *dotPartials = nullptr,
-                             double *dotOut = nullptr);
+                             double *dotOut = nullptr, int alignX = 0);
+// x-chunk length the Walsh K u uses on g with alignX (as above)
+int walsh_xchunk(const Grid &g, int mode, int alignX);
 // fixed-order sum of count partials into *out
 void launch_final_sum(const double *partials, int count, double *out, hipStream_t s);
 // One colour of the fine block Gauss-Seidel sweep on the structured element matrix, register gather
@@ -275,6 +279,54 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
 template <typename T>
 void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes, const T *b,
                               T *x, double *tmp_in, double *tmp_out, hipStream_t s);
+// the same for nsys independent systems in one launch sequence: system z has n[z] (device) unknowns in nb
+// blocks, its element matrices at Ke + z keStride on the sub-grid g, its reduced map at red + z redStride;
+// band storage band_storage(nb * 64, kb) per system, Linv nb * 4096, X and XT (nb * 64)^2; flags[z] != 0 if
+// system z is not positive definite
+void band_cholesky_inverse_batched(const Grid &g, const double *Ke, size_t keStride, const int *red, size_t redStride,
+                                   int nsys, const int *n, int nb, int kb, double *B, double *Linv, double *X,
+                                   double *XT, int *flags, hipStream_t s);
+// x = X_z^T X_z b over every system's reduced unknowns (DoF i of system z: redToFull[z redStride + i] on the
+// nodes shifted by z nodeStride), zero elsewhere; tmp_in / tmp_out hold nsys * ld doubles
+template <typename T>
+void launch_coarse_chol_apply_batched(int nsys, const int *n, int ld, const double *X, const double *XT,
+                                      const int *redToFull, size_t redStride, int nnodes, size_t nodeStride, const T *b,
+                                      T *x, double *tmp_in, double *tmp_out, hipStream_t s);
+
+// Stacked layer-by-layer batches: segment k of the stacked grid spans the node planes [k P, (k + 1) P)
+// (the last one up to the end); per-segment scalars at sc + 16 k in the PCG layout; state[k]: 0 iterating,
+// 1 converged at the last check (deferred x update due), 2 done.
+int active_partials(const Grid &g);  // partial sums of one active-row reduction on g
+void launch_seg_sum_rows(const Grid &g, const double *partials, int nsets, size_t setStride, int P, int nseg,
+                         double *sc, int off, hipStream_t s);
+void launch_seg_sum_chunks(const double *partials, int tiles, int xchunk, int nchunks, int P, int nseg, double *sc,
+                           int off, hipStream_t s);
+void launch_seg_dot(const Grid &g, const double *a, const double *b, double *partials, int P, int nseg, double *sc,
+                    int off, hipStream_t s);
+template <typename TS>
+void launch_seg_dots_s(const Grid &g, const double *r, const TS *sv, const double *adp, double *partials, int P,
+                       int nseg, double *sc, hipStream_t s);
+void launch_seg_scalars(double *sc, const int *state, int nseg, int op, hipStream_t s);
+template <typename TS>
+void launch_seg_direction_x(const Grid &g, const TS *sv, double *d, double *x, const double *sc, const int *state,
+                            int P, int nseg, int mode, int first, hipStream_t s);
+void launch_seg_residual(const Grid &g, double *r, const double *Ad, double *sc, const int *state, int P, int nseg,
+                         double *partials, float *r32, hipStream_t s);
+void launch_seg_finish_x(const Grid &g, double *x, const double *d, const double *sc, const int *state, int P,
+                         int nseg, hipStream_t s);
+template <typename T>
+void launch_stack_elems(const Grid &gs, const Grid &go, int P, int nseg, const int *top, const T *src, T *dst,
+                        hipStream_t s);
+void launch_stack_nodes(const Grid &gs, const Grid &go, int P, int nseg, const int *top, const double *src,
+                        double *dst, hipStream_t s);
+void launch_fold_gradient(const Grid &go, const Grid &gs, int P, int nseg, const int *top, const double *gst,
+                          double *g, hipStream_t s);
+// mask[n] = 7 for the nodes of level grid g in the void columns (local x > Wl in segments of Pl planes) or at
+// or above their segment's first detached row ydet[k]
+void launch_stack_mask(const Grid &g, int Pl, int Wl, int nseg, const int *ydet, uint8_t *mask, hipStream_t s);
+// v = 0 on every node whose mask is 7
+template <typename T>
+void launch_zero_held(const Grid &g, const uint8_t *mask, T *v, hipStream_t s);
 
 // elementwise physics
 void launch_young(const Grid &g, const double *rho, double *E, float *E32, int law, double E0, double Emin,

@@ -27,8 +27,13 @@ __device__ __forceinline__ const double *bandblk(const double *B, int kb, int I,
 
 // ---- assembly of the lower band from the cached coarsest element matrices (one work-item per
 //      (row DoF, neighbour slot)); rows/cols of fixed or detached DoFs are absent (red[] < 0).
+// Batched forms (blockIdx.z = system): every array argument advances by its z stride per system.
 __global__ void __launch_bounds__(TPB) k_band_assemble(Grid g, const double *__restrict__ Ke, const int *__restrict__ red,
-                                                       int kb, double *__restrict__ B) {
+                                                       int kb, double *__restrict__ B, size_t zKe, size_t zRed,
+                                                       size_t zB) {
+    Ke += blockIdx.z * zKe;
+    red += blockIdx.z * zRed;
+    B += blockIdx.z * zB;
     const long t = (long)blockIdx.x * TPB + threadIdx.x;
     if (t >= (long)g.nnodes * 27 * 3) return;
     const int row = (int)(t % (3L * g.nnodes)), slot = (int)(t / (3L * g.nnodes));
@@ -67,6 +72,12 @@ __global__ void __launch_bounds__(TPB) k_band_assemble(Grid g, const double *__r
 __global__ void k_band_pad(int n, int nb, int kb, double *B) {
     const int i = n + threadIdx.x;
     if (i < nb * NB) bandblk(B, kb, nb - 1, 0)[(i % NB) * NB + i % NB] = 1.0;
+}
+// batched: system z holds n[z] unknowns in nb blocks; identity on every row from n[z] on (grid: nb x systems)
+__global__ void k_band_pad_b(const int *__restrict__ n, int nb, int kb, double *B, size_t zB) {
+    B += blockIdx.y * zB;
+    const int i = blockIdx.x * NB + threadIdx.x;
+    if (i >= n[blockIdx.y] && i < nb * NB) bandblk(B, kb, blockIdx.x, 0)[(i % NB) * NB + i % NB] = 1.0;
 }
 
 // ---- L_kk = chol(A_kk) and Linv_kk = L_kk^-1 by one workgroup of 256 work-items, four per row.
@@ -157,8 +168,10 @@ __device__ __forceinline__ bool potrf_block(int k, int kb, double *__restrict__ 
     return true;
 }
 __global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
-                                                     int *__restrict__ flag) {
-    if (!potrf_block(k, kb, B, Linv) && threadIdx.x == 0) atomicOr(flag, 1);
+                                                     int *__restrict__ flag, size_t zB, size_t zL) {
+    B += blockIdx.z * zB;
+    Linv += blockIdx.z * zL;
+    if (!potrf_block(k, kb, B, Linv) && threadIdx.x == 0) atomicOr(flag + blockIdx.z, 1);
 }
 
 // C (+)= alpha * P * Q^T for 64 x 64 row-major blocks, 256 work-items with 4 x 4 register tiles.
@@ -182,7 +195,9 @@ __device__ __forceinline__ void gemm_nt_64(const double (*sP)[NB + 1], const dou
 
 // ---- L_ik = A_ik Linv_kk^T for i = k+1 .. min(k+kb, nb-1)   (grid: one workgroup per block)
 __global__ void __launch_bounds__(TPB) k_trsm_panel(int k, int nb, int kb, double *__restrict__ B,
-                                                    const double *__restrict__ Linv) {
+                                                    const double *__restrict__ Linv, size_t zB, size_t zL) {
+    B += blockIdx.z * zB;
+    Linv += blockIdx.z * zL;
     __shared__ double sP[NB][NB + 1];
     __shared__ double sQ[NB][NB + 1];
     const int q = 1 + blockIdx.x, I = k + q;
@@ -206,7 +221,8 @@ __global__ void __launch_bounds__(TPB) k_trsm_panel(int k, int nb, int kb, doubl
 }
 
 // ---- A_IJ -= L_Ik L_Jk^T for k < J <= I <= min(k+kb, nb-1)  (grid: one workgroup per pair)
-__global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, double *__restrict__ B) {
+__global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, double *__restrict__ B, size_t zB) {
+    B += blockIdx.z * zB;
     __shared__ double sP[NB][NB + 1];
     __shared__ double sQ[NB][NB + 1];
     // decode pair index -> (a, b) with 1 <= b <= a <= m
@@ -243,7 +259,11 @@ __global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, doub
 typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int SW = 16;
 __global__ void __launch_bounds__(TPB) k_band_inverse(int nb, int kb, const double *__restrict__ B,
-                                                      const double *__restrict__ Linv, double *__restrict__ X) {
+                                                      const double *__restrict__ Linv, double *__restrict__ X,
+                                                      size_t zB, size_t zL, size_t zX) {
+    B += blockIdx.z * zB;
+    Linv += blockIdx.z * zL;
+    X += blockIdx.z * zX;
     __shared__ double sT[NB][SW + 1];
     const int J = blockIdx.x / (NB / SW), cs = (blockIdx.x % (NB / SW)) * SW;  // strip columns J*NB+cs ..
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
@@ -311,7 +331,10 @@ __global__ void __launch_bounds__(TPB) k_trmv_lower(int n, int ld, const double 
 
 // ---- XT = X^T for the 64 x 64 tiles on and below the diagonal of X (so XT is filled on and above
 //      its diagonal; the upper triangle of X is zero)
-__global__ void __launch_bounds__(TPB) k_transpose_lower(int ld, const double *__restrict__ X, double *__restrict__ XT) {
+__global__ void __launch_bounds__(TPB) k_transpose_lower(int ld, const double *__restrict__ X, double *__restrict__ XT,
+                                                         size_t zX) {
+    X += blockIdx.z * zX;
+    XT += blockIdx.z * zX;
     const int bi = blockIdx.y, bj = blockIdx.x;  // tile row / column of X
     if (bj > bi) return;
     __shared__ double tl[NB][NB + 1];
@@ -364,6 +387,58 @@ __global__ void __launch_bounds__(TPB) k_cfill(size_t n, T *x, T v) {
     if (i < n) x[i] = v;
 }
 
+// ---- batched coarse solve over independent systems (blockIdx.y = system z): system z owns n[z] reduced
+//      unknowns; its DoF i is component f % 3 of node f / 3 + z * nodeStride of the level vector (N nodes),
+//      f = redToFull[z * redStride + i]; its factors X, XT (leading dimension ld) start at z * ld * ld.
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_cgather_b(const int *__restrict__ n, const int *__restrict__ redToFull,
+                                                   size_t redStride, int N, size_t nodeStride, int ld,
+                                                   const T *__restrict__ b, double *__restrict__ out) {
+    const int z = blockIdx.y, i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= ld) return;
+    double v = 0.0;
+    if (i < n[z]) {
+        const int f = redToFull[z * redStride + i];
+        v = (double)b[(size_t)(f % 3) * N + f / 3 + z * nodeStride];
+    }
+    out[(size_t)z * ld + i] = v;
+}
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_cscatter_b(const int *__restrict__ n, const int *__restrict__ redToFull,
+                                                    size_t redStride, int N, size_t nodeStride, int ld,
+                                                    const double *__restrict__ in, T *__restrict__ x) {
+    const int z = blockIdx.y, i = blockIdx.x * TPB + threadIdx.x;
+    if (i >= n[z]) return;
+    const int f = redToFull[z * redStride + i];
+    x[(size_t)(f % 3) * N + f / 3 + z * nodeStride] = (T)in[(size_t)z * ld + i];
+}
+// y_z = X_z b_z (lower triangle) and x_z = XT_z y_z (upper triangle), one wavefront per row
+__global__ void __launch_bounds__(TPB) k_trmv_lower_b(const int *__restrict__ n, int ld, const double *__restrict__ X,
+                                                      const double *__restrict__ b, double *__restrict__ y) {
+    const int z = blockIdx.y, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    if (row >= n[z]) return;
+    const double *Xr = X + (size_t)z * ld * ld + (size_t)row * ld;
+    const double *bz = b + (size_t)z * ld;
+    double s = 0;
+    for (int j = lane; j <= row; j += 64) s += Xr[j] * bz[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) y[(size_t)z * ld + row] = s;
+}
+__global__ void __launch_bounds__(TPB) k_trmv_upper_b(const int *__restrict__ n, int ld, const double *__restrict__ XT,
+                                                      const double *__restrict__ y, double *__restrict__ x) {
+    const int z = blockIdx.y, lane = threadIdx.x & 63;
+    const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
+    const int nz = n[z];
+    if (row >= nz) return;
+    const double *Xr = XT + (size_t)z * ld * ld + (size_t)row * ld;
+    const double *yz = y + (size_t)z * ld;
+    double s = 0;
+    for (int j = row + lane; j < nz; j += 64) s += Xr[j] * yz[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (lane == 0) x[(size_t)z * ld + row] = s;
+}
+
 inline unsigned nblk(long count) { return (unsigned)((count + TPB - 1) / TPB); }
 
 }  // namespace
@@ -381,20 +456,63 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
     launch_fill<double>(band_storage(n, kb), B, 0.0, s);
     launch_fill<double>(ld * ld, X, 0.0, s);
     launch_fill<int>(1, flag, 0, s);
-    hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B);
+    hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B, 0, 0, 0);
     if ((int)ld > n) hipLaunchKernelGGL(k_band_pad, dim3(1), dim3(NB), 0, s, n, nb, kb, B);
     for (int k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(PTPB), 0, s, k, kb, B, Linv, flag);
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(PTPB), 0, s, k, kb, B, Linv, flag, 0, 0);
         const int m = std::min(kb, nb - 1 - k);
         if (m > 0) {
-            hipLaunchKernelGGL(k_trsm_panel, dim3(m), dim3(TPB), 0, s, k, nb, kb, B, Linv);
-            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B);
+            hipLaunchKernelGGL(k_trsm_panel, dim3(m), dim3(TPB), 0, s, k, nb, kb, B, Linv, 0, 0);
+            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B, 0);
         }
     }
-    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X);
-    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT);
+    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X, 0, 0, 0);
+    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT, 0);
     VF_HIP(hipGetLastError());
 }
+
+void band_cholesky_inverse_batched(const Grid &g, const double *Ke, size_t keStride, const int *red, size_t redStride,
+                                   int nsys, const int *n, int nb, int kb, double *B, double *Linv, double *X,
+                                   double *XT, int *flags, hipStream_t s) {
+    if (nsys < 1 || nsys > 65535 || nb < 1) throw std::runtime_error("batched band Cholesky: bad system count or size");
+    const size_t ld = (size_t)nb * NB, zB = band_storage(nb * NB, kb), zL = (size_t)nb * BB, zX = ld * ld;
+    launch_fill<double>(zB * nsys, B, 0.0, s);
+    launch_fill<double>(zX * nsys, X, 0.0, s);
+    launch_fill<int>(nsys, flags, 0, s);
+    hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81), 1, nsys), dim3(TPB), 0, s, g, Ke, red, kb, B,
+                       keStride, redStride, zB);
+    hipLaunchKernelGGL(k_band_pad_b, dim3(nb, nsys), dim3(NB), 0, s, n, nb, kb, B, zB);
+    for (int k = 0; k < nb; ++k) {
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, nsys), dim3(PTPB), 0, s, k, kb, B, Linv, flags, zB, zL);
+        const int m = std::min(kb, nb - 1 - k);
+        if (m > 0) {
+            hipLaunchKernelGGL(k_trsm_panel, dim3(m, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, Linv, zB, zL);
+            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, zB);
+        }
+    }
+    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW), 1, nsys), dim3(TPB), 0, s, nb, kb, B, Linv, X, zB, zL, zX);
+    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb, nsys), dim3(TPB), 0, s, (int)ld, X, XT, zX);
+    VF_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_coarse_chol_apply_batched(int nsys, const int *n, int ld, const double *X, const double *XT,
+                                      const int *redToFull, size_t redStride, int nnodes, size_t nodeStride, const T *b,
+                                      T *x, double *tmp_in, double *tmp_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_cfill<T>, dim3(nblk(3L * nnodes)), dim3(TPB), 0, s, (size_t)3 * nnodes, x, T(0));
+    const dim3 gv(nblk(ld), nsys), gr((ld + 3) / 4, nsys);
+    hipLaunchKernelGGL(k_cgather_b<T>, gv, dim3(TPB), 0, s, n, redToFull, redStride, nnodes, nodeStride, ld, b, tmp_in);
+    hipLaunchKernelGGL(k_trmv_lower_b, gr, dim3(TPB), 0, s, n, ld, X, tmp_in, tmp_out);
+    hipLaunchKernelGGL(k_trmv_upper_b, gr, dim3(TPB), 0, s, n, ld, XT, tmp_out, tmp_in);
+    hipLaunchKernelGGL(k_cscatter_b<T>, gv, dim3(TPB), 0, s, n, redToFull, redStride, nnodes, nodeStride, ld, tmp_in, x);
+    VF_HIP(hipGetLastError());
+}
+template void launch_coarse_chol_apply_batched<double>(int, const int *, int, const double *, const double *,
+                                                       const int *, size_t, int, size_t, const double *, double *,
+                                                       double *, double *, hipStream_t);
+template void launch_coarse_chol_apply_batched<float>(int, const int *, int, const double *, const double *,
+                                                      const int *, size_t, int, size_t, const float *, float *,
+                                                      double *, double *, hipStream_t);
 
 template <typename T>
 void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes,

@@ -477,6 +477,193 @@ __global__ void __launch_bounds__(TPB) k_oc_update(size_t n, const double *__res
     out[i] = r;
 }
 
+
+// ---- layer-by-layer batches as one stacked problem. The partial structures of a batch lie side by side
+// along x, segment k over the node planes [k P, k P + W] of the stacked grid (P = W + 2^L element columns,
+// W the design's width); between segments, 2^L - 1 planes of void elements with constrained nodes. Segment
+// states: 0 iterating, 1 converged at the last check (the deferred x update is still due), 2 done.
+constexpr int SEG_ACTIVE = 0, SEG_FINISHING = 1;
+__device__ __forceinline__ int seg_of(int ix, int P, int nseg) { return min(ix / P, nseg - 1); }
+
+// Fixed-order sums of partials per segment: grid.x = segment. Rows layout (Active kernels): partial
+// (c nn0 + ix) gx + bx for component c, node plane ix, block bx; a segment sums its planes' rows.
+// Chunk layout (Walsh K u): partial chunk * tiles + tile, chunk c over planes [c xc, c xc + xc), xc | P.
+// nsets sets, set j at partials + j setStride, written to sc[16 segment + off + j].
+__global__ void __launch_bounds__(TPB) k_seg_sum(const double *__restrict__ partials, int layout, int a0, int a1,
+                                                 int a2, int P, int nseg, int nsets, size_t setStride,
+                                                 double *__restrict__ sc, int off) {
+    __shared__ double sh[TPB / 64];
+    const int k = blockIdx.x;
+    for (int set = 0; set < nsets; ++set) {
+        const double *p = partials + set * setStride;
+        double v = 0;
+        if (layout == 0) {  // rows: a0 = gx, a1 = nn0
+            const int gx = a0, nn0 = a1;
+            const int ix0 = k * P, ix1 = k == nseg - 1 ? nn0 : min((k + 1) * P, nn0);
+            const int per = (ix1 - ix0) * gx, count = 3 * per;
+            for (int i = threadIdx.x; i < count; i += TPB) {
+                const int c = i / per, rem = i - c * per;
+                v += p[(size_t)(c * nn0 + ix0) * gx + rem];
+            }
+        } else {  // chunks: a0 = tiles, a1 = xc, a2 = number of chunks
+            const int tiles = a0, xc = a1, nch = a2;
+            const int c0 = k * P / xc, c1 = k == nseg - 1 ? nch : min((k + 1) * P / xc, nch);
+            const int count = (c1 - c0) * tiles;
+            for (int i = threadIdx.x; i < count; i += TPB) v += p[(size_t)c0 * tiles + i];
+        }
+        v = block_sum(v, sh);
+        if (threadIdx.x == 0) sc[16 * k + off + set] = v;
+        __syncthreads();
+    }
+}
+
+// op 0: previous <r,s> <- <r,s>; op 1: alpha = <r,s> / <d,Ad> (segments still iterating only)
+__global__ void k_seg_scalars(double *sc, const int *__restrict__ state, int nseg, int op) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nseg || state[k] != SEG_ACTIVE) return;
+    double *c = sc + 16 * k;
+    if (op == 0) c[2] = c[0];
+    else c[5] = c[0] / c[3];
+}
+
+// k_pcg_direction_x per segment; a segment that converged at the last check only takes its deferred
+// x += alpha d
+template <typename TS>
+__global__ void __launch_bounds__(TPB) k_seg_direction_x(Active a, int P, int nseg, const int *__restrict__ state,
+                                                         const TS *__restrict__ s, double *__restrict__ d,
+                                                         double *__restrict__ x, const double *__restrict__ scAll,
+                                                         int mode, int first) {
+    const int ix = blockIdx.y % a.nn0, seg = seg_of(ix, P, nseg), st = state[seg];
+    if (st > SEG_FINISHING) return;
+    const double *sc = scAll + 16 * seg;
+    const size_t base = active_base(a);
+    if (st == SEG_FINISHING) {
+        const double alpha = sc[5];
+        for (int k = 0; k < ITEMS; ++k) {
+            const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+            if (i < a.rowLen) x[base + i] += alpha * d[base + i];
+        }
+        return;
+    }
+    if (first) {
+        for (int k = 0; k < ITEMS; ++k) {
+            const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+            if (i < a.rowLen) d[base + i] = (double)s[base + i];
+        }
+        return;
+    }
+    const double alpha = sc[5];
+    const double beta = mode == 0 ? sc[0] / sc[2] : -sc[5] * sc[1] / sc[2];
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            const double dd = d[base + i];
+            x[base + i] += alpha * dd;
+            d[base + i] = (double)s[base + i] + beta * dd;
+        }
+    }
+}
+
+// k_pcg_residual per segment (iterating segments only; the others leave r and write zero partials)
+__global__ void __launch_bounds__(TPB) k_seg_residual(Active a, int P, int nseg, const int *__restrict__ state,
+                                                      double *__restrict__ r, const double *__restrict__ Ad,
+                                                      const double *__restrict__ scAll, double *__restrict__ partials,
+                                                      float *__restrict__ r32) {
+    __shared__ double sh[TPB / 64];
+    const int ix = blockIdx.y % a.nn0, seg = seg_of(ix, P, nseg);
+    if (state[seg] != SEG_ACTIVE) {  // uniform over the workgroup (one plane)
+        if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = 0.0;
+        return;
+    }
+    const double *sc = scAll + 16 * seg;
+    const double alpha = sc[0] / sc[3];
+    const size_t base = active_base(a);
+    double s = 0;
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) {
+            const double rn = r[base + i] - alpha * Ad[base + i];
+            r[base + i] = rn;
+            if (r32) r32[base + i] = (float)rn;
+            s += rn * rn;
+        }
+    }
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) partials[blockIdx.y * gridDim.x + blockIdx.x] = s;
+}
+
+// the deferred x += alpha d of the segments that converged at the last check
+__global__ void __launch_bounds__(TPB) k_seg_finish_x(Active a, int P, int nseg, const int *__restrict__ state,
+                                                      double *__restrict__ x, const double *__restrict__ d,
+                                                      const double *__restrict__ scAll) {
+    const int ix = blockIdx.y % a.nn0, seg = seg_of(ix, P, nseg);
+    if (state[seg] != SEG_FINISHING) return;
+    const double alpha = scAll[16 * seg + 5];
+    const size_t base = active_base(a);
+    for (int k = 0; k < ITEMS; ++k) {
+        const int i = (blockIdx.x * ITEMS + k) * TPB + threadIdx.x;
+        if (i < a.rowLen) x[base + i] += alpha * d[base + i];
+    }
+}
+
+// stacked element fields: segment k holds the design's elements (x - k P, y, z) below its first masked row
+// top[k]; void (zero) elsewhere
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_stack_elems(Grid gs, Grid go, int P, int nseg, const int *__restrict__ top,
+                                                     const T *__restrict__ src, T *__restrict__ dst) {
+    const int e = blockIdx.x * TPB + threadIdx.x;
+    if (e >= gs.nelems) return;
+    const int ez = e % gs.ne[2], ey = (e / gs.ne[2]) % gs.ne[1], ex = e / (gs.ne[1] * gs.ne[2]);
+    const int k = min(ex / P, nseg - 1), lx = ex - k * P;
+    T v = T(0);
+    if (lx < go.ne[0] && ey < top[k]) v = src[((size_t)lx * go.ne[1] + ey) * go.ne[2] + ez];
+    dst[e] = v;
+}
+// stacked node field (3 components): segment k holds the design's nodes (x - k P, y, z) up to its first
+// detached row top[k] + 1; zero in the gaps and on the detached rows
+__global__ void __launch_bounds__(TPB) k_stack_nodes(Grid gs, Grid go, int P, int nseg, const int *__restrict__ top,
+                                                     const double *__restrict__ src, double *__restrict__ dst) {
+    const int n = blockIdx.x * TPB + threadIdx.x;
+    if (n >= gs.nnodes) return;
+    const int nz = gs.nn[2], ix = n / (gs.nn[1] * nz), iy = (n / nz) % gs.nn[1], iz = n % nz;
+    const int k = min(ix / P, nseg - 1), lx = ix - k * P;
+    const bool in = lx < go.nn[0] && iy <= top[k];
+    const size_t m = ((size_t)lx * go.nn[1] + iy) * go.nn[2] + iz;
+    for (int c = 0; c < 3; ++c) dst[(size_t)c * gs.nnodes + n] = in ? src[(size_t)c * go.nnodes + m] : 0.0;
+}
+// per-node masks of one level of the stacked grid: every component of a node in the void columns (local x
+// beyond the segment width Wl) or on a row at or above its segment's first detached row ydet[k] is held
+// (mask 7); the design's own constraints are already in the mask
+__global__ void __launch_bounds__(TPB) k_stack_mask(Grid g, int Pl, int Wl, int nseg, const int *__restrict__ ydet,
+                                                    uint8_t *__restrict__ mask) {
+    const int n = blockIdx.x * TPB + threadIdx.x;
+    if (n >= g.nnodes) return;
+    const int nz = g.nn[2], ix = n / (g.nn[1] * nz), iy = (n / nz) % g.nn[1];
+    const int k = min(ix / Pl, nseg - 1), lx = ix - k * Pl;
+    if (lx > Wl || iy >= ydet[k]) mask[n] = 7;
+}
+// v = 0 on every node whose mask holds all three components
+template <typename T>
+__global__ void __launch_bounds__(TPB) k_zero_held(int N, const uint8_t *__restrict__ mask, T *__restrict__ v) {
+    const int n = blockIdx.x * TPB + threadIdx.x;
+    if (n >= N || mask[n] != 7) return;
+    v[n] = T(0);
+    v[N + n] = T(0);
+    v[2 * (size_t)N + n] = T(0);
+}
+// g[e] += the stacked gradient of each segment in turn (segment order = layer order), for the element
+// rows below the segment's first masked row (masked elements contribute nothing)
+__global__ void __launch_bounds__(TPB) k_fold_gradient(Grid go, Grid gs, int P, int nseg, const int *__restrict__ top,
+                                                       const double *__restrict__ gst, double *__restrict__ g) {
+    const int e = blockIdx.x * TPB + threadIdx.x;
+    if (e >= go.nelems) return;
+    const int ez = e % go.ne[2], ey = (e / go.ne[2]) % go.ne[1], ex = e / (go.ne[1] * go.ne[2]);
+    double acc = g[e];
+    for (int k = 0; k < nseg; ++k)
+        if (ey < top[k]) acc += gst[((size_t)(k * P + ex) * gs.ne[1] + ey) * gs.ne[2] + ez];
+    g[e] = acc;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ launchers
@@ -805,5 +992,105 @@ template void launch_copy_active<float, double>(const Grid &, int, const double 
 template void launch_copy_active<double, float>(const Grid &, int, const float *, double *, hipStream_t);
 template void launch_zero_dirichlet<double>(int, const int *, const uint8_t *, int, double *, hipStream_t);
 template void launch_zero_dirichlet<float>(int, const int *, const uint8_t *, int, float *, hipStream_t);
+
+// ---- stacked layer-by-layer batches
+void launch_seg_sum_rows(const Grid &g, const double *partials, int nsets, size_t setStride, int P, int nseg,
+                         double *sc, int off, hipStream_t s) {
+    const Active a = active_of(g, 0);
+    const dim3 gr = active_grid(a);
+    hipLaunchKernelGGL(k_seg_sum, dim3(nseg), dim3(TPB), 0, s, partials, 0, (int)gr.x, a.nn0, 0, P, nseg, nsets,
+                       setStride, sc, off);
+    VF_HIP(hipGetLastError());
+}
+void launch_seg_sum_chunks(const double *partials, int tiles, int xchunk, int nchunks, int P, int nseg, double *sc,
+                           int off, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_sum, dim3(nseg), dim3(TPB), 0, s, partials, 1, tiles, xchunk, nchunks, P, nseg, 1,
+                       (size_t)0, sc, off);
+    VF_HIP(hipGetLastError());
+}
+int active_partials(const Grid &g) {
+    const Active a = active_of(g, 0);
+    const dim3 gr = active_grid(a);
+    return (int)(gr.x * gr.y);
+}
+void launch_seg_dot(const Grid &g, const double *a, const double *b, double *partials, int P, int nseg, double *sc,
+                    int off, hipStream_t s) {
+    const Active ac = active_of(g, 0);
+    const dim3 gr = active_grid(ac);
+    hipLaunchKernelGGL(k_dot_active, gr, dim3(TPB), 0, s, ac, a, b, partials);
+    launch_seg_sum_rows(g, partials, 1, 0, P, nseg, sc, off, s);
+}
+template <typename TS>
+void launch_seg_dots_s(const Grid &g, const double *r, const TS *sv, const double *adp, double *partials, int P,
+                       int nseg, double *sc, hipStream_t s) {
+    const Active a = active_of(g, 0);
+    const dim3 gr = active_grid(a);
+    const int cnt = (int)(gr.x * gr.y);
+    hipLaunchKernelGGL(k_pcg_dots_s<TS>, gr, dim3(TPB), 0, s, a, r, sv, adp, partials, cnt);
+    launch_seg_sum_rows(g, partials, adp ? 2 : 1, (size_t)cnt, P, nseg, sc, 0, s);
+}
+void launch_seg_scalars(double *sc, const int *state, int nseg, int op, hipStream_t s) {
+    hipLaunchKernelGGL(k_seg_scalars, dim3((nseg + 63) / 64), dim3(64), 0, s, sc, state, nseg, op);
+    VF_HIP(hipGetLastError());
+}
+template <typename TS>
+void launch_seg_direction_x(const Grid &g, const TS *sv, double *d, double *x, const double *sc, const int *state,
+                            int P, int nseg, int mode, int first, hipStream_t s) {
+    const Active a = active_of(g, 0);
+    hipLaunchKernelGGL(k_seg_direction_x<TS>, active_grid(a), dim3(TPB), 0, s, a, P, nseg, state, sv, d, x, sc, mode,
+                       first);
+    VF_HIP(hipGetLastError());
+}
+void launch_seg_residual(const Grid &g, double *r, const double *Ad, double *sc, const int *state, int P, int nseg,
+                         double *partials, float *r32, hipStream_t s) {
+    const Active a = active_of(g, 0);
+    hipLaunchKernelGGL(k_seg_residual, active_grid(a), dim3(TPB), 0, s, a, P, nseg, state, r, Ad, sc, partials, r32);
+    launch_seg_sum_rows(g, partials, 1, 0, P, nseg, sc, 4, s);
+}
+void launch_seg_finish_x(const Grid &g, double *x, const double *d, const double *sc, const int *state, int P,
+                         int nseg, hipStream_t s) {
+    const Active a = active_of(g, 0);
+    hipLaunchKernelGGL(k_seg_finish_x, active_grid(a), dim3(TPB), 0, s, a, P, nseg, state, x, d, sc);
+    VF_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_stack_elems(const Grid &gs, const Grid &go, int P, int nseg, const int *top, const T *src, T *dst,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_stack_elems<T>, dim3(nblocks(gs.nelems)), dim3(TPB), 0, s, gs, go, P, nseg, top, src, dst);
+    VF_HIP(hipGetLastError());
+}
+void launch_stack_nodes(const Grid &gs, const Grid &go, int P, int nseg, const int *top, const double *src,
+                        double *dst, hipStream_t s) {
+    hipLaunchKernelGGL(k_stack_nodes, dim3(nblocks(gs.nnodes)), dim3(TPB), 0, s, gs, go, P, nseg, top, src, dst);
+    VF_HIP(hipGetLastError());
+}
+void launch_fold_gradient(const Grid &go, const Grid &gs, int P, int nseg, const int *top, const double *gst,
+                          double *g, hipStream_t s) {
+    hipLaunchKernelGGL(k_fold_gradient, dim3(nblocks(go.nelems)), dim3(TPB), 0, s, go, gs, P, nseg, top, gst, g);
+    VF_HIP(hipGetLastError());
+}
+void launch_stack_mask(const Grid &g, int Pl, int Wl, int nseg, const int *ydet, uint8_t *mask, hipStream_t s) {
+    hipLaunchKernelGGL(k_stack_mask, dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g, Pl, Wl, nseg, ydet, mask);
+    VF_HIP(hipGetLastError());
+}
+template <typename T>
+void launch_zero_held(const Grid &g, const uint8_t *mask, T *v, hipStream_t s) {
+    hipLaunchKernelGGL(k_zero_held<T>, dim3(nblocks(g.nnodes)), dim3(TPB), 0, s, g.nnodes, mask, v);
+    VF_HIP(hipGetLastError());
+}
+template void launch_zero_held<double>(const Grid &, const uint8_t *, double *, hipStream_t);
+template void launch_zero_held<float>(const Grid &, const uint8_t *, float *, hipStream_t);
+template void launch_seg_dots_s<double>(const Grid &, const double *, const double *, const double *, double *, int,
+                                        int, double *, hipStream_t);
+template void launch_seg_dots_s<float>(const Grid &, const double *, const float *, const double *, double *, int,
+                                       int, double *, hipStream_t);
+template void launch_seg_direction_x<double>(const Grid &, const double *, double *, double *, const double *,
+                                             const int *, int, int, int, int, hipStream_t);
+template void launch_seg_direction_x<float>(const Grid &, const float *, double *, double *, const double *,
+                                            const int *, int, int, int, int, hipStream_t);
+template void launch_stack_elems<double>(const Grid &, const Grid &, int, int, const int *, const double *, double *,
+                                         hipStream_t);
+template void launch_stack_elems<float>(const Grid &, const Grid &, int, int, const int *, const float *, float *,
+                                        hipStream_t);
 
 }  // namespace vf

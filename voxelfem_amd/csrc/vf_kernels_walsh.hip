@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(NTW, sizeof(T) == 8 ? 3 : 5) k_ku_walsh(Grid g
         if (t == 0) {
             double r = 0;
             for (int i = 0; i < NTW / 64; ++i) r += sred[i];
-            dotp[blockIdx.y * gridDim.x + blockIdx.x] = r;
+            dotp[logical] = r;  // chunk-major: the partials of one x-chunk are contiguous
         }
     }
 }
@@ -279,9 +279,27 @@ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
 
 }  // namespace
 
+static int walsh_xchunk_of(const Grid &g, const void *kern, int alignX) {
+    const int tiles = cdiv(g.ydet, TN) * cdiv(g.nn[2], TN);
+    const int chunks0 = plane_chunks(tiles, g.nn[0], 4 * cu_count_w(), resident_workgroups(kern, NTW));
+    int xchunk = cdiv(g.nn[0], chunks0);
+    if (alignX > 0) {  // the largest divisor of alignX not above xchunk: no x-chunk straddles a multiple of alignX
+        int best = 1;
+        for (int d = 1; d <= alignX && d <= xchunk; ++d)
+            if (alignX % d == 0) best = d;
+        xchunk = best;
+    }
+    return xchunk;
+}
+int walsh_xchunk(const Grid &g, int mode, int alignX) {
+    const void *kern = mode == AM_RESIDUAL ? (const void *)k_ku_walsh<double, AM_RESIDUAL> : (const void *)k_ku_walsh<double, AM_SET>;
+    return walsh_xchunk_of(g, kern, alignX);
+}
+
 template <typename T>
 void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, T *y, const T *b,
-                             const uint8_t *dmask, int mode, hipStream_t s, double *dotPartials, double *dotOut) {
+                             const uint8_t *dmask, int mode, hipStream_t s, double *dotPartials, double *dotOut,
+                             int alignX) {
     if (mode != AM_SET && mode != AM_RESIDUAL) throw std::runtime_error("Walsh K u supports the SET and RESIDUAL modes");
     if (dotOut && (mode != AM_SET || !dotPartials)) throw std::runtime_error("the fused <u, K u> needs SET mode and partials");
     if (dotOut && g.ydet <= 0) launch_fill<double>(1, dotOut, 0.0, s);
@@ -290,8 +308,8 @@ void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, 
         const int ntY = cdiv(g.ydet, TN), ntZ = cdiv(g.nn[2], TN);
         const int tiles = ntY * ntZ;
         const void *kern = mode == AM_RESIDUAL ? (const void *)k_ku_walsh<T, AM_RESIDUAL> : (const void *)k_ku_walsh<T, AM_SET>;
-        const int chunks = plane_chunks(tiles, g.nn[0], 4 * cu_count_w(), resident_workgroups(kern, NTW));
-        const int xchunk = cdiv(g.nn[0], chunks);
+        const int xchunk = walsh_xchunk_of(g, kern, alignX);
+        const int chunks = cdiv(g.nn[0], xchunk);
         if (chunks > 65535) throw std::runtime_error("grid too large for the Walsh K u");
         const dim3 grid(tiles, chunks);
         if (mode == AM_RESIDUAL)
@@ -310,8 +328,9 @@ void launch_fine_apply_walsh(const Grid &g, const T *C, const T *E, const T *u, 
 }
 
 template void launch_fine_apply_walsh<double>(const Grid &, const double *, const double *, const double *, double *,
-                                              const double *, const uint8_t *, int, hipStream_t, double *, double *);
+                                              const double *, const uint8_t *, int, hipStream_t, double *, double *,
+                                              int);
 template void launch_fine_apply_walsh<float>(const Grid &, const float *, const float *, const float *, float *,
-                                             const float *, const uint8_t *, int, hipStream_t, double *, double *);
+                                             const float *, const uint8_t *, int, hipStream_t, double *, double *, int);
 
 }  // namespace vf
