@@ -491,8 +491,9 @@ struct vf_mg {
         Grid gsub{};
         std::vector<int> n;
         std::vector<int> ydet;  // first detached coarsest row of each segment
+        const int *state = nullptr;  // device: segments in state 2 (done) are left out of the coarse solve
         int nb = 1, kb = 1, ld = 64;
-        DBuf<int> dn, red, redToFull, flags;
+        DBuf<int> dn, dnb, red, redToFull, flags;
         DBuf<double> B, Linv, X, XT, tin, tout;
         bool pending = false;
     };
@@ -914,7 +915,7 @@ struct vf_mg {
             const Grid &g = lv[L]->g;
             launch_coarse_chol_apply_batched<T>(S.nseg, S.dn.data(), S.ld, S.X.data(), S.XT.data(), S.redToFull.data(),
                                                 (size_t)S.ld, g.nnodes, (size_t)S.Pc * g.nn[1] * g.nn[2], b, x,
-                                                S.tin.data(), S.tout.data(), st());
+                                                S.tin.data(), S.tout.data(), st(), S.state);
             return;
         }
         ensure_factor();
@@ -1303,6 +1304,10 @@ struct vf_mg {
         h2d(S.red.data(), red.data(), red.size(), s);
         h2d(S.redToFull.data(), r2fAll.data(), r2fAll.size(), s);
         h2d(S.dn.data(), S.n.data(), S.n.size(), s);
+        std::vector<int> nbz(S.nseg);
+        for (int k = 0; k < S.nseg; ++k) nbz[k] = std::max(1, (S.n[k] + 63) / 64);
+        S.dnb.resize(S.nseg);
+        h2d(S.dnb.data(), nbz.data(), nbz.size(), s);
         S.B.resize((size_t)S.nseg * band_storage(S.ld, S.kb));
         S.Linv.resize((size_t)S.nseg * S.nb * 4096);
         S.X.resize((size_t)S.nseg * S.ld * S.ld);
@@ -1316,7 +1321,8 @@ struct vf_mg {
         SegCoarse &S = *seg;
         const Grid &g = lv[L]->g;
         band_cholesky_inverse_batched(S.gsub, lv[L]->Ke.data(), (size_t)S.Pc * g.ne[1] * g.ne[2] * 576, S.red.data(),
-                                      3 * (size_t)S.gsub.nnodes, S.nseg, S.dn.data(), S.nb, S.kb, S.B.data(),
+                                      3 * (size_t)S.gsub.nnodes, S.nseg, S.dn.data(), S.dnb.data(), S.nb, S.kb,
+                                      S.B.data(),
                                       S.Linv.data(), S.X.data(), S.XT.data(), S.flags.data(), s);
         S.pending = true;
     }
@@ -2857,12 +2863,12 @@ static void lbl_stacked_batch(vf_mg *base, std::vector<std::shared_ptr<StackedBa
     vf_sim *S = B.sim.get();
     refresh_stacked_sim(*S, *bs, P, B.dtop.data(), nseg, Efull, E32full);
     vf_mg &m = *B.mg;
+    m.seg->state = B.dstate.data();
     m.copy_coarsening(*base);
     m.symmetricGS = base->symmetricGS;
     m.useGraphs = useGraphs;
-    std::copy(base->kuChoice, base->kuChoice + 2, m.kuChoice);
-    std::copy(base->gsChoice, base->gsChoice + 2, m.gsChoice);
-    std::copy(&base->l1Tuned[0][0], &base->l1Tuned[0][0] + 4, &m.l1Tuned[0][0]);
+    // the batch's own kernel choices: its solver times the kernel families on the stacked grid at its first
+    // cycle in each precision (kept with the cached batch)
     m.sync_masks();
     const Grid &g = S->g;
     const size_t n0 = 3 * (size_t)g.nnodes;
@@ -3019,12 +3025,6 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
         for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) heights.push_back((int)l);
         const int nl = (int)heights.size();
         hipStream_t s = mg->st();
-        // the base solver's kernel choices serve every batch (tuned once, on the full design)
-        sim->set_mask_height(sim->h[1] * (double)numLayers, true);
-        mg->sync_masks();
-        mg->ensure_stiffness();
-        mg->tune<double>();
-        if (precision == VF_PREC_FP32) mg->tune<float>();
         // the design's moduli without any mask
         const size_t ne = sim->nel();
         DBuf<double> Efull(ne);
@@ -3045,26 +3045,41 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
                 if (hm[n]) bcRow = std::max(bcRow, (int)((n / gl.nn[2]) % gl.nn[1]) << l);
         }
         const int G = 1 << L, P = (int)sim->ne[0] + G;
-        // batch size: at most `pipelines` structures (0: no limit) and 2^25 stacked nodes (so that one FP64
-        // vector stays below 1 GiB, the limit of the 32-bit buffer offsets of the plane-marching kernels)
-        const int64_t maxNodes = (int64_t)1 << 25;
+        // Batches: consecutive structures (tallest first), at most `pipelines` of them (0: no limit) and 2^25
+        // stacked nodes (so that one FP64 vector stays below 1 GiB, the limit of the 32-bit buffer offsets of
+        // the plane-marching kernels). The split minimises the sum over batches of (2^20 + stacked nodes): a
+        // batch costs a fixed amount (setup, and per iteration the launch latency of ~100 kernels, measured
+        // worth about 2^20 nodes of work) plus its grid, which is as tall as its tallest structure.
+        const int64_t maxNodes = (int64_t)1 << 25, fixedNodes = (int64_t)1 << 20;
+        auto rows_of = [&](int h) {
+            int r = std::min<int>((int)sim->ne[1], (h + G - 1) / G * G);
+            if (bcRow > r) r = (int)sim->ne[1];
+            return r;
+        };
+        auto seg_nodes = [&](int rows) { return (int64_t)P * (rows + 1) * (sim->ne[2] + 1); };
+        std::vector<int64_t> best(nl + 1, INT64_MAX);
+        std::vector<int> cut(nl + 1, nl);
+        best[nl] = 0;
+        for (int i = nl - 1; i >= 0; --i) {
+            const int64_t per = seg_nodes(rows_of(heights[i]));
+            for (int j = i + 1; j <= nl; ++j) {
+                const int cnt = j - i;
+                if (cnt > 1 && (cnt * per > maxNodes || (pipelines > 0 && cnt > pipelines))) break;
+                const int64_t c = fixedNodes + cnt * per + best[j];
+                if (c < best[i]) { best[i] = c; cut[i] = j; }
+            }
+        }
         std::vector<double> comp(nl, 0.0);
         std::vector<int64_t> its(nl, 0);
         std::vector<std::shared_ptr<StackedBatch>> next;  // this evaluation's batches (the cache afterwards)
         DBuf<double> gsum(ne);
         launch_fill<double>(ne, gsum.data(), 0.0, s);
         for (int k0 = 0; k0 < nl;) {
-            const int rowsNeed = heights[k0];
-            int rows = std::min<int>((int)sim->ne[1], (rowsNeed + G - 1) / G * G);
-            if (bcRow > rows) rows = (int)sim->ne[1];
-            const int64_t perSeg = (int64_t)P * (rows + 1) * (sim->ne[2] + 1);
-            int cap = (int)std::max<int64_t>(1, maxNodes / perSeg);
-            if (pipelines > 0) cap = std::min<int>(cap, (int)pipelines);
-            const int k1 = std::min(nl, k0 + cap);
+            const int k1 = cut[k0];
             std::vector<int> top(heights.begin() + k0, heights.begin() + k1);
-            lbl_stacked_batch(mg, next, top, rows, maxIter, tol, (int)mgIterations, (int)smoothing, fullMultigrid != 0,
-                              precision, useGraphs != 0, Efull.data(), E32full.data(), comp.data() + k0,
-                              its.data() + k0, gsum.data());
+            lbl_stacked_batch(mg, next, top, rows_of(heights[k0]), maxIter, tol, (int)mgIterations, (int)smoothing,
+                              fullMultigrid != 0, precision, useGraphs != 0, Efull.data(), E32full.data(),
+                              comp.data() + k0, its.data() + k0, gsum.data());
             k0 = k1;
         }
         mg->stackCache = std::move(next);

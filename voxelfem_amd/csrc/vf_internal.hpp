@@ -279,19 +279,21 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
 template <typename T>
 void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes, const T *b,
                               T *x, double *tmp_in, double *tmp_out, hipStream_t s);
-// the same for nsys independent systems in one launch sequence: system z has n[z] (device) unknowns in nb
-// blocks, its element matrices at Ke + z keStride on the sub-grid g, its reduced map at red + z redStride;
-// band storage band_storage(nb * 64, kb) per system, Linv nb * 4096, X and XT (nb * 64)^2; flags[z] != 0 if
-// system z is not positive definite
+// the same for nsys independent systems in one launch sequence: system z has n[z] (device) unknowns in
+// nbz[z] = ceil(n[z] / 64) of the nb blocks (only those are factored and inverted), its element matrices at
+// Ke + z keStride on the sub-grid g, its reduced map at red + z redStride; band storage
+// band_storage(nb * 64, kb) per system, Linv nb * 4096, X and XT (nb * 64)^2 (leading dimension nb * 64);
+// flags[z] != 0 if system z is not positive definite
 void band_cholesky_inverse_batched(const Grid &g, const double *Ke, size_t keStride, const int *red, size_t redStride,
-                                   int nsys, const int *n, int nb, int kb, double *B, double *Linv, double *X,
-                                   double *XT, int *flags, hipStream_t s);
+                                   int nsys, const int *n, const int *nbz, int nb, int kb, double *B, double *Linv,
+                                   double *X, double *XT, int *flags, hipStream_t s);
 // x = X_z^T X_z b over every system's reduced unknowns (DoF i of system z: redToFull[z redStride + i] on the
-// nodes shifted by z nodeStride), zero elsewhere; tmp_in / tmp_out hold nsys * ld doubles
+// nodes shifted by z nodeStride), zero elsewhere; tmp_in / tmp_out hold nsys * ld doubles; skip (optional,
+// device): systems z with skip[z] == 2 are left out (zero)
 template <typename T>
 void launch_coarse_chol_apply_batched(int nsys, const int *n, int ld, const double *X, const double *XT,
                                       const int *redToFull, size_t redStride, int nnodes, size_t nodeStride, const T *b,
-                                      T *x, double *tmp_in, double *tmp_out, hipStream_t s);
+                                      T *x, double *tmp_in, double *tmp_out, hipStream_t s, const int *skip = nullptr);
 
 // Stacked layer-by-layer batches: segment k of the stacked grid spans the node planes [k P, (k + 1) P)
 // (the last one up to the end); per-segment scalars at sc + 16 k in the PCG layout; state[k]: 0 iterating,

@@ -167,8 +167,11 @@ __device__ __forceinline__ bool potrf_block(int k, int kb, double *__restrict__ 
     }
     return true;
 }
+// nbz (batched, optional): system z uses only its first nbz[z] blocks (the rest is padding nothing reads)
 __global__ void __launch_bounds__(PTPB) k_potrf_diag(int k, int kb, double *__restrict__ B, double *__restrict__ Linv,
-                                                     int *__restrict__ flag, size_t zB, size_t zL) {
+                                                     int *__restrict__ flag, size_t zB, size_t zL,
+                                                     const int *__restrict__ nbz) {
+    if (nbz && k >= nbz[blockIdx.z]) return;
     B += blockIdx.z * zB;
     Linv += blockIdx.z * zL;
     if (!potrf_block(k, kb, B, Linv) && threadIdx.x == 0) atomicOr(flag + blockIdx.z, 1);
@@ -195,7 +198,9 @@ __device__ __forceinline__ void gemm_nt_64(const double (*sP)[NB + 1], const dou
 
 // ---- L_ik = A_ik Linv_kk^T for i = k+1 .. min(k+kb, nb-1)   (grid: one workgroup per block)
 __global__ void __launch_bounds__(TPB) k_trsm_panel(int k, int nb, int kb, double *__restrict__ B,
-                                                    const double *__restrict__ Linv, size_t zB, size_t zL) {
+                                                    const double *__restrict__ Linv, size_t zB, size_t zL,
+                                                    const int *__restrict__ nbz) {
+    if (nbz) nb = min(nb, nbz[blockIdx.z]);
     B += blockIdx.z * zB;
     Linv += blockIdx.z * zL;
     __shared__ double sP[NB][NB + 1];
@@ -221,7 +226,9 @@ __global__ void __launch_bounds__(TPB) k_trsm_panel(int k, int nb, int kb, doubl
 }
 
 // ---- A_IJ -= L_Ik L_Jk^T for k < J <= I <= min(k+kb, nb-1)  (grid: one workgroup per pair)
-__global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, double *__restrict__ B, size_t zB) {
+__global__ void __launch_bounds__(TPB) k_band_update(int k, int nb, int kb, double *__restrict__ B, size_t zB,
+                                                     const int *__restrict__ nbz) {
+    if (nbz) nb = min(nb, nbz[blockIdx.z]);
     B += blockIdx.z * zB;
     __shared__ double sP[NB][NB + 1];
     __shared__ double sQ[NB][NB + 1];
@@ -260,14 +267,16 @@ typedef double dx4 __attribute__((ext_vector_type(4)));
 constexpr int SW = 16;
 __global__ void __launch_bounds__(TPB) k_band_inverse(int nb, int kb, const double *__restrict__ B,
                                                       const double *__restrict__ Linv, double *__restrict__ X,
-                                                      size_t zB, size_t zL, size_t zX) {
+                                                      size_t zB, size_t zL, size_t zX, const int *__restrict__ nbz) {
+    const size_t ld = (size_t)nb * NB;  // leading dimension of X (the padded size)
+    if (nbz) nb = min(nb, nbz[blockIdx.z]);
     B += blockIdx.z * zB;
     Linv += blockIdx.z * zL;
     X += blockIdx.z * zX;
     __shared__ double sT[NB][SW + 1];
     const int J = blockIdx.x / (NB / SW), cs = (blockIdx.x % (NB / SW)) * SW;  // strip columns J*NB+cs ..
+    if (J >= nb) return;
     const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
-    const size_t ld = (size_t)nb * NB;
     const int col0 = J * NB + cs;
     {
         const double *Lj = Linv + (size_t)J * BB;
@@ -332,11 +341,11 @@ __global__ void __launch_bounds__(TPB) k_trmv_lower(int n, int ld, const double 
 // ---- XT = X^T for the 64 x 64 tiles on and below the diagonal of X (so XT is filled on and above
 //      its diagonal; the upper triangle of X is zero)
 __global__ void __launch_bounds__(TPB) k_transpose_lower(int ld, const double *__restrict__ X, double *__restrict__ XT,
-                                                         size_t zX) {
+                                                         size_t zX, const int *__restrict__ nbz) {
     X += blockIdx.z * zX;
     XT += blockIdx.z * zX;
     const int bi = blockIdx.y, bj = blockIdx.x;  // tile row / column of X
-    if (bj > bi) return;
+    if (bj > bi || (nbz && bi >= nbz[blockIdx.z])) return;
     __shared__ double tl[NB][NB + 1];
     const int t = threadIdx.x;
     for (int w = t; w < NB * NB; w += TPB) {
@@ -390,12 +399,14 @@ __global__ void __launch_bounds__(TPB) k_cfill(size_t n, T *x, T v) {
 // ---- batched coarse solve over independent systems (blockIdx.y = system z): system z owns n[z] reduced
 //      unknowns; its DoF i is component f % 3 of node f / 3 + z * nodeStride of the level vector (N nodes),
 //      f = redToFull[z * redStride + i]; its factors X, XT (leading dimension ld) start at z * ld * ld.
+// skip (optional): systems with skip[z] == 2 are left out (their x stays zero)
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_cgather_b(const int *__restrict__ n, const int *__restrict__ redToFull,
                                                    size_t redStride, int N, size_t nodeStride, int ld,
-                                                   const T *__restrict__ b, double *__restrict__ out) {
+                                                   const T *__restrict__ b, double *__restrict__ out,
+                                                   const int *__restrict__ skip) {
     const int z = blockIdx.y, i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= ld) return;
+    if (i >= ld || (skip && skip[z] == 2)) return;
     double v = 0.0;
     if (i < n[z]) {
         const int f = redToFull[z * redStride + i];
@@ -406,18 +417,20 @@ __global__ void __launch_bounds__(TPB) k_cgather_b(const int *__restrict__ n, co
 template <typename T>
 __global__ void __launch_bounds__(TPB) k_cscatter_b(const int *__restrict__ n, const int *__restrict__ redToFull,
                                                     size_t redStride, int N, size_t nodeStride, int ld,
-                                                    const double *__restrict__ in, T *__restrict__ x) {
+                                                    const double *__restrict__ in, T *__restrict__ x,
+                                                    const int *__restrict__ skip) {
     const int z = blockIdx.y, i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= n[z]) return;
+    if (i >= n[z] || (skip && skip[z] == 2)) return;
     const int f = redToFull[z * redStride + i];
     x[(size_t)(f % 3) * N + f / 3 + z * nodeStride] = (T)in[(size_t)z * ld + i];
 }
 // y_z = X_z b_z (lower triangle) and x_z = XT_z y_z (upper triangle), one wavefront per row
 __global__ void __launch_bounds__(TPB) k_trmv_lower_b(const int *__restrict__ n, int ld, const double *__restrict__ X,
-                                                      const double *__restrict__ b, double *__restrict__ y) {
+                                                      const double *__restrict__ b, double *__restrict__ y,
+                                                      const int *__restrict__ skip) {
     const int z = blockIdx.y, lane = threadIdx.x & 63;
     const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
-    if (row >= n[z]) return;
+    if (row >= n[z] || (skip && skip[z] == 2)) return;
     const double *Xr = X + (size_t)z * ld * ld + (size_t)row * ld;
     const double *bz = b + (size_t)z * ld;
     double s = 0;
@@ -426,11 +439,12 @@ __global__ void __launch_bounds__(TPB) k_trmv_lower_b(const int *__restrict__ n,
     if (lane == 0) y[(size_t)z * ld + row] = s;
 }
 __global__ void __launch_bounds__(TPB) k_trmv_upper_b(const int *__restrict__ n, int ld, const double *__restrict__ XT,
-                                                      const double *__restrict__ y, double *__restrict__ x) {
+                                                      const double *__restrict__ y, double *__restrict__ x,
+                                                      const int *__restrict__ skip) {
     const int z = blockIdx.y, lane = threadIdx.x & 63;
     const int row = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);
     const int nz = n[z];
-    if (row >= nz) return;
+    if (row >= nz || (skip && skip[z] == 2)) return;
     const double *Xr = XT + (size_t)z * ld * ld + (size_t)row * ld;
     const double *yz = y + (size_t)z * ld;
     double s = 0;
@@ -459,21 +473,21 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
     hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B, 0, 0, 0);
     if ((int)ld > n) hipLaunchKernelGGL(k_band_pad, dim3(1), dim3(NB), 0, s, n, nb, kb, B);
     for (int k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(PTPB), 0, s, k, kb, B, Linv, flag, 0, 0);
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(PTPB), 0, s, k, kb, B, Linv, flag, 0, 0, nullptr);
         const int m = std::min(kb, nb - 1 - k);
         if (m > 0) {
-            hipLaunchKernelGGL(k_trsm_panel, dim3(m), dim3(TPB), 0, s, k, nb, kb, B, Linv, 0, 0);
-            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B, 0);
+            hipLaunchKernelGGL(k_trsm_panel, dim3(m), dim3(TPB), 0, s, k, nb, kb, B, Linv, 0, 0, nullptr);
+            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B, 0, nullptr);
         }
     }
-    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X, 0, 0, 0);
-    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT, 0);
+    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X, 0, 0, 0, nullptr);
+    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT, 0, nullptr);
     VF_HIP(hipGetLastError());
 }
 
 void band_cholesky_inverse_batched(const Grid &g, const double *Ke, size_t keStride, const int *red, size_t redStride,
-                                   int nsys, const int *n, int nb, int kb, double *B, double *Linv, double *X,
-                                   double *XT, int *flags, hipStream_t s) {
+                                   int nsys, const int *n, const int *nbz, int nb, int kb, double *B, double *Linv,
+                                   double *X, double *XT, int *flags, hipStream_t s) {
     if (nsys < 1 || nsys > 65535 || nb < 1) throw std::runtime_error("batched band Cholesky: bad system count or size");
     const size_t ld = (size_t)nb * NB, zB = band_storage(nb * NB, kb), zL = (size_t)nb * BB, zX = ld * ld;
     launch_fill<double>(zB * nsys, B, 0.0, s);
@@ -483,36 +497,39 @@ void band_cholesky_inverse_batched(const Grid &g, const double *Ke, size_t keStr
                        keStride, redStride, zB);
     hipLaunchKernelGGL(k_band_pad_b, dim3(nb, nsys), dim3(NB), 0, s, n, nb, kb, B, zB);
     for (int k = 0; k < nb; ++k) {
-        hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, nsys), dim3(PTPB), 0, s, k, kb, B, Linv, flags, zB, zL);
+        hipLaunchKernelGGL(k_potrf_diag, dim3(1, 1, nsys), dim3(PTPB), 0, s, k, kb, B, Linv, flags, zB, zL, nbz);
         const int m = std::min(kb, nb - 1 - k);
         if (m > 0) {
-            hipLaunchKernelGGL(k_trsm_panel, dim3(m, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, Linv, zB, zL);
-            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, zB);
+            hipLaunchKernelGGL(k_trsm_panel, dim3(m, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, Linv, zB, zL, nbz);
+            hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, zB, nbz);
         }
     }
-    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW), 1, nsys), dim3(TPB), 0, s, nb, kb, B, Linv, X, zB, zL, zX);
-    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb, nsys), dim3(TPB), 0, s, (int)ld, X, XT, zX);
+    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW), 1, nsys), dim3(TPB), 0, s, nb, kb, B, Linv, X, zB, zL, zX,
+                       nbz);
+    hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb, nsys), dim3(TPB), 0, s, (int)ld, X, XT, zX, nbz);
     VF_HIP(hipGetLastError());
 }
 
 template <typename T>
 void launch_coarse_chol_apply_batched(int nsys, const int *n, int ld, const double *X, const double *XT,
                                       const int *redToFull, size_t redStride, int nnodes, size_t nodeStride, const T *b,
-                                      T *x, double *tmp_in, double *tmp_out, hipStream_t s) {
+                                      T *x, double *tmp_in, double *tmp_out, hipStream_t s, const int *skip) {
     hipLaunchKernelGGL(k_cfill<T>, dim3(nblk(3L * nnodes)), dim3(TPB), 0, s, (size_t)3 * nnodes, x, T(0));
     const dim3 gv(nblk(ld), nsys), gr((ld + 3) / 4, nsys);
-    hipLaunchKernelGGL(k_cgather_b<T>, gv, dim3(TPB), 0, s, n, redToFull, redStride, nnodes, nodeStride, ld, b, tmp_in);
-    hipLaunchKernelGGL(k_trmv_lower_b, gr, dim3(TPB), 0, s, n, ld, X, tmp_in, tmp_out);
-    hipLaunchKernelGGL(k_trmv_upper_b, gr, dim3(TPB), 0, s, n, ld, XT, tmp_out, tmp_in);
-    hipLaunchKernelGGL(k_cscatter_b<T>, gv, dim3(TPB), 0, s, n, redToFull, redStride, nnodes, nodeStride, ld, tmp_in, x);
+    hipLaunchKernelGGL(k_cgather_b<T>, gv, dim3(TPB), 0, s, n, redToFull, redStride, nnodes, nodeStride, ld, b, tmp_in,
+                       skip);
+    hipLaunchKernelGGL(k_trmv_lower_b, gr, dim3(TPB), 0, s, n, ld, X, tmp_in, tmp_out, skip);
+    hipLaunchKernelGGL(k_trmv_upper_b, gr, dim3(TPB), 0, s, n, ld, XT, tmp_out, tmp_in, skip);
+    hipLaunchKernelGGL(k_cscatter_b<T>, gv, dim3(TPB), 0, s, n, redToFull, redStride, nnodes, nodeStride, ld, tmp_in, x,
+                       skip);
     VF_HIP(hipGetLastError());
 }
 template void launch_coarse_chol_apply_batched<double>(int, const int *, int, const double *, const double *,
                                                        const int *, size_t, int, size_t, const double *, double *,
-                                                       double *, double *, hipStream_t);
+                                                       double *, double *, hipStream_t, const int *);
 template void launch_coarse_chol_apply_batched<float>(int, const int *, int, const double *, const double *,
                                                       const int *, size_t, int, size_t, const float *, float *,
-                                                      double *, double *, hipStream_t);
+                                                      double *, double *, hipStream_t, const int *);
 
 template <typename T>
 void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes,
