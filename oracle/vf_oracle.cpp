@@ -825,6 +825,8 @@ struct MG {
     }
     // applyPreconditionerInv (MultigridSolver.hh:577-580) -> result in x[0]
     void precondition(const double *res, long mgIter, long nu, bool fmg) {
+        // the hierarchy must have been built (updateStiffness), as before the reference's applyPreconditionerInv
+        if (L > 0 && nu > 0 && sim(L).Ke.empty()) throw std::runtime_error("precondition before update_stiffness");
         Sim &s = sim(0);
         std::fill(x[0].begin(), x[0].end(), 0.0);
         if (b[0].data() != res) std::copy(res, res + 3 * s.nnodes, b[0].begin());

@@ -273,3 +273,40 @@ def test_lbl_batched_on_reused_nan_memory(vf, prec):
     ev.run(mg, True, 1, 500, 1e-10)
     assert np.isfinite(ev.objective())
     assert ev.objective() == pytest.approx(obj_seq, rel=1e-13)
+
+
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_zero_start_vcycle_masked_layers_on_nan_memory(vf, prec, tol):
+    """The V-cycle from a zero fine iterate with the two-launch smoother (variant 2) clears only the detached
+    rows of the iterate and lets its first sweep read the rest as zero without loading it. On a structure with
+    masked top layers, and with the device memory the new solver allocates first filled with NaN and handed
+    back to the runtime, the cycle is finite and equals the register-gather family's (variant 1) and, in
+    FP64, the oracle's."""
+    import torch
+
+    pv = vf.pyVoxelFEM
+    ne = (16, 24, 40)
+    rho = wl.noise_density(ne, 8)
+    tps = wl.configure(pv, ne, wl.CANTILEVER_BC, rho)
+    layer = 17
+    tps.setFabricationMaskHeightByLayer(layer)
+    junk = torch.full((1 << 26,), float("nan"), dtype=torch.float64, device="cuda:0")  # 512 MiB
+    torch.cuda.synchronize()
+    del junk
+    torch.cuda.empty_cache()
+    mg = tps.multigridSolver(3)
+    r = RNG.normal(size=(tps.numNodes(), 3))
+    s = orc.Sim(ne)
+    s.set_etensor(1.0, 0.3)
+    s.set_law(0, 1.0, 1e-9, 3.0, 3.0)
+    s.set_densities(rho)
+    s.set_dirichlet(wl.cantilever_mask(ne))
+    om = orc.MG(s, 3)
+    om.set_mask_layer(layer)
+    om.update_stiffness()
+    for nu in (1, 2):
+        a, b = both(lambda: mg._precondition(r, 1, nu, False, prec), 2)
+        assert np.isfinite(a).all() and np.isfinite(b).all(), nu
+        assert rel(a, b) < tol, nu
+        if prec == "fp64":
+            assert rel(a, om.precondition(r, 1, nu)) < 1e-12, nu
