@@ -325,6 +325,65 @@ __global__ void __launch_bounds__(TPB) k_band_inverse(int nb, int kb, const doub
     }
 }
 
+// ---- block row I of X = L^-1 for every block column J <= I at once (batched systems): the same products as
+//      one step of k_band_inverse, so X is identical. Row by row, each band block L_{I,I-q} is read by the
+//      strips of one launch while it is hot in the caches, instead of once per strip and row as in the
+//      column-walking form, whose re-reads of the band dominate when nb is large (grid: (I + 1) * 4 strips,
+//      systems; a system with nbz[z] <= I is done).
+__global__ void __launch_bounds__(TPB) k_band_inverse_row(int I, int nb, int kb, const double *__restrict__ B,
+                                                          const double *__restrict__ Linv, double *__restrict__ X,
+                                                          size_t zB, size_t zL, size_t zX,
+                                                          const int *__restrict__ nbz) {
+    const size_t ld = (size_t)nb * NB;
+    if (nbz && I >= nbz[blockIdx.z]) return;
+    B += blockIdx.z * zB;
+    Linv += blockIdx.z * zL;
+    X += blockIdx.z * zX;
+    __shared__ double sT[NB][SW + 1];
+    const int J = blockIdx.x / (NB / SW), cs = (blockIdx.x % (NB / SW)) * SW;
+    const int t = threadIdx.x, w = t >> 6, l = t & 63, li = l & 15, lk = l >> 4;
+    const int col0 = J * NB + cs;
+    if (J == I) {
+        const double *Lj = Linv + (size_t)J * BB;
+        for (int e = t; e < NB * SW; e += TPB) {
+            const int r = e / SW, c = e % SW;
+            X[(size_t)(J * NB + r) * ld + col0 + c] = Lj[r * NB + cs + c];
+        }
+        return;
+    }
+    const int rowA = 16 * w + li;
+    dx4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int qmax = min(kb, I - J);
+    for (int q = 1; q <= qmax; ++q) {
+        const double *Lb = bandblk(B, kb, I, q) + rowA * NB + lk;
+        const double *Xb = X + ((size_t)(I - q) * NB + lk) * ld + col0 + li;
+        double a[16], b[16];
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            a[s] = Lb[4 * s];
+            b[s] = Xb[(size_t)(4 * s) * ld];
+        }
+#pragma unroll
+        for (int s = 0; s < 16; ++s) acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) sT[16 * w + 4 * v + lk][li] = acc[v];
+    __syncthreads();
+    const double *Li = Linv + (size_t)I * BB + rowA * NB + lk;
+    double a[16], b[16];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+        a[s] = Li[4 * s];
+        b[s] = sT[4 * s + lk][li];
+    }
+    dx4 out = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < 16; ++s) out = __builtin_amdgcn_mfma_f64_16x16x4f64(a[s], b[s], out, 0, 0, 0);
+    double *Xi = X + ((size_t)I * NB + 16 * w + lk) * ld + col0 + li;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) Xi[(size_t)(4 * v) * ld] = -out[v];
+}
+
 // ---- y = X b over the lower triangle (one wavefront per row)
 __global__ void __launch_bounds__(TPB) k_trmv_lower(int n, int ld, const double *__restrict__ X,
                                                     const double *__restrict__ b, double *__restrict__ y) {
@@ -480,7 +539,14 @@ void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int 
             hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2), dim3(TPB), 0, s, k, nb, kb, B, 0, nullptr);
         }
     }
-    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X, 0, 0, 0, nullptr);
+    // small systems: one launch whose strips walk down their block columns; large ones (where re-reading the
+    // band per strip and row dominates): one launch per block row
+    if (nb <= 32)
+        hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW)), dim3(TPB), 0, s, nb, kb, B, Linv, X, 0, 0, 0, nullptr);
+    else
+        for (int I = 0; I < nb; ++I)
+            hipLaunchKernelGGL(k_band_inverse_row, dim3((I + 1) * (NB / SW)), dim3(TPB), 0, s, I, nb, kb, B, Linv, X, 0,
+                               0, 0, nullptr);
     hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb), dim3(TPB), 0, s, (int)ld, X, XT, 0, nullptr);
     VF_HIP(hipGetLastError());
 }
@@ -504,8 +570,9 @@ void band_cholesky_inverse_batched(const Grid &g, const double *Ke, size_t keStr
             hipLaunchKernelGGL(k_band_update, dim3(m * (m + 1) / 2, 1, nsys), dim3(TPB), 0, s, k, nb, kb, B, zB, nbz);
         }
     }
-    hipLaunchKernelGGL(k_band_inverse, dim3(nb * (NB / SW), 1, nsys), dim3(TPB), 0, s, nb, kb, B, Linv, X, zB, zL, zX,
-                       nbz);
+    for (int I = 0; I < nb; ++I)
+        hipLaunchKernelGGL(k_band_inverse_row, dim3((I + 1) * (NB / SW), 1, nsys), dim3(TPB), 0, s, I, nb, kb, B, Linv,
+                           X, zB, zL, zX, nbz);
     hipLaunchKernelGGL(k_transpose_lower, dim3(nb, nb, nsys), dim3(TPB), 0, s, (int)ld, X, XT, zX, nbz);
     VF_HIP(hipGetLastError());
 }
