@@ -729,14 +729,18 @@ struct vf_mg {
     }
     template <typename T>
     bool tiled_gs() const { return gs_family<T>() == 1; }
-    // Autotuner: time each applicable kernel family on this grid (HIP events on the solver's stream,
-    // best of two runs after a warm-up) and keep the fastest per operation and precision.
+    // Autotuner: time each applicable kernel family on this grid (HIP events on the solver's stream, best of
+    // three runs after a warm-up, with nothing else running on the device) and keep the fastest per operation
+    // and precision.
     template <typename T>
     void tune() {
         ensure_blocks<T>(true);
         const int p = pidx<T>();
         if (g_kernel_variant != 0 || !sim->structured || (kuChoice[p] >= 0 && gsChoice[p] >= 0)) return;
         wait_hier();
+        // nothing else may run meanwhile: the coarsest factorization (its own stream) would compete for the CUs
+        finish_factor();
+        sync(st());
         const Grid &g = lv[0]->g;
         const size_t n = 3 * (size_t)g.nnodes;
         hipStream_t s = st();
@@ -756,7 +760,7 @@ struct vf_mg {
         auto best = [&](auto &&fn) {
             float ms = 1e30f;
             fn();
-            for (int r = 0; r < 2; ++r) {
+            for (int r = 0; r < 3; ++r) {
                 VF_HIP(hipEventRecord(a, s));
                 fn();
                 VF_HIP(hipEventRecord(b, s));
