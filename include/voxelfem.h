@@ -45,8 +45,8 @@ extern "C" {
 #define VF_PREC_FP64 0
 #define VF_PREC_FP32 1
 
-typedef struct vf_sim vf_sim;  /* TensorProductSimulator<double,1,1,1> */
-typedef struct vf_mg vf_mg;    /* MultigridSolver<double,1,1,1> */
+typedef struct vf_sim vf_sim;  /* TensorProductSimulator<double,1,1,1> (or <double,1,1>, vf_sim_create_2d) */
+typedef struct vf_mg vf_mg;    /* MultigridSolver of the same instantiation */
 
 /* ---- library -------------------------------------------------------------------------- */
 const char *vf_last_error(void);
@@ -70,6 +70,19 @@ int vf_stream_destroy(void *stream);
 /* ---- simulator: TensorProductSimulator (TensorProductSimulator.hh:209-285, factory
  *      python_bindings/VoxelFEM.cc:423-430). stream: a hipStream_t or NULL for the null stream. */
 int vf_sim_create(vf_sim **out, const int64_t ne[3], const double minCorner[3], const double maxCorner[3], void *stream);
+/* the planar instantiation TensorProductSimulator<double,1,1> (python_bindings/VoxelFEM.cc:301-308, factory
+ * 198-212): bilinear quads on ne[0] x ne[1] elements, plane-stress isotropic tensor. Every vf_sim_* / vf_mg_* /
+ * vf_lbl_* entry point accepts a planar simulator (and a hierarchy built on it) with these differences: nodal
+ * fields are SoA 2*nn (x then y plane), node n = ix * (ne[1] + 1) + iy; Dirichlet masks use bits 0..1; K0 is
+ * 8x8 (64 doubles); gravity and stretchings use their first two entries (get_stretchings sets h[2] = 0);
+ * constant_strain_load takes the 2x2 strain in eps[0..3]; sample_nodal_field takes np x 2 points;
+ * level_nodes reports nn[2] = 1; the V-cycle runs in FP64 only (VF_PREC_FP32 is refused), the batched
+ * layer-by-layer entry solves the layers one after the other, and the autotuner and kernel benchmarks
+ * (3D kernel families) are refused. */
+int vf_sim_create_2d(vf_sim **out, const int64_t ne[2], const double minCorner[2], const double maxCorner[2],
+                     void *stream);
+/* 2 for a planar simulator, 3 otherwise (the Degrees... of the instantiation) */
+int vf_sim_dimension(vf_sim *sim, int *dim);
 int vf_sim_destroy(vf_sim *sim);
 int vf_sim_set_stream(vf_sim *sim, void *stream);
 /* ETensor of an isotropic material (setETensor, TensorProductSimulator.hh:333-347) */
@@ -253,6 +266,10 @@ int vf_lbl_run_zero_batched(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, 
 int vf_smoothing_filter(const int64_t dims[3], double rmin, int type, const double *in, double *out, int where,
                         void *stream);
 /* ProjectionFilter::apply / backprop (TopologyOptimizationFilter.hh:199-225) */
+/* SmoothingFilter::apply on a planar grid of dims[0] x dims[1] cells (the same filter at N = 2: 2D offsets,
+ * Linear weight rmin - |(a, b)|) */
+int vf_smoothing_filter_2d(const int64_t dims[2], double rmin, int type, const double *in, double *out, int where,
+                           void *stream);
 int vf_projection(double beta, int64_t n, const double *in, double *out, int where, void *stream);
 int vf_projection_backprop(double beta, int64_t n, const double *g, const double *vars, double *out, int where,
                            void *stream);

@@ -15,6 +15,12 @@ pyVoxelFEM API, on the device.
       sequential evaluator with the subspace initial guess (N=3).
   c4  384x192x192 cantilever, OC loop (filter radius 3, vf=0.3, m=0.2, p=0.5) with the FP32 V-cycle in
       the FP64 PCG, the FP64-only loop beside it, and the FP64 gradient at every mixed iterate.
+
+Planar ([1, 1]) workloads, the settings of the reference's 2D demo notebooks:
+  p0  384x192 cantilever on [0,2]x[0,1] (load at the lower right corner), vf=0.33, smoothing radius 3 (Linear) +
+      projection beta=1, optMGLevel levels, OC: 50 iterations, then beta=8 and SIMP exponent 5 for 50 more.
+  p1  256x128 MBB beam on [0,2]x[0,1], layer-by-layer evaluation (one layer per step, clamped base,
+      self-weight) with the subspace guess N=3 and tol 1e-5, and with the zero guess.
 """
 import argparse
 import json
@@ -236,13 +242,71 @@ def c4(iters=50):
     return out
 
 
+def p0(iters=50):
+    pv = _pv()
+    ne = (384, 192)
+    n = ne[0] * ne[1]
+    vf = 0.33
+    levels = optMGLevel(ne, 2)
+    tps = pv.TensorProductSimulator([1, 1], [[0.0, 0.0], [2.0, 1.0]], list(ne))
+    d = wl.write_json(wl.MATERIAL, name="material.json")
+    tps.readMaterial(d)
+    tps.applyDisplacementsAndLoadsFromFile(wl.write_json(wl.CANTILEVER2D_BC, os.path.dirname(d)))
+    obj = pv.MultigridComplianceObjective(tps.multigridSolver(levels))
+    sf = pv.SmoothingFilter(3, pv.SmoothingFilter.Type.Linear)
+    pf = pv.ProjectionFilter(1.0)
+    top = pv.TopologyOptimizationProblem(tps, obj, [pv.TotalVolumeConstraint(vf)], [sf, pf])
+    top.setVars(np.full(n, pf.invert(vf)))
+    oc = pv.OCOptimizer(top)
+    hist, its = [], []
+    _sync()
+    t0 = time.perf_counter()
+    for k in range(2 * iters):
+        if k == iters:
+            top.filterChain.filters[-1].beta = 8
+            tps.gamma = 5
+        oc.step()
+        hist.append(top.evaluateObjective())
+        its.append(int(obj.lastIterations))
+    _sync()
+    dt = time.perf_counter() - t0
+    return {"config": "p0", "grid": ne, "dof": 2 * (ne[0] + 1) * (ne[1] + 1), "mg_levels": levels,
+            "oc_iterations": 2 * iters, "s": dt, "s_per_iteration": dt / (2 * iters), "final_compliance": hist[-1],
+            "compliance_history": hist, "pcg_iterations": its,
+            "volume_constraint": float(top.evaluateConstraints()[0])}
+
+
+def p1():
+    pv = _pv()
+    ne = (256, 128)
+    tps = pv.TensorProductSimulator([1, 1], [[0.0, 0.0], [2.0, 1.0]], list(ne))
+    d = wl.write_json(wl.MATERIAL, name="material.json")
+    tps.readMaterial(d)
+    tps.applyDisplacementsAndLoadsFromFile(wl.write_json(wl.MBB2D_BC, os.path.dirname(d)))
+    tps.setDensities(wl.smoothed_density(ne, 1, sigma=3.0, mean=0.5, std=0.15))
+    sim = tps.getIntermediateFabricationShape(1.0, False, pv.InterpolationLaw.SIMP)
+    levels = optMGLevel(ne, 2)
+    mg = sim.multigridSolver(levels)
+    out = {"config": "p1", "grid": ne, "layers": ne[1], "mg_levels": levels}
+    for method in ("N=3", "zero"):
+        ev = pv.LayerByLayerEvaluator(sim)
+        ev.selectInitMethod(method)
+        ev.batched = False
+        ts = []
+        for _ in range(2):
+            ev.run(mg, True, 1, 1000, 1e-5)
+            ts.append(ev.lastRunSeconds)
+        out[method] = {"s": min(ts), "objective": ev.objective(), "pcg_iterations_total": int(sum(ev.perLayerIterations()))}
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("configs", nargs="+", choices=["c0", "c1", "c2", "c3", "c4"])
+    ap.add_argument("configs", nargs="+", choices=["c0", "c1", "c2", "c3", "c4", "p0", "p1"])
     ap.add_argument("--out", default=None)
     ap.add_argument("--c4-iters", type=int, default=50)
     args = ap.parse_args(argv)
-    fns = {"c0": c0, "c1": c1, "c2": c2, "c3": c3, "c4": lambda: c4(args.c4_iters)}
+    fns = {"c0": c0, "c1": c1, "c2": c2, "c3": c3, "c4": lambda: c4(args.c4_iters), "p0": p0, "p1": p1}
     for c in args.configs:
         print(f"running {c}", file=sys.stderr, flush=True)
         res = fns[c]()

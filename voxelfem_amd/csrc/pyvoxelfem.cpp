@@ -1,9 +1,9 @@
 // pyvoxelfem.cpp -- pybind11 module `pyVoxelFEM` with the reference's Python API
 // (python_bindings/VoxelFEM.cc), implemented in C++ on top of the C ABI of include/voxelfem.h.
 //
-// Only the [1,1,1] (trilinear hex) double-precision simulator is provided: the solve-and-sensitivity
-// path. Fields cross this boundary as numpy arrays exactly like the reference: nodal fields are
-// (numNodes, 3) float64 arrays, element fields flat C-order float64 vectors. Device-resident state
+// The [1,1,1] (trilinear hex) and [1,1] (bilinear quad) double-precision simulators are provided, as in
+// the reference. Fields cross this boundary as numpy arrays exactly like the reference: nodal fields are
+// (numNodes, N) float64 arrays (N = 3 or 2), element fields flat C-order float64 vectors. Device-resident state
 // (densities, Young's moduli, displacements, load vectors, filter intermediates) stays in HBM between
 // calls; host copies are made only when a value is handed back to Python.
 #include <pybind11/functional.h>
@@ -77,19 +77,19 @@ struct DevVec {
     }
 };
 
-// nodal (nn,3) numpy <-> SoA component planes
-std::vector<double> vfield_to_soa(const Arr &a, size_t nn) {
-    if (a.ndim() != 2 || (size_t)a.shape(0) != nn || a.shape(1) != 3)
-        throw std::runtime_error("expected a nodal field of shape (" + std::to_string(nn) + ", 3)");
+// nodal (nn, N) numpy <-> SoA component planes (N = 3, or 2 on a planar grid)
+std::vector<double> vfield_to_soa(const Arr &a, size_t nn, int N = 3) {
+    if (a.ndim() != 2 || (size_t)a.shape(0) != nn || a.shape(1) != N)
+        throw std::runtime_error("expected a nodal field of shape (" + std::to_string(nn) + ", " + std::to_string(N) + ")");
     auto r = a.unchecked<2>();
-    std::vector<double> out(3 * nn);
+    std::vector<double> out((size_t)N * nn);
     for (size_t n = 0; n < nn; ++n)
-        for (int c = 0; c < 3; ++c) out[c * nn + n] = r(n, c);
+        for (int c = 0; c < N; ++c) out[c * nn + n] = r(n, c);
     return out;
 }
-py::array_t<double> soa_to_vfield(const double *p, size_t nn) {
-    py::array_t<double, py::array::f_style> out({(py::ssize_t)nn, (py::ssize_t)3});
-    std::memcpy(out.mutable_data(), p, 3 * nn * sizeof(double));
+py::array_t<double> soa_to_vfield(const double *p, size_t nn, int N = 3) {
+    py::array_t<double, py::array::f_style> out({(py::ssize_t)nn, (py::ssize_t)N});
+    std::memcpy(out.mutable_data(), p, (size_t)N * nn * sizeof(double));
     return out;
 }
 std::vector<double> to_vec(const Arr &a, size_t n, const char *what) {
@@ -110,8 +110,11 @@ enum class NumberType { DOUBLE = 0, FLOAT = 1 };
 // ====================================================================================================
 // TensorProductSimulator<double,1,1,1>
 // ====================================================================================================
+// A planar (2D) simulator is held with N = 2 and a third axis of one element (ne[2] = 1, no node along it:
+// nnodes3()[2] = 1), so node and element indices read the same as the 3D ones with iz = ez = 0.
 struct TPS : std::enable_shared_from_this<TPS> {
     vf_sim *sim = nullptr;
+    int N = 3;
     std::array<int64_t, 3> ne;
     std::array<double, 3> lo, hi;
     size_t nn = 0, nel = 0;
@@ -125,23 +128,35 @@ struct TPS : std::enable_shared_from_this<TPS> {
 
     void *stream = nullptr;  // the simulator's own stream (ordered with the null stream)
 
-    TPS(const std::array<double, 3> &minC, const std::array<double, 3> &maxC, const std::array<int64_t, 3> &n)
-        : ne(n), lo(minC), hi(maxC) {
+    TPS(const std::array<double, 3> &minC, const std::array<double, 3> &maxC, const std::array<int64_t, 3> &n, int dim = 3)
+        : N(dim), ne(n), lo(minC), hi(maxC) {
         check(vf_stream_create(&stream));
-        check(vf_sim_create(&sim, ne.data(), lo.data(), hi.data(), stream));
-        nn = (size_t)(ne[0] + 1) * (ne[1] + 1) * (ne[2] + 1);
+        if (N == 2) {
+            ne[2] = 1;
+            lo[2] = 0;
+            hi[2] = 1;
+            check(vf_sim_create_2d(&sim, ne.data(), lo.data(), hi.data(), stream));
+        } else {
+            check(vf_sim_create(&sim, ne.data(), lo.data(), hi.data(), stream));
+        }
+        const auto nd = nnodes3();
+        nn = (size_t)(nd[0] * nd[1] * nd[2]);
         nel = (size_t)ne[0] * ne[1] * ne[2];
         dmask.assign(nn, 0);
-        dvals.assign(3 * nn, 0.0);
-        forces.assign(3 * nn, 0.0);
+        dvals.assign((size_t)N * nn, 0.0);
+        forces.assign((size_t)N * nn, 0.0);
         push_law();
     }
+    uint8_t fullMask() const { return N == 2 ? 3 : 7; }
+    // the first N entries of a 3-array, for Python
+    template <typename T>
+    std::vector<T> dims(const std::array<T, 3> &a) const { return std::vector<T>(a.begin(), a.begin() + N); }
     ~TPS() {
         if (sim) vf_sim_destroy(sim);
         if (stream) vf_stream_destroy(stream);
     }
 
-    std::array<int64_t, 3> nnodes3() const { return {ne[0] + 1, ne[1] + 1, ne[2] + 1}; }
+    std::array<int64_t, 3> nnodes3() const { return {ne[0] + 1, ne[1] + 1, N == 2 ? 1 : ne[2] + 1}; }
     std::array<double, 3> h() const { return {(hi[0] - lo[0]) / ne[0], (hi[1] - lo[1]) / ne[1], (hi[2] - lo[2]) / ne[2]}; }
     // element stretchings (the node spacing h() unless set through the dx property)
     std::array<double, 3> stretch() const {
@@ -167,7 +182,7 @@ struct TPS : std::enable_shared_from_this<TPS> {
     }
     // BCBuilder::setDirichlet (TensorProductSimulator.hh:488-504)
     void setDirichlet(size_t n, const std::array<double, 3> &val, uint8_t cm) {
-        for (int c = 0; c < 3; ++c) {
+        for (int c = 0; c < N; ++c) {
             if (!((cm >> c) & 1)) continue;
             if (!((dmask[n] >> c) & 1)) {
                 dmask[n] |= (uint8_t)(1 << c);
@@ -189,6 +204,7 @@ struct TPS : std::enable_shared_from_this<TPS> {
         check(vf_sim_get_young(sim, v.data(), VF_HOST));
         return v;
     }
+    int keSize() const { return N == 2 ? 8 : 24; }
     std::vector<double> K0() const {
         std::vector<double> k(576);
         check(vf_sim_get_K0(sim, k.data()));
@@ -204,7 +220,9 @@ struct TPS : std::enable_shared_from_this<TPS> {
         check(vf_sim_get_mask_info(sim, &m, nullptr, nullptr));
         return m;
     }
+    // local node l (2D: l = 2 i + j, mapped onto the 3D numbering 4 i + 2 j)
     size_t elemNode(size_t e, int l) const {
+        if (N == 2) l *= 2;
         const size_t ez = e % ne[2], ey = (e / ne[2]) % ne[1], ex = e / (ne[1] * ne[2]);
         const auto N = nnodes3();
         return ((ex + ((l >> 2) & 1)) * N[1] + ey + ((l >> 1) & 1)) * N[2] + ez + (l & 1);
@@ -235,7 +253,7 @@ using MGPtr = std::shared_ptr<MG>;
 
 std::vector<double> TPS::solve(const std::vector<double> &fsoa) {
     if (!directSolver) directSolver = std::make_shared<MG>(shared_from_this(), 0);
-    std::vector<double> x(3 * nn, 0.0);
+    std::vector<double> x((size_t)N * nn, 0.0);
     check(vf_mg_update_stiffness(directSolver->mg));
     check(vf_mg_coarse_solve(directSolver->mg, fsoa.data(), x.data(), VF_HOST));
     return x;
@@ -251,16 +269,19 @@ struct LevelView {
 // Filters (TopologyOptimizationFilter.hh:90-400) -- device-resident
 // ====================================================================================================
 struct Filter {
-    std::array<int64_t, 3> dims{0, 0, 0};
+    std::array<int64_t, 3> dims{0, 0, 0};  // a planar grid: {nx, ny, 1} with nd = 2
+    int nd = 3;
     bool dimsSet = false;
     virtual ~Filter() = default;
     virtual std::string name() const = 0;
     size_t size() const { return (size_t)(dims[0] * dims[1] * dims[2]); }
     void setDims(const std::vector<int64_t> &d) {
-        if (d.size() != 3) throw std::runtime_error("Dimension count mismatch");
-        dims = {d[0], d[1], d[2]};
+        if (d.size() != 3 && d.size() != 2) throw std::runtime_error("Dimension count mismatch");
+        nd = (int)d.size();
+        dims = {d[0], d[1], nd == 3 ? d[2] : 1};
         dimsSet = true;
     }
+    std::vector<int64_t> gridDims() const { return std::vector<int64_t>(dims.begin(), dims.begin() + nd); }
     void checkDims() const {
         if (!dimsSet)
             throw std::runtime_error(
@@ -283,7 +304,8 @@ struct SmoothingFilter : Filter {
     void apply(const DevVec &in, DevVec &out) override {
         checkDims();
         out.resize(size());
-        check(vf_smoothing_filter(dims.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
+        if (nd == 2) check(vf_smoothing_filter_2d(dims.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
+        else check(vf_smoothing_filter(dims.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
     }
     void backprop(const DevVec &g, const DevVec &, DevVec &out) override { apply(g, out); }  // S^T = S
 };
@@ -333,9 +355,11 @@ struct FilterChain {
     std::vector<FilterPtr> filters;
     std::vector<std::unique_ptr<DevVec>> vars;  // vars[0] design, vars.back() physical
     std::array<int64_t, 3> dims;
+    int nd = 3;
     FilterChain(std::vector<FilterPtr> f, const std::vector<int64_t> &outDims) : filters(std::move(f)) {
-        if (outDims.size() != 3) throw std::runtime_error("Dimension count mismatch");
-        dims = {outDims[0], outDims[1], outDims[2]};
+        if (outDims.size() != 3 && outDims.size() != 2) throw std::runtime_error("Dimension count mismatch");
+        nd = (int)outDims.size();
+        dims = {outDims[0], outDims[1], nd == 3 ? outDims[2] : 1};
         for (auto &fl : filters) fl->setDims(outDims);
         for (size_t i = 0; i <= filters.size(); ++i) vars.emplace_back(std::make_unique<DevVec>(size()));
     }
@@ -406,8 +430,8 @@ struct ComplianceObjective : Objective {
     TPSPtr tps;
     DevVec u, f;
     explicit ComplianceObjective(TPSPtr t, bool skipSolve = false) : tps(std::move(t)) {
-        f.resize(3 * tps->nn);
-        u.resize(3 * tps->nn);
+        f.resize((size_t)tps->N * tps->nn);
+        u.resize((size_t)tps->N * tps->nn);
         u.fill_host(0.0);
         check(vf_sim_build_load(tps->sim, f.p, VF_DEVICE));
         check(vf_stream_sync(nullptr));
@@ -453,7 +477,7 @@ struct MultigridComplianceObjective : ComplianceObjective {
     static int trampoline(void *user, int64_t it, const double *, const double *r) {
         auto *self = (MultigridComplianceObjective *)user;
         double s = 0;
-        for (size_t i = 0; i < 3 * self->tps->nn; ++i) s += r[i] * r[i];
+        for (size_t i = 0; i < (size_t)self->tps->N * self->tps->nn; ++i) s += r[i] * r[i];
         py::gil_scoped_acquire g;
         self->residual_cb((size_t)it, std::sqrt(s));
         return 0;
@@ -481,7 +505,7 @@ struct Problem {
 
     Problem(TPSPtr t, ObjectivePtr o, std::vector<ConstraintPtr> c, std::vector<FilterPtr> f)
         : tps(std::move(t)), objective(std::move(o)), constraints(std::move(c)), filterList(f) {
-        chain = std::make_shared<FilterChain>(f, std::vector<int64_t>(tps->ne.begin(), tps->ne.end()));
+        chain = std::make_shared<FilterChain>(f, std::vector<int64_t>(tps->ne.begin(), tps->ne.begin() + tps->N));
     }
     virtual ~Problem() = default;
     size_t numVars() const { return chain->size(); }
@@ -674,7 +698,7 @@ struct LBL {
         const auto t0 = std::chrono::steady_clock::now();
         // the zero guess makes the layer solves independent, so they can run batched; the other guesses
         // chain each layer to the previous solutions and run one after the other
-        const size_t nDof = 3 * (size_t)tps->nn;
+        const size_t nDof = (size_t)tps->N * tps->nn;
         const bool fullValid = uFullDesign.size() == nDof;
         // callbacks see every iterate and layer in order: they run with the sequential evaluator
         if (batched && !callbacks && initMethod == VF_LBL_INIT_ZERO && (zeroInit || !fullValid)) {
@@ -689,15 +713,16 @@ struct LBL {
                 std::function<void(size_t, py::array_t<double>, py::array_t<double>)> *it;
                 py::object *lbl;
                 size_t nn, ne;
-            } ctx{&it_cb, &lblCallback, (size_t)tps->nn, (size_t)tps->nel};
+                int N;
+            } ctx{&it_cb, &lblCallback, (size_t)tps->nn, (size_t)tps->nel, tps->N};
             auto itr = [](void *user, int64_t it, const double *xx, const double *rr) -> int {
                 auto *c = (Ctx *)user;
-                (*c->it)((size_t)it, soa_to_vfield(xx, c->nn), soa_to_vfield(rr, c->nn));
+                (*c->it)((size_t)it, soa_to_vfield(xx, c->nn, c->N), soa_to_vfield(rr, c->nn, c->N));
                 return 0;
             };
             auto lbr = [](void *user, int64_t l, double compliance, const double *g, const double *u) -> int {
                 auto *c = (Ctx *)user;
-                (*c->lbl)((size_t)l, compliance, from_vec(std::vector<double>(g, g + c->ne)), soa_to_vfield(u, c->nn));
+                (*c->lbl)((size_t)l, compliance, from_vec(std::vector<double>(g, g + c->ne)), soa_to_vfield(u, c->nn, c->N));
                 return 0;
             };
             check(vf_lbl_run(solver.mg, (int64_t)layerIncrement, (int64_t)maxIter, tol, (int64_t)mgIterations,
@@ -731,6 +756,7 @@ void applyBCFile(TPS &t, const std::string &path) {
         data = json.attr("loads")(ss.str());
     }
     const std::array<double, 3> size = {t.hi[0] - t.lo[0], t.hi[1] - t.lo[1], t.hi[2] - t.lo[2]};
+    const int N = t.N;  // a planar grid reads the first two entries of every corner and value
     for (auto reg : data["regions"]) {
         auto r = reg.cast<py::dict>();
         const std::string type = r["type"].cast<std::string>();
@@ -738,27 +764,30 @@ void applyBCFile(TPS &t, const std::string &path) {
         if (r.contains("box%")) {
             auto b = r["box%"].cast<py::dict>();
             auto mn = b["minCorner"].cast<std::vector<double>>(), mx = b["maxCorner"].cast<std::vector<double>>();
-            for (int d = 0; d < 3; ++d) { bmin[d] = t.lo[d] + mn.at(d) * size[d]; bmax[d] = t.lo[d] + mx.at(d) * size[d]; }
+            bmin = t.lo; bmax = t.hi;
+            for (int d = 0; d < N; ++d) { bmin[d] = t.lo[d] + mn.at(d) * size[d]; bmax[d] = t.lo[d] + mx.at(d) * size[d]; }
         } else if (r.contains("box")) {
             auto b = r["box"].cast<py::dict>();
             auto mn = b["minCorner"].cast<std::vector<double>>(), mx = b["maxCorner"].cast<std::vector<double>>();
-            for (int d = 0; d < 3; ++d) { bmin[d] = mn.at(d); bmax[d] = mx.at(d); }
+            bmin = t.lo; bmax = t.hi;
+            for (int d = 0; d < N; ++d) { bmin[d] = mn.at(d); bmax[d] = mx.at(d); }
         } else {
             throw std::runtime_error("Unsupported region specification (need box% or box)");
         }
         auto value = r["value"].cast<std::vector<double>>();
-        if (value.size() != 3) throw std::runtime_error("BC value must have 3 components");
+        if ((int)value.size() < N) throw std::runtime_error("BC value must have " + std::to_string(N) + " components");
+        value.resize(3, 0.0);
         std::vector<size_t> nodes;
         for (size_t n = 0; n < t.nn; ++n) {
             auto p = t.nodePos(n);
             bool in = true;
-            for (int d = 0; d < 3; ++d) in &= p[d] >= bmin[d] && p[d] <= bmax[d];
+            for (int d = 0; d < N; ++d) in &= p[d] >= bmin[d] && p[d] <= bmax[d];
             if (in) nodes.push_back(n);
         }
         if (type == "force") {
             if (nodes.empty()) throw std::runtime_error("Force constraint region unmatched");
             for (size_t n : nodes)
-                for (int c = 0; c < 3; ++c) t.forces[c * t.nn + n] = value[c] / (double)nodes.size();
+                for (int c = 0; c < N; ++c) t.forces[c * t.nn + n] = value[c] / (double)nodes.size();
         } else if (type.rfind("dirichlet", 0) == 0) {
             if (nodes.empty()) throw std::runtime_error("Dirichlet region unmatched");
             const std::string comps = type.size() > 9 ? type.substr(9) : "xyz";
@@ -769,7 +798,7 @@ void applyBCFile(TPS &t, const std::string &path) {
                 else if (ch == 'z') cm |= 4;
                 else throw std::runtime_error("Illegal constraint type, only \"dirichlet\" and \"force\" accepted");
             }
-            for (size_t n : nodes) t.setDirichlet(n, {value[0], value[1], value[2]}, cm);
+            for (size_t n : nodes) t.setDirichlet(n, {value[0], value[1], value[2]}, (uint8_t)(cm & t.fullMask()));
         } else {
             throw std::runtime_error("Illegal constraint type, only \"dirichlet\" and \"force\" accepted");
         }
@@ -788,8 +817,9 @@ uint8_t parseMask(const std::string &s) {
     return cm;
 }
 
-TPSPtr makeTPS(const std::array<double, 3> &lo, const std::array<double, 3> &hi, const std::array<int64_t, 3> &ne) {
-    return std::make_shared<TPS>(lo, hi, ne);
+TPSPtr makeTPS(const std::array<double, 3> &lo, const std::array<double, 3> &hi, const std::array<int64_t, 3> &ne,
+               int N = 3) {
+    return std::make_shared<TPS>(lo, hi, ne, N);
 }
 
 void copyMaterial(const TPS &src, TPS &dst) {
@@ -827,7 +857,7 @@ TPSPtr intermediateShape(const TPS &t, double hfrac, bool validate, Interpolatio
     ne[1] = helems;
     std::array<double, 3> hi = t.hi;
     hi[1] = t.lo[1] + hfrac * (t.hi[1] - t.lo[1]);
-    auto r = makeTPS(t.lo, hi, ne);
+    auto r = makeTPS(t.lo, hi, ne, t.N);
     copyMaterial(t, *r);
     r->law = law;
     r->push_law();
@@ -847,7 +877,7 @@ TPSPtr intermediateShape(const TPS &t, double hfrac, bool validate, Interpolatio
     }
     check(vf_sim_set_gravity(r->sim, r->grav.data()));
     for (size_t n = 0; n < r->nn; ++n)
-        if (r->nodePos(n)[1] == r->lo[1]) r->setDirichlet(n, {0, 0, 0}, 7);
+        if (r->nodePos(n)[1] == r->lo[1]) r->setDirichlet(n, {0, 0, 0}, r->fullMask());
     r->push_bc();
     return r;
 }
@@ -909,13 +939,13 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
             }, py::arg("density"))
         .def("setDensitiesFromCoarseGrid", [](TPS &t, size_t f, const Arr &rho) {
                 const size_t nc = (size_t)rho.size();
-                if (t.nel != f * f * f * nc) throw std::runtime_error("Incorrect rho size");
+                if (t.nel != (t.N == 2 ? f * f : f * f * f) * nc) throw std::runtime_error("Incorrect rho size");
                 auto r = to_vec(rho, nc, "rho");
-                const int64_t c1 = t.ne[1] / f, c2 = t.ne[2] / f;
+                const int64_t c1 = t.ne[1] / f, c2 = t.N == 2 ? 1 : t.ne[2] / f;
                 std::vector<double> out(t.nel);
                 for (int64_t e = 0; e < (int64_t)t.nel; ++e) {
                     const int64_t ez = e % t.ne[2], ey = (e / t.ne[2]) % t.ne[1], ex = e / (t.ne[1] * t.ne[2]);
-                    out[e] = r[((ex / f) * c1 + ey / f) * c2 + ez / f];
+                    out[e] = r[((ex / f) * c1 + ey / f) * c2 + (t.N == 2 ? 0 : ez / f)];
                 }
                 t.setDensities(out);
             }, py::arg("upscalingFactor"), py::arg("rho"))
@@ -925,56 +955,57 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def("setFabricationMaskHeightByLayer", [](TPS &t, int64_t l) { check(vf_sim_set_mask_layer(t.sim, l)); }, py::arg("l"))
         .def("getFabricationMaskHeight", [](const TPS &t) { return t.maskHeight(); })
         .def("solve", [](TPS &t, const Arr &f) {
-                auto x = t.solve(vfield_to_soa(f, t.nn));
-                return soa_to_vfield(x.data(), t.nn);
+                auto x = t.solve(vfield_to_soa(f, t.nn, t.N));
+                return soa_to_vfield(x.data(), t.nn, t.N);
             }, py::arg("f"))
         .def("solveWithImposedLoads", [](TPS &t) {
-                std::vector<double> f(3 * t.nn);
+                std::vector<double> f((size_t)t.N * t.nn);
                 check(vf_sim_build_load(t.sim, f.data(), VF_HOST));
                 auto x = t.solve(f);
-                return soa_to_vfield(x.data(), t.nn);
+                return soa_to_vfield(x.data(), t.nn, t.N);
             })
         .def("complianceGradient", [](const TPS &t, const Arr &u) {
-                auto us = vfield_to_soa(u, t.nn);
+                auto us = vfield_to_soa(u, t.nn, t.N);
                 std::vector<double> g(t.nel);
                 check(vf_sim_compliance_gradient(t.sim, us.data(), g.data(), VF_HOST));
                 return from_vec(g);
             }, py::arg("u"))
         .def("multigridSolver", [](TPSPtr t, size_t L) { return std::make_shared<MG>(t, (int64_t)L); }, py::arg("numCoarseningLevels"))
         .def("applyK", [](const TPS &t, const Arr &u) {
-                auto us = vfield_to_soa(u, t.nn);
-                std::vector<double> y(3 * t.nn);
+                auto us = vfield_to_soa(u, t.nn, t.N);
+                std::vector<double> y((size_t)t.N * t.nn);
                 check(vf_sim_apply_K(t.sim, us.data(), y.data(), VF_HOST));
-                return soa_to_vfield(y.data(), t.nn);
+                return soa_to_vfield(y.data(), t.nn, t.N);
             }, py::arg("u"))
         .def("getDirichletVarsAndValues", [](const TPS &t) {
                 std::vector<size_t> vars; std::vector<double> vals;
                 for (size_t n = 0; n < t.nn; ++n)
-                    for (int c = 0; c < 3; ++c)
-                        if ((t.dmask[n] >> c) & 1) { vars.push_back(3 * n + c); vals.push_back(t.dvals[c * t.nn + n]); }
+                    for (int c = 0; c < t.N; ++c)
+                        if ((t.dmask[n] >> c) & 1) { vars.push_back(t.N * n + c); vals.push_back(t.dvals[c * t.nn + n]); }
                 return std::make_tuple(vars, vals);
             })
         .def("buildLoadVector", [](const TPS &t) {
-                std::vector<double> f(3 * t.nn);
+                std::vector<double> f((size_t)t.N * t.nn);
                 check(vf_sim_build_load(t.sim, f.data(), VF_HOST));
-                return soa_to_vfield(f.data(), t.nn);
+                return soa_to_vfield(f.data(), t.nn, t.N);
             })
-        .def("nodePosition", [](const TPS &t, size_t n) { return t.nodePos(n); }, py::arg("ni"))
+        .def("nodePosition", [](const TPS &t, size_t n) { return t.dims(t.nodePos(n)); }, py::arg("ni"))
         .def("applyDisplacementsAndLoadsFromFile", [](TPS &t, const std::string &p) { applyBCFile(t, p); }, py::arg("bcPath"))
-        .def("elementIndexForGridCell", [](const TPS &t, const std::array<int64_t, 3> &c) {
-                for (int d = 0; d < 3; ++d) if (c[d] < 0 || c[d] >= t.ne[d]) throw std::runtime_error("Indices out of bounds of NDVector");
-                return (size_t)((c[0] * t.ne[1] + c[1]) * t.ne[2] + c[2]);
+        .def("elementIndexForGridCell", [](const TPS &t, const std::vector<int64_t> &c) {
+                if ((int)c.size() != t.N) throw std::runtime_error("expected " + std::to_string(t.N) + " cell indices");
+                for (int d = 0; d < t.N; ++d) if (c[d] < 0 || c[d] >= t.ne[d]) throw std::runtime_error("Indices out of bounds of NDVector");
+                return (size_t)((c[0] * t.ne[1] + c[1]) * t.ne[2] + (t.N == 3 ? c[2] : 0));
             }, py::arg("cellIdxs"))
         .def("getForceMask", [](const TPS &t) {
-                py::array_t<bool> out({(py::ssize_t)t.nn, (py::ssize_t)3});
+                py::array_t<bool> out({(py::ssize_t)t.nn, (py::ssize_t)t.N});
                 auto w = out.mutable_unchecked<2>();
-                for (size_t n = 0; n < t.nn; ++n) for (int c = 0; c < 3; ++c) w(n, c) = t.forces[c * t.nn + n] != 0;
+                for (size_t n = 0; n < t.nn; ++n) for (int c = 0; c < t.N; ++c) w(n, c) = t.forces[c * t.nn + n] != 0;
                 return out;
             })
         .def("getDirichletMask", [](const TPS &t) {
-                py::array_t<bool> out({(py::ssize_t)t.nn, (py::ssize_t)3});
+                py::array_t<bool> out({(py::ssize_t)t.nn, (py::ssize_t)t.N});
                 auto w = out.mutable_unchecked<2>();
-                for (size_t n = 0; n < t.nn; ++n) for (int c = 0; c < 3; ++c) w(n, c) = (t.dmask[n] >> c) & 1;
+                for (size_t n = 0; n < t.nn; ++n) for (int c = 0; c < t.N; ++c) w(n, c) = (t.dmask[n] >> c) & 1;
                 return out;
             })
         .def("getBCIndicatorField", [](const TPS &t) {
@@ -982,43 +1013,52 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 for (size_t n = 0; n < t.nn; ++n) {
                     r[n] += t.dmask[n] & 7;
                     int fb = 0;
-                    for (int c = 0; c < 3; ++c) if (t.forces[c * t.nn + n] != 0) fb |= 1 << c;
-                    r[n] += 8 * fb;
+                    for (int c = 0; c < t.N; ++c) if (t.forces[c * t.nn + n] != 0) fb |= 1 << c;
+                    r[n] += (t.N == 2 ? 4 : 8) * fb;
                 }
                 return from_vec(r);
             })
         .def("elementStiffnessMatrix", [](const TPS &t, size_t e) {
                 auto K = t.K0(); const double E = t.getE().at(e);
-                py::array_t<double> out({24, 24});
+                const int ks = t.keSize();
+                py::array_t<double> out({ks, ks});
                 auto w = out.mutable_unchecked<2>();
-                for (int a = 0; a < 24; ++a) for (int b = 0; b < 24; ++b) w(a, b) = E * K[a * 24 + b];
+                for (int a = 0; a < ks; ++a) for (int b = 0; b < ks; ++b) w(a, b) = E * K[a * ks + b];
                 return out;
             }, py::arg("ei"))
         .def("fullDensityElementStiffnessMatrix", [](const TPS &t) {
                 auto K = t.K0();
-                py::array_t<double> out({24, 24});
-                std::memcpy(out.mutable_data(), K.data(), 576 * sizeof(double));
+                const int ks = t.keSize();
+                py::array_t<double> out({ks, ks});
+                std::memcpy(out.mutable_data(), K.data(), (size_t)ks * ks * sizeof(double));
                 return out;
             })
         .def("elementNodes", [](const TPS &t, size_t e) {
-                std::vector<size_t> r(8); for (int l = 0; l < 8; ++l) r[l] = t.elemNode(e, l); return r;
+                const int nv = t.N == 2 ? 4 : 8;
+                std::vector<size_t> r(nv); for (int l = 0; l < nv; ++l) r[l] = t.elemNode(e, l); return r;
             }, py::arg("ei"))
         .def("elemNodeGlobalIndex", [](const TPS &t, size_t e, int l) { return t.elemNode(e, l); }, py::arg("ei"), py::arg("n"))
-        .def("addDirichletCondition", [](TPS &t, const std::array<double, 3> &u, const std::array<double, 3> &mn,
-                                         const std::array<double, 3> &mx, const std::string &mask) {
-                const uint8_t cm = parseMask(mask);
+        .def("addDirichletCondition", [](TPS &t, const std::vector<double> &u, const std::vector<double> &mn,
+                                         const std::vector<double> &mx, const std::string &mask) {
+                if ((int)u.size() != t.N || (int)mn.size() != t.N || (int)mx.size() != t.N)
+                    throw std::runtime_error("expected " + std::to_string(t.N) + "-vectors");
+                const uint8_t cm = parseMask(mask) & t.fullMask();
+                std::array<double, 3> uv{0, 0, 0};
+                for (int d = 0; d < t.N; ++d) uv[d] = u[d];
                 for (size_t n = 0; n < t.nn; ++n) {
                     auto p = t.nodePos(n);
                     bool in = true;
-                    for (int d = 0; d < 3; ++d) in &= p[d] >= mn[d] && p[d] <= mx[d];
-                    if (in) t.setDirichlet(n, u, cm);
+                    for (int d = 0; d < t.N; ++d) in &= p[d] >= mn[d] && p[d] <= mx[d];
+                    if (in) t.setDirichlet(n, uv, cm);
                 }
                 t.push_bc();
             }, py::arg("u"), py::arg("minCorner"), py::arg("maxCorner"), py::arg("componentMask") = "xyz")
-        .def("applySymmetryConditions", [](TPS &t, const std::array<bool, 3> &axes, const std::array<bool, 3> &face) {
+        .def("applySymmetryConditions", [](TPS &t, const std::vector<bool> &axes, std::vector<bool> face) {
+                if ((int)axes.size() != t.N) throw std::runtime_error("expected " + std::to_string(t.N) + " symmetry flags");
+                face.resize(t.N, false);
                 for (size_t n = 0; n < t.nn; ++n) {
                     auto p = t.nodePos(n);
-                    for (int d = 0; d < 3; ++d) {
+                    for (int d = 0; d < t.N; ++d) {
                         if (!axes[d]) continue;
                         if (std::abs(p[d] - (face[d] ? t.hi[d] : t.lo[d])) < 1e-10) {
                             t.dmask[n] |= (uint8_t)(1 << d);
@@ -1027,20 +1067,24 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                     }
                 }
                 t.push_bc();
-            }, py::arg("symmetry_axes"), py::arg("minMaxFace") = std::array<bool, 3>{false, false, false})
-        .def_property_readonly("domain", [](const TPS &t) { return std::make_pair(t.lo, t.hi); })
-        .def_property_readonly("bbox", [](const TPS &t) { return std::make_pair(t.lo, t.hi); })
-        .def_property_readonly("gridShape", [](const TPS &t) { return t.ne; })
-        .def_property_readonly("NbElementsPerDimension", [](const TPS &t) { return t.ne; })
-        .def_property_readonly("NbNodesPerDimension", [](const TPS &t) { return t.nnodes3(); })
+            }, py::arg("symmetry_axes"), py::arg("minMaxFace") = std::vector<bool>{})
+        .def_property_readonly("domain", [](const TPS &t) { return std::make_pair(t.dims(t.lo), t.dims(t.hi)); })
+        .def_property_readonly("bbox", [](const TPS &t) { return std::make_pair(t.dims(t.lo), t.dims(t.hi)); })
+        .def_property_readonly("gridShape", [](const TPS &t) { return t.dims(t.ne); })
+        .def_property_readonly("NbElementsPerDimension", [](const TPS &t) { return t.dims(t.ne); })
+        .def_property_readonly("NbNodesPerDimension", [](const TPS &t) { return t.dims(t.nnodes3()); })
         .def_property("interpolationLaw", [](const TPS &t) { return t.law; },
                       [](TPS &t, InterpolationLaw l) { t.law = l; t.push_law(); })
         .def_property("E_0", [](const TPS &t) { return t.E0; }, [](TPS &t, double v) { t.E0 = v; t.push_law(); })
         .def_property("E_min", [](const TPS &t) { return t.Emin; }, [](TPS &t, double v) { t.Emin = v; t.push_law(); })
         .def_property("gamma", [](const TPS &t) { return t.gamma; }, [](TPS &t, double v) { t.gamma = v; t.push_law(); })
         .def_property("q", [](const TPS &t) { return t.q; }, [](TPS &t, double v) { t.q = v; t.push_law(); })
-        .def_property("gravity", [](const TPS &t) { return t.grav; },
-                      [](TPS &t, const std::array<double, 3> &g) { t.grav = g; check(vf_sim_set_gravity(t.sim, g.data())); })
+        .def_property("gravity", [](const TPS &t) { return t.dims(t.grav); },
+                      [](TPS &t, const std::vector<double> &g) {
+                          if ((int)g.size() != t.N) throw std::runtime_error("gravity must have " + std::to_string(t.N) + " components");
+                          t.grav = {g[0], g[1], t.N == 3 ? g[2] : 0.0};
+                          check(vf_sim_set_gravity(t.sim, t.grav.data()));
+                      })
         .def_property("ETensor", [](const TPS &t) { IsoETensor e; e.E = t.young; e.nu = t.poisson; return e; },
                       [](TPS &t, py::object o) {
                           if (py::isinstance<IsoETensor>(o)) {
@@ -1052,12 +1096,19 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                           }
                           check(vf_sim_set_material(t.sim, t.young, t.poisson));
                       }, "Isotropic elasticity tensor (an ElasticityTensor, or a (Young's modulus, Poisson ratio) pair)")
-        .def_property("dx", [](const TPS &t) { return t.stretch(); },
-                      [](TPS &t, const std::array<double, 3> &h) { check(vf_sim_set_stretchings(t.sim, h.data())); },
+        .def_property("dx", [](const TPS &t) { return t.dims(t.stretch()); },
+                      [](TPS &t, const std::vector<double> &h) {
+                          if ((int)h.size() != t.N) throw std::runtime_error("expected " + std::to_string(t.N) + " stretchings");
+                          const double hh[3] = {h[0], h[1], t.N == 3 ? h[2] : 1.0};
+                          check(vf_sim_set_stretchings(t.sim, hh));
+                      },
                       "Get the dimensions of a grid cell")
-        .def_property_readonly("elementVolume", [](const TPS &t) { auto h = t.stretch(); return h[0] * h[1] * h[2]; })
+        .def_property_readonly("elementVolume", [](const TPS &t) {
+                auto h = t.stretch();
+                return t.N == 2 ? h[0] * h[1] : h[0] * h[1] * h[2];
+            })
         .def("elementEnergyDensity", [](const TPS &t, const Arr &u) {
-                auto us = vfield_to_soa(u, t.nn);
+                auto us = vfield_to_soa(u, t.nn, t.N);
                 std::vector<double> out(t.nel);
                 check(vf_sim_energy_density(t.sim, us.data(), out.data(), VF_HOST));
                 return from_vec(out);
@@ -1067,7 +1118,8 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 // triangle (row <= column, DoF 3 node + component) of the stiffness matrix, as a scipy CSC matrix
                 const auto K = t.K0();
                 const auto E = t.getE();
-                const size_t nt = t.nel * 300;
+                const int ks = t.keSize(), nv = ks / t.N;
+                const size_t nt = t.nel * (size_t)(ks * (ks + 1) / 2);
                 py::array_t<double> vals((py::ssize_t)nt);
                 py::array_t<int64_t> rows((py::ssize_t)nt), cols((py::ssize_t)nt);
                 double *v = vals.mutable_data();
@@ -1075,25 +1127,36 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 size_t k = 0;
                 for (size_t e = 0; e < t.nel; ++e) {
                     size_t dofs[24];
-                    for (int l = 0; l < 8; ++l)
-                        for (int c = 0; c < 3; ++c) dofs[3 * l + c] = 3 * t.elemNode(e, l) + c;
-                    for (int a = 0; a < 24; ++a)
-                        for (int b = 0; b < 24; ++b) {
+                    for (int l = 0; l < nv; ++l)
+                        for (int c = 0; c < t.N; ++c) dofs[t.N * l + c] = t.N * t.elemNode(e, l) + c;
+                    for (int a = 0; a < ks; ++a)
+                        for (int b = 0; b < ks; ++b) {
                             if (dofs[a] > dofs[b]) continue;
-                            v[k] = E[e] * K[a * 24 + b];
+                            v[k] = E[e] * K[a * ks + b];
                             ri[k] = (int64_t)dofs[a];
                             ci[k] = (int64_t)dofs[b];
                             ++k;
                         }
                 }
                 py::module_ sp = py::module_::import("scipy.sparse");
-                const py::ssize_t n = (py::ssize_t)(3 * t.nn);
+                const py::ssize_t n = (py::ssize_t)(t.N * t.nn);
                 return sp.attr("csc_matrix")(py::make_tuple(vals, py::make_tuple(rows, cols)), py::make_tuple(n, n));
             })
         .def("constantStrainLoad", [](const TPS &t, const Arr &eps) {
-                // the strain as a 3x3 symmetric matrix or as the 6 values (xx, yy, zz, yz, xz, xy)
-                double e[9];
-                if (eps.ndim() == 2 && eps.shape(0) == 3 && eps.shape(1) == 3) {
+                // the strain as a 3x3 symmetric matrix or as the 6 values (xx, yy, zz, yz, xz, xy); on a planar
+                // grid a 2x2 matrix or the 3 values (xx, yy, xy)
+                double e[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                if (t.N == 2) {
+                    if (eps.ndim() == 2 && eps.shape(0) == 2 && eps.shape(1) == 2) {
+                        auto r = eps.unchecked<2>();
+                        for (int a = 0; a < 2; ++a) for (int b = 0; b < 2; ++b) e[2 * a + b] = r(a, b);
+                    } else if (eps.size() == 3) {
+                        const double *q = eps.data();
+                        e[0] = q[0]; e[3] = q[1]; e[1] = e[2] = q[2];
+                    } else {
+                        throw std::runtime_error("constantStrainLoad: eps must be a 2x2 symmetric matrix or 3 values");
+                    }
+                } else if (eps.ndim() == 2 && eps.shape(0) == 3 && eps.shape(1) == 3) {
                     auto r = eps.unchecked<2>();
                     for (int a = 0; a < 3; ++a) for (int b = 0; b < 3; ++b) e[3 * a + b] = r(a, b);
                 } else if (eps.size() == 6) {
@@ -1105,16 +1168,17 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 } else {
                     throw std::runtime_error("constantStrainLoad: eps must be a 3x3 symmetric matrix or 6 values");
                 }
-                std::vector<double> F(3 * t.nn);
+                std::vector<double> F((size_t)t.N * t.nn);
                 check(vf_sim_constant_strain_load(t.sim, e, F.data(), VF_HOST));
-                return soa_to_vfield(F.data(), t.nn);
+                return soa_to_vfield(F.data(), t.nn, t.N);
             }, py::arg("eps"))
         .def("sampleNodalField", [](TPS &t, const Arr &u, const Arr &p) {
                 if (u.ndim() < 1 || (size_t)u.shape(0) != t.nn) throw std::runtime_error("u must have one row per node");
-                if (p.ndim() != 2 || p.shape(1) != 3) throw std::runtime_error("p must hold one 3D point per row");
+                if (p.ndim() != 2 || p.shape(1) != t.N)
+                    throw std::runtime_error("p must hold one " + std::to_string(t.N) + "D point per row");
                 const int64_t k = u.ndim() == 1 ? 1 : (int64_t)u.shape(1);
                 const int64_t np = (int64_t)p.shape(0);
-                std::vector<double> us((size_t)k * t.nn), ps(3 * (size_t)np), out((size_t)k * np);
+                std::vector<double> us((size_t)k * t.nn), ps((size_t)t.N * np), out((size_t)k * np);
                 auto uu = u.unchecked();
                 for (size_t n = 0; n < t.nn; ++n)
                     for (int64_t c = 0; c < k; ++c)
@@ -1140,17 +1204,17 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
             }, py::arg("intermediateTPS"))
         .def("transferVFieldToIntermediateFabricationShape", [](const TPS &t, const TPS &part, const Arr &u) {
                 checkIntermediateSize(t, part);
-                if (u.ndim() != 2 || (size_t)u.shape(0) != t.nn || u.shape(1) != 3)
-                    throw std::runtime_error("expected a nodal field of shape (" + std::to_string(t.nn) + ", 3)");
+                if (u.ndim() != 2 || (size_t)u.shape(0) != t.nn || u.shape(1) != t.N)
+                    throw std::runtime_error("expected a nodal field of shape (" + std::to_string(t.nn) + ", " + std::to_string(t.N) + ")");
                 const auto N = t.nnodes3(), P = part.nnodes3();
-                py::array_t<double> res({(py::ssize_t)part.nn, (py::ssize_t)3});
+                py::array_t<double> res({(py::ssize_t)part.nn, (py::ssize_t)t.N});
                 auto w = res.mutable_unchecked<2>();
                 auto r = u.unchecked<2>();
                 for (int64_t ix = 0; ix < P[0]; ++ix)
                     for (int64_t iy = 0; iy < P[1]; ++iy)
                         for (int64_t iz = 0; iz < P[2]; ++iz) {
                             const size_t src = (ix * N[1] + iy) * N[2] + iz, dst = (ix * P[1] + iy) * P[2] + iz;
-                            for (int c = 0; c < 3; ++c) w(dst, c) = r(src, c);
+                            for (int c = 0; c < t.N; ++c) w(dst, c) = r(src, c);
                         }
                 return res;
             }, py::arg("intermediateTPS"), py::arg("u"))
@@ -1171,11 +1235,20 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
              }, py::arg("intermediateTPS"), py::arg("rho_in"), py::arg("rho_accum"))
         .def("getMesh", [](const TPS &t) {
                 // getMesh (TensorProductSimulator.hh:747-776): node positions and hexahedra in Gmsh node order
-                py::array_t<double> V({(py::ssize_t)t.nn, (py::ssize_t)3});
+                py::array_t<double> V({(py::ssize_t)t.nn, (py::ssize_t)t.N});
                 auto v = V.mutable_unchecked<2>();
                 for (size_t n = 0; n < t.nn; ++n) {
                     auto pos = t.nodePos(n);
-                    for (int c = 0; c < 3; ++c) v(n, c) = pos[c];
+                    for (int c = 0; c < t.N; ++c) v(n, c) = pos[c];
+                }
+                if (t.N == 2) {
+                    // quads, counter-clockwise from the lower-left corner (local nodes 0, 2, 3, 1)
+                    py::array_t<int64_t> Q({(py::ssize_t)t.nel, (py::ssize_t)4});
+                    auto q = Q.mutable_unchecked<2>();
+                    const int order[4] = {0, 2, 3, 1};
+                    for (size_t e = 0; e < t.nel; ++e)
+                        for (int k = 0; k < 4; ++k) q(e, k) = (int64_t)t.elemNode(e, order[k]);
+                    return py::make_tuple(V, Q);
                 }
                 py::array_t<int64_t> F({(py::ssize_t)t.nel, (py::ssize_t)8});
                 auto f = F.mutable_unchecked<2>();
@@ -1197,6 +1270,7 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 std::vector<double> r(t.nel, 0.0);
                 double i = 0;
                 for (int colour = 0; colour < 8; ++colour) {
+                    if (t.N == 2 && (colour & 1)) continue;  // planar: z parity 0 only (ne[2] = 1)
                     const int cx = (colour >> 2) & 1, cy = (colour >> 1) & 1, cz = colour & 1;
                     for (int64_t x = cx; x < t.ne[0]; x += 2)
                         for (int64_t y = cy; y < t.ne[1]; y += 2)
@@ -1210,51 +1284,53 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def("downsample", [](const TPS &t, size_t levels) {
                 const int64_t f = (int64_t)1 << levels;
                 std::array<int64_t, 3> ne = t.ne;
-                for (int d = 0; d < 3; ++d) {
+                for (int d = 0; d < t.N; ++d) {
                     if (ne[d] % f) throw std::runtime_error("Grid size must be divisible by 2^downsamplingLevels");
                     ne[d] /= f;
                 }
-                auto r = makeTPS(t.lo, t.hi, ne);
+                auto r = makeTPS(t.lo, t.hi, ne, t.N);
                 copyMaterial(t, *r);
                 r->push_law();
                 return r;
             }, py::arg("downsamplingLevels"))
         .def("downsampleDensityFieldTo", [](const TPS &t, const Arr &rho, TPS &c) {
                 const int64_t f = t.ne[0] / c.ne[0];
-                for (int d = 0; d < 3; ++d) if (f * c.ne[d] != t.ne[d]) throw std::runtime_error("Invalid downsampled simulator");
+                for (int d = 0; d < t.N; ++d) if (f * c.ne[d] != t.ne[d]) throw std::runtime_error("Invalid downsampled simulator");
                 auto r = to_vec(rho, t.nel, "input densities");
                 std::vector<double> out(c.nel, 0.0);
-                const double w = 1.0 / (double)(f * f * f);
+                const double w = 1.0 / (double)(t.N == 2 ? f * f : f * f * f);
                 for (int64_t e = 0; e < (int64_t)t.nel; ++e) {
                     const int64_t ez = e % t.ne[2], ey = (e / t.ne[2]) % t.ne[1], ex = e / (t.ne[1] * t.ne[2]);
-                    out[((ex / f) * c.ne[1] + ey / f) * c.ne[2] + ez / f] += r[e] * w;
+                    out[((ex / f) * c.ne[1] + ey / f) * c.ne[2] + (t.N == 2 ? 0 : ez / f)] += r[e] * w;
                 }
                 c.setDensities(out);
             }, py::arg("densities"), py::arg("coarseTPS"))
         .def("upsampleDensityGradientFrom", [](const TPS &t, const TPS &c, const Arr &g) {
                 const int64_t f = t.ne[0] / c.ne[0];
-                for (int d = 0; d < 3; ++d) if (f * c.ne[d] != t.ne[d]) throw std::runtime_error("Invalid downsampled simulator");
+                for (int d = 0; d < t.N; ++d) if (f * c.ne[d] != t.ne[d]) throw std::runtime_error("Invalid downsampled simulator");
                 auto gc = to_vec(g, c.nel, "coarse gradient");
                 std::vector<double> out(t.nel);
-                const double w = 1.0 / (double)(f * f * f);
+                const double w = 1.0 / (double)(t.N == 2 ? f * f : f * f * f);
                 for (int64_t e = 0; e < (int64_t)t.nel; ++e) {
                     const int64_t ez = e % t.ne[2], ey = (e / t.ne[2]) % t.ne[1], ex = e / (t.ne[1] * t.ne[2]);
-                    out[e] = gc[((ex / f) * c.ne[1] + ey / f) * c.ne[2] + ez / f] * w;
+                    out[e] = gc[((ex / f) * c.ne[1] + ey / f) * c.ne[2] + (t.N == 2 ? 0 : ez / f)] * w;
                 }
                 return from_vec(out);
             }, py::arg("coarseTPS"), py::arg("g_coarse"))
         // extension: the C ABI handle (vf_sim*) as an integer, for device-resident callers
         .def("_handle", [](const TPS &t) { return (uintptr_t)t.sim; })
         // extension (no reference counterpart): nodal point forces set directly, (numNodes, 3)
-        .def("setForces", [](TPS &t, const Arr &f) { t.forces = vfield_to_soa(f, t.nn); t.push_bc(); }, py::arg("f"))
+        .def("setForces", [](TPS &t, const Arr &f) { t.forces = vfield_to_soa(f, t.nn, t.N); t.push_bc(); }, py::arg("f"))
         // extension: Dirichlet conditions from a (numNodes, 3) boolean mask (zero displacement)
         .def("setDirichletMask", [](TPS &t, py::array_t<bool, py::array::forcecast> mask) {
-                if (mask.ndim() != 2 || (size_t)mask.shape(0) != t.nn || mask.shape(1) != 3)
-                    throw std::runtime_error("expected a (numNodes, 3) mask");
+                if (mask.ndim() != 2 || (size_t)mask.shape(0) != t.nn || mask.shape(1) != t.N)
+                    throw std::runtime_error("expected a (numNodes, " + std::to_string(t.N) + ") mask");
                 auto r = mask.unchecked<2>();
                 for (size_t n = 0; n < t.nn; ++n) {
-                    t.dmask[n] = (uint8_t)((r(n, 0) ? 1 : 0) | (r(n, 1) ? 2 : 0) | (r(n, 2) ? 4 : 0));
-                    for (int c = 0; c < 3; ++c) t.dvals[c * t.nn + n] = 0.0;
+                    uint8_t m = 0;
+                    for (int c = 0; c < t.N; ++c) m |= r(n, c) ? (uint8_t)(1 << c) : 0;
+                    t.dmask[n] = m;
+                    for (int c = 0; c < t.N; ++c) t.dvals[c * t.nn + n] = 0.0;
                 }
                 t.push_bc();
             }, py::arg("mask"));
@@ -1263,19 +1339,22 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def("numNodes", [](const LevelView &v) { return v.owner->nodes(v.l); })
         .def_property_readonly("NbNodesPerDimension", [](const LevelView &v) {
                 int64_t n3[3]; check(vf_mg_level_nodes(v.owner->mg, v.l, n3));
-                return std::array<int64_t, 3>{n3[0], n3[1], n3[2]};
+                return std::vector<int64_t>(n3, n3 + v.owner->tps->N);
             })
         .def_property_readonly("NbElementsPerDimension", [](const LevelView &v) {
                 int64_t n3[3]; check(vf_mg_level_nodes(v.owner->mg, v.l, n3));
-                return std::array<int64_t, 3>{n3[0] - 1, n3[1] - 1, n3[2] - 1};
+                std::vector<int64_t> r;
+                for (int d = 0; d < v.owner->tps->N; ++d) r.push_back(n3[d] - 1);
+                return r;
             })
         .def("getDirichletMask", [](const LevelView &v) {
                 const size_t nn = v.owner->nodes(v.l);
                 std::vector<uint8_t> m(nn);
                 check(vf_mg_level_dirichlet(v.owner->mg, v.l, m.data()));
-                py::array_t<bool> out({(py::ssize_t)nn, (py::ssize_t)3});
+                const int N = v.owner->tps->N;
+                py::array_t<bool> out({(py::ssize_t)nn, (py::ssize_t)N});
                 auto w = out.mutable_unchecked<2>();
-                for (size_t n = 0; n < nn; ++n) for (int c = 0; c < 3; ++c) w(n, c) = (m[n] >> c) & 1;
+                for (size_t n = 0; n < nn; ++n) for (int c = 0; c < N; ++c) w(n, c) = (m[n] >> c) & 1;
                 return out;
             });
 
@@ -1288,23 +1367,23 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
             }, py::arg("l"))
         .def("computeResidual", [](MG &mg, size_t l, const Arr &u, const Arr &b) {
                 const size_t nn = mg.nodes(l);
-                auto us = vfield_to_soa(u, nn), bs = vfield_to_soa(b, nn);
-                std::vector<double> r(3 * nn);
+                auto us = vfield_to_soa(u, nn, mg.tps->N), bs = vfield_to_soa(b, nn, mg.tps->N);
+                std::vector<double> r((size_t)mg.tps->N * nn);
                 check(vf_mg_residual(mg.mg, l, us.data(), bs.data(), r.data(), VF_HOST));
-                return soa_to_vfield(r.data(), nn);
+                return soa_to_vfield(r.data(), nn, mg.tps->N);
             }, py::arg("l"), py::arg("u"), py::arg("b"))
         .def("applyK", [](MG &mg, size_t l, const Arr &u) {
                 const size_t nn = mg.nodes(l);
-                auto us = vfield_to_soa(u, nn);
-                std::vector<double> y(3 * nn);
+                auto us = vfield_to_soa(u, nn, mg.tps->N);
+                std::vector<double> y((size_t)mg.tps->N * nn);
                 check(vf_mg_apply_K(mg.mg, l, us.data(), y.data(), VF_HOST));
-                return soa_to_vfield(y.data(), nn);
+                return soa_to_vfield(y.data(), nn, mg.tps->N);
             }, py::arg("l"), py::arg("u"))
         .def("zeroOutDirichletComponents", [](MG &mg, size_t l, const Arr &u) {
                 const size_t nn = mg.nodes(l);
-                auto us = vfield_to_soa(u, nn);
+                auto us = vfield_to_soa(u, nn, mg.tps->N);
                 check(vf_mg_zero_dirichlet(mg.mg, l, us.data(), VF_HOST));
-                return soa_to_vfield(us.data(), nn);
+                return soa_to_vfield(us.data(), nn, mg.tps->N);
             }, py::arg("l"), py::arg("u"))
         .def("updateStiffnessMatrices", [](MG &mg) {
             check(vf_mg_update_stiffness(mg.mg));
@@ -1314,16 +1393,16 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def("solve", [](MG &mg, const Arr &u, const Arr &f, size_t numSteps, size_t nu, bool stiffnessUpdated,
                          bool zeroDirichlet, std::function<void(size_t, py::array_t<double>)> cb, bool fmg) {
                 const size_t nn = mg.tps->nn;
-                auto x = vfield_to_soa(u, nn), fs = vfield_to_soa(f, nn);
-                struct Ctx { std::function<void(size_t, py::array_t<double>)> *cb; size_t nn; } ctx{&cb, nn};
+                auto x = vfield_to_soa(u, nn, mg.tps->N), fs = vfield_to_soa(f, nn, mg.tps->N);
+                struct Ctx { std::function<void(size_t, py::array_t<double>)> *cb; size_t nn; int N; } ctx{&cb, nn, mg.tps->N};
                 auto tr = [](void *user, int64_t it, const double *xx) -> int {
                     auto *c = (Ctx *)user;
-                    (*c->cb)((size_t)it, soa_to_vfield(xx, c->nn));
+                    (*c->cb)((size_t)it, soa_to_vfield(xx, c->nn, c->N));
                     return 0;
                 };
                 check(vf_mg_solve(mg.mg, x.data(), fs.data(), numSteps, nu, stiffnessUpdated ? 1 : 0, zeroDirichlet ? 1 : 0,
                                   fmg ? 1 : 0, cb ? +tr : nullptr, &ctx, VF_HOST));
-                return soa_to_vfield(x.data(), nn);
+                return soa_to_vfield(x.data(), nn, mg.tps->N);
             }, py::arg("u"), py::arg("f"), py::arg("numSteps"), py::arg("numSmoothingSteps"),
             py::arg("stiffnessUpdated") = false, py::arg("zeroDirichlet") = false, py::arg("it_callback") = nullptr,
             py::arg("fullMultigrid") = false)
@@ -1343,21 +1422,21 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
                 std::function<void(size_t, py::array_t<double>, py::array_t<double>)> cb, size_t mgIterations,
                 size_t mgSmoothingIterations, bool fmg, const std::string &precision) {
                 const size_t nn = mg.tps->nn;
-                auto x = vfield_to_soa(u, nn), bs = vfield_to_soa(b, nn);
+                auto x = vfield_to_soa(u, nn, mg.tps->N), bs = vfield_to_soa(b, nn, mg.tps->N);
                 int prec;
                 if (precision == "fp64" || precision == "double") prec = VF_PREC_FP64;
                 else if (precision == "fp32" || precision == "float" || precision == "mixed") prec = VF_PREC_FP32;
                 else throw std::runtime_error("precision must be 'fp64' or 'fp32'");
-                struct Ctx { std::function<void(size_t, py::array_t<double>, py::array_t<double>)> *cb; size_t nn; } ctx{&cb, nn};
+                struct Ctx { std::function<void(size_t, py::array_t<double>, py::array_t<double>)> *cb; size_t nn; int N; } ctx{&cb, nn, mg.tps->N};
                 auto tr = [](void *user, int64_t it, const double *xx, const double *rr) -> int {
                     auto *c = (Ctx *)user;
-                    (*c->cb)((size_t)it, soa_to_vfield(xx, c->nn), soa_to_vfield(rr, c->nn));
+                    (*c->cb)((size_t)it, soa_to_vfield(xx, c->nn, c->N), soa_to_vfield(rr, c->nn, c->N));
                     return 0;
                 };
                 vf_pcg_stats st{};
                 check(vf_mg_pcg(mg.mg, x.data(), bs.data(), maxIter, tol, mgIterations, mgSmoothingIterations, fmg ? 1 : 0,
                                 prec, 0, cb ? +tr : nullptr, &ctx, VF_HOST, &st));
-                return soa_to_vfield(x.data(), nn);
+                return soa_to_vfield(x.data(), nn, mg.tps->N);
             },
              py::arg("u"), py::arg("b"), py::arg("maxIter"), py::arg("tol"), py::arg("it_callback") = nullptr,
              py::arg("mgIterations") = 1, py::arg("mgSmoothingIterations") = 1, py::arg("fullMultigrid") = false,
@@ -1366,50 +1445,50 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         // direct access to single hierarchy operations (C ABI), used by parity tests and drivers
         .def("_smooth", [](MG &mg, size_t l, const Arr &u, const Arr &b, bool forward) {
                 const size_t nn = mg.nodes(l);
-                auto us = vfield_to_soa(u, nn), bs = vfield_to_soa(b, nn);
+                auto us = vfield_to_soa(u, nn, mg.tps->N), bs = vfield_to_soa(b, nn, mg.tps->N);
                 check(vf_mg_smooth(mg.mg, l, us.data(), bs.data(), forward ? 1 : 0, VF_HOST));
-                return soa_to_vfield(us.data(), nn);
+                return soa_to_vfield(us.data(), nn, mg.tps->N);
             }, py::arg("l"), py::arg("u"), py::arg("b"), py::arg("forward") = true)
         .def("_restrict", [](MG &mg, size_t l, const Arr &r) {
                 const size_t nf = mg.nodes(l), nc = mg.nodes(l + 1);
-                auto rs = vfield_to_soa(r, nf);
-                std::vector<double> out(3 * nc);
+                auto rs = vfield_to_soa(r, nf, mg.tps->N);
+                std::vector<double> out((size_t)mg.tps->N * nc);
                 check(vf_mg_restrict(mg.mg, l, rs.data(), out.data(), VF_HOST));
-                return soa_to_vfield(out.data(), nc);
+                return soa_to_vfield(out.data(), nc, mg.tps->N);
             }, py::arg("l"), py::arg("r"))
         .def("_interpolate", [](MG &mg, size_t l, const Arr &xc, const Arr &xf, bool accumulate) {
                 const size_t nf = mg.nodes(l), nc = mg.nodes(l + 1);
-                auto cs = vfield_to_soa(xc, nc), fs = vfield_to_soa(xf, nf);
+                auto cs = vfield_to_soa(xc, nc, mg.tps->N), fs = vfield_to_soa(xf, nf, mg.tps->N);
                 check(vf_mg_interpolate(mg.mg, l, cs.data(), fs.data(), accumulate ? 1 : 0, VF_HOST));
-                return soa_to_vfield(fs.data(), nf);
+                return soa_to_vfield(fs.data(), nf, mg.tps->N);
             }, py::arg("l"), py::arg("xc"), py::arg("xf"), py::arg("accumulate") = false)
         .def("_coarseSolve", [](MG &mg, const Arr &b) {
                 const size_t nc = mg.nodes(mg.levels);
-                auto bs = vfield_to_soa(b, nc);
-                std::vector<double> x(3 * nc);
+                auto bs = vfield_to_soa(b, nc, mg.tps->N);
+                std::vector<double> x((size_t)mg.tps->N * nc);
                 check(vf_mg_coarse_solve(mg.mg, bs.data(), x.data(), VF_HOST));
-                return soa_to_vfield(x.data(), nc);
+                return soa_to_vfield(x.data(), nc, mg.tps->N);
             }, py::arg("b"))
         .def("_precondition", [](MG &mg, const Arr &r, size_t mgIterations, size_t nu, bool fmg, const std::string &prec) {
                 const size_t nn = mg.tps->nn;
-                auto rs = vfield_to_soa(r, nn);
-                std::vector<double> out(3 * nn);
+                auto rs = vfield_to_soa(r, nn, mg.tps->N);
+                std::vector<double> out((size_t)mg.tps->N * nn);
                 check(vf_mg_precondition(mg.mg, rs.data(), out.data(), mgIterations, nu, fmg ? 1 : 0,
                                          prec == "fp32" ? VF_PREC_FP32 : VF_PREC_FP64, VF_HOST));
-                return soa_to_vfield(out.data(), nn);
+                return soa_to_vfield(out.data(), nn, mg.tps->N);
             }, py::arg("r"), py::arg("mgIterations") = 1, py::arg("nu") = 1, py::arg("fullMultigrid") = false,
             py::arg("precision") = "fp64")
         .def("debug_get_x", [](MG &mg, int l) {
                 const size_t nn = mg.nodes(l);
-                std::vector<double> v(3 * nn);
+                std::vector<double> v((size_t)mg.tps->N * nn);
                 check(vf_mg_debug_get(mg.mg, l, 0, v.data(), VF_HOST));
-                return soa_to_vfield(v.data(), nn);
+                return soa_to_vfield(v.data(), nn, mg.tps->N);
             }, py::arg("l"))
         .def("debug_get_b", [](MG &mg, int l) {
                 const size_t nn = mg.nodes(l);
-                std::vector<double> v(3 * nn);
+                std::vector<double> v((size_t)mg.tps->N * nn);
                 check(vf_mg_debug_get(mg.mg, l, 1, v.data(), VF_HOST));
-                return soa_to_vfield(v.data(), nn);
+                return soa_to_vfield(v.data(), nn, mg.tps->N);
             }, py::arg("l"))
         .def("debugMulticolorVisit", [](MG &mg) {
                 auto &t = *mg.tps;
@@ -1429,8 +1508,8 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
     py::class_<Filter, FilterPtr>(detail, "Filter")
         .def("setInputDimensions", [](Filter &f, const std::vector<int64_t> &d) { f.setDims(d); }, py::arg("gridDims"))
         .def("setOutputDimensions", [](Filter &f, const std::vector<int64_t> &d) { f.setDims(d); }, py::arg("gridDims"))
-        .def_property_readonly("inputDimensions", [](const Filter &f) { return f.dims; })
-        .def_property_readonly("outputDimensions", [](const Filter &f) { return f.dims; })
+        .def_property_readonly("inputDimensions", [](const Filter &f) { return f.gridDims(); })
+        .def_property_readonly("outputDimensions", [](const Filter &f) { return f.gridDims(); })
         .def("apply", [](Filter &f, const Arr &x) {
                 f.checkDims();
                 DevVec in, out;
@@ -1444,8 +1523,8 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def(py::init<std::vector<FilterPtr>, const std::vector<int64_t> &>(), py::arg("filters"), py::arg("outGridDimensions"))
         .def("numVars", &FilterChain::size)
         .def("numPhysicalVars", &FilterChain::size)
-        .def("gridDims", [](const FilterChain &c) { return c.dims; })
-        .def("physicalGridDims", [](const FilterChain &c) { return c.dims; })
+        .def("gridDims", [](const FilterChain &c) { return std::vector<int64_t>(c.dims.begin(), c.dims.begin() + c.nd); })
+        .def("physicalGridDims", [](const FilterChain &c) { return std::vector<int64_t>(c.dims.begin(), c.dims.begin() + c.nd); })
         .def("setDesignVars", [](FilterChain &c, const Arr &x) {
                 auto v = to_vec(x, c.size(), "design variable");
                 DevVec d; d.upload(v.data(), v.size());
@@ -1495,8 +1574,8 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
     py::class_<ComplianceObjective, Objective, std::shared_ptr<ComplianceObjective>>(detail, "ComplianceObjective1_1_1")
         .def("gradient", [](ComplianceObjective &o) { DevVec g; o.gradient(g); return from_vec(g.host()); })
         .def("compliance", &ComplianceObjective::compliance)
-        .def("u", [](ComplianceObjective &o) { auto h = o.u.host(); return soa_to_vfield(h.data(), o.tps->nn); })
-        .def("f", [](ComplianceObjective &o) { auto h = o.f.host(); return soa_to_vfield(h.data(), o.tps->nn); });
+        .def("u", [](ComplianceObjective &o) { auto h = o.u.host(); return soa_to_vfield(h.data(), o.tps->nn, o.tps->N); })
+        .def("f", [](ComplianceObjective &o) { auto h = o.f.host(); return soa_to_vfield(h.data(), o.tps->nn, o.tps->N); });
     py::class_<MultigridComplianceObjective, ComplianceObjective, std::shared_ptr<MultigridComplianceObjective>>(
         detail, "MultigridComplianceObjective1_1_1")
         .def_readonly("mg", &MultigridComplianceObjective::mg)
@@ -1570,6 +1649,10 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
     // ------------------------------------------------------------------ factories (VoxelFEM.cc:221-298, 423-430)
     m.def("TensorProductSimulator", [](const std::vector<size_t> &degrees, const std::array<std::vector<double>, 2> &bbox,
                                        const std::vector<int64_t> &elems, NumberType nt) {
+            if (degrees == std::vector<size_t>{1, 1} && nt == NumberType::DOUBLE && elems.size() == 2) {
+                if (bbox[0].size() != 2 || bbox[1].size() != 2) throw std::runtime_error("domainBBox must hold two 2D corners");
+                return makeTPS({bbox[0][0], bbox[0][1], 0.0}, {bbox[1][0], bbox[1][1], 1.0}, {elems[0], elems[1], 1}, 2);
+            }
             if (degrees != std::vector<size_t>{1, 1, 1} || nt != NumberType::DOUBLE || elems.size() != 3)
                 throw std::runtime_error("No template instantiation matching degreesPerDimension/number type!");
             if (bbox[0].size() != 3 || bbox[1].size() != 3) throw std::runtime_error("domainBBox must hold two 3D corners");
@@ -1585,7 +1668,15 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
           py::arg("mg_solver"));
     m.def("LayerByLayerEvaluator", [](TPSPtr t) { return std::make_unique<LBL>(t); }, py::arg("lblSim"));
     m.def("OCOptimizer", [](Problem &p) { return std::make_unique<OCOptimizer>(p); }, py::keep_alive<0, 1>());
-    m.def("getClassName", [](py::object, const std::string &name) { return "pyVoxelFEM.detail." + name + "1_1_1"; },
-          py::arg("simulator"), py::arg("name"));
+    m.def("getClassName", [](TPSPtr t, const std::string &name) {
+            return "pyVoxelFEM.detail." + name + (t && t->N == 2 ? "1_1" : "1_1_1");
+        }, py::arg("simulator"), py::arg("name"));
+    // the planar instantiation's classes: the same Python types (one implementation serves both dimensions)
+    for (const char *name : {"TensorProductSimulator", "MultigridSolver", "Objective", "ComplianceObjective",
+                             "MultigridComplianceObjective", "TopologyOptimizationProblem", "OCOptimizer",
+                             "LayerByLayerEvaluator"}) {
+        const std::string n3 = std::string(name) + "1_1_1";
+        if (py::hasattr(detail, n3.c_str())) detail.attr((std::string(name) + "1_1").c_str()) = detail.attr(n3.c_str());
+    }
     m.def("backendVersion", []() { return std::string(vf_version()); });
 }

@@ -8,6 +8,7 @@
 // V-cycle / full multigrid preconditioner and the PCG iteration entirely on the device; the host
 // reads back one scalar per PCG iteration for the stopping test.
 #include "vf_internal.hpp"
+#include "vf_planar.hpp"
 #include "../../include/voxelfem.h"
 
 #include <algorithm>
@@ -130,6 +131,8 @@ void coarsen_into(const double *phi, int fi, const double *K, double *out) {
 // =================================================================================================
 struct vf_sim {
     hipStream_t stream = nullptr;
+    // planar (2D) simulator: when set, every entry point works on it and the 3D state below stays empty
+    std::unique_ptr<vf::planar::PSim> p2;
     int64_t ne[3];
     double lo[3], hi[3], h[3];
     Grid g;
@@ -401,6 +404,7 @@ struct StackedBatch;  // a stacked layer-by-layer batch (see lbl_stacked_batch)
 struct vf_mg {
     vf_sim *sim;
     int L;
+    std::unique_ptr<vf::planar::PMG> p2;  // planar hierarchy (the 3D state below stays empty)
     std::vector<std::unique_ptr<LevelData>> lv;  // lv[0] mirrors the fine simulator
     double phi[512];
     double Chat[8 * 576];
@@ -565,6 +569,7 @@ struct vf_mg {
     vf_mg(const vf_mg &) = delete;
     vf_mg &operator=(const vf_mg &) = delete;
 
+    vf_mg(vf_sim *s, std::unique_ptr<vf::planar::PMG> p) : sim(s), L(p->L), p2(std::move(p)) {}
     vf_mg(vf_sim *s, int levels) : sim(s), L(levels) {
         if (levels < 0) throw std::runtime_error("numCoarseningLevels must be >= 0");
         auto l0 = std::make_unique<LevelData>();
@@ -1710,6 +1715,24 @@ void need(bool ok, const char *msg) {
 }
 
 // Run `fn(dev_in..., dev_out)` with staging for host pointers.
+// host-pointer staging of the planar entry points (slots 0..2 of the simulator's staging buffers)
+struct P2Stage {
+    vf::planar::PSim &s;
+    int where;
+    const double *in(const double *p, size_t n, int slot) {
+        if (where == VF_DEVICE) return p;
+        double *d = s.stage(slot, n);
+        h2d(d, p, n, s.stream);
+        return d;
+    }
+    double *inout(double *p, size_t n, int slot) { return const_cast<double *>(in(p, n, slot)); }
+    double *out(double *p, size_t n, int slot) { return where == VF_DEVICE ? p : s.stage(slot, n); }
+    void finish(double *p, const double *d, size_t n) {
+        if (where == VF_HOST) d2h(p, d, n, s.stream);
+        sync(s.stream);
+    }
+};
+
 struct Staged {
     vf_sim *sim;
     int where;
@@ -1879,6 +1902,33 @@ int vf_sim_create(vf_sim **out, const int64_t ne[3], const double minCorner[3], 
     VF_API_END
 }
 
+int vf_sim_create_2d(vf_sim **out, const int64_t ne[2], const double minCorner[2], const double maxCorner[2],
+                     void *stream) {
+    VF_API_BEGIN
+    need(out && ne && minCorner && maxCorner, "null argument");
+    *out = nullptr;
+    auto s = std::make_unique<vf_sim>();
+    s->stream = (hipStream_t)stream;
+    s->p2 = std::make_unique<vf::planar::PSim>(ne, minCorner, maxCorner, s->stream);
+    for (int d = 0; d < 2; ++d) {
+        s->ne[d] = ne[d];
+        s->lo[d] = minCorner[d];
+        s->hi[d] = maxCorner[d];
+        s->h[d] = s->p2->h[d];
+    }
+    s->ne[2] = 0;
+    s->lo[2] = s->hi[2] = s->h[2] = 0;
+    *out = s.release();
+    VF_API_END
+}
+
+int vf_sim_dimension(vf_sim *sim, int *dim) {
+    VF_API_BEGIN
+    need(sim && dim, "null argument");
+    *dim = sim->p2 ? 2 : 3;
+    VF_API_END
+}
+
 int vf_sim_destroy(vf_sim *sim) {
     VF_API_BEGIN
     if (sim) { (void)hipStreamSynchronize(sim->stream); delete sim; }
@@ -1890,6 +1940,7 @@ int vf_sim_set_stream(vf_sim *sim, void *stream) {
     need(sim, "null sim");
     sync(sim->stream);
     sim->stream = (hipStream_t)stream;
+    if (sim->p2) sim->p2->stream = sim->stream;
     VF_API_END
 }
 
@@ -1897,6 +1948,7 @@ int vf_sim_set_material(vf_sim *sim, double young, double poisson) {
     VF_API_BEGIN
     need(sim, "null sim");
     need(young > 0 && poisson > -1 && poisson < 0.5, "invalid isotropic material");
+    if (sim->p2) { sim->p2->set_material(young, poisson); return VF_OK; }
     sim->young = young;
     sim->poisson = poisson;
     sim->update_K0();
@@ -1909,6 +1961,7 @@ int vf_sim_set_law(vf_sim *sim, int law, double E0, double Emin, double gamma, d
     VF_API_BEGIN
     need(sim, "null sim");
     need(law == VF_SIMP || law == VF_RAMP, "unknown interpolation law");
+    if (sim->p2) { sim->p2->set_law(law, E0, Emin, gamma, q); return VF_OK; }
     sim->law = law;
     sim->E0 = E0;
     sim->Emin = Emin;
@@ -1921,6 +1974,11 @@ int vf_sim_set_law(vf_sim *sim, int law, double E0, double Emin, double gamma, d
 int vf_sim_set_densities(vf_sim *sim, const double *rho, int where) {
     VF_API_BEGIN
     need(sim && rho, "null argument");
+    if (sim->p2) {
+        sim->p2->set_densities(rho, where == VF_DEVICE);
+        if (where == VF_HOST) sync(sim->stream);
+        return VF_OK;
+    }
     if (where == VF_DEVICE) d2d(sim->rho.data(), rho, sim->nel(), sim->stream);
     else h2d(sim->rho.data(), rho, sim->nel(), sim->stream);
     sim->update_E();
@@ -1931,6 +1989,12 @@ int vf_sim_set_densities(vf_sim *sim, const double *rho, int where) {
 int vf_sim_get_densities(vf_sim *sim, double *rho, int where) {
     VF_API_BEGIN
     need(sim && rho, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        if (where == VF_DEVICE) d2d(rho, p2.rho.data(), p2.nel(), p2.stream);
+        else { d2h(rho, p2.rho.data(), p2.nel(), p2.stream); sync(p2.stream); }
+        return VF_OK;
+    }
     if (where == VF_DEVICE) d2d(rho, sim->rho.data(), sim->nel(), sim->stream);
     else { d2h(rho, sim->rho.data(), sim->nel(), sim->stream); sync(sim->stream); }
     VF_API_END
@@ -1939,6 +2003,12 @@ int vf_sim_get_densities(vf_sim *sim, double *rho, int where) {
 int vf_sim_get_young(vf_sim *sim, double *E, int where) {
     VF_API_BEGIN
     need(sim && E, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        if (where == VF_DEVICE) d2d(E, p2.E.data(), p2.nel(), p2.stream);
+        else { d2h(E, p2.E.data(), p2.nel(), p2.stream); sync(p2.stream); }
+        return VF_OK;
+    }
     if (where == VF_DEVICE) d2d(E, sim->E.data(), sim->nel(), sim->stream);
     else { d2h(E, sim->E.data(), sim->nel(), sim->stream); sync(sim->stream); }
     VF_API_END
@@ -1947,6 +2017,7 @@ int vf_sim_get_young(vf_sim *sim, double *E, int where) {
 int vf_sim_get_K0(vf_sim *sim, double *K0) {
     VF_API_BEGIN
     need(sim && K0, "null argument");
+    if (sim->p2) { std::copy(sim->p2->K0h, sim->p2->K0h + 64, K0); return VF_OK; }
     std::copy(sim->K0h, sim->K0h + 576, K0);
     VF_API_END
 }
@@ -1954,6 +2025,7 @@ int vf_sim_get_K0(vf_sim *sim, double *K0) {
 int vf_sim_set_dirichlet(vf_sim *sim, const uint8_t *mask, const double *values) {
     VF_API_BEGIN
     need(sim && mask, "null argument");
+    if (sim->p2) { sim->p2->set_dirichlet(mask, values); return VF_OK; }
     const size_t nn = sim->nn();
     for (size_t n = 0; n < nn; ++n) sim->hmask[n] = mask[n] & 7;
     for (size_t i = 0; i < 3 * nn; ++i) sim->hvals[i] = values ? values[i] : 0.0;
@@ -1975,6 +2047,7 @@ int vf_sim_set_dirichlet(vf_sim *sim, const uint8_t *mask, const double *values)
 int vf_sim_set_forces(vf_sim *sim, const double *f) {
     VF_API_BEGIN
     need(sim && f, "null argument");
+    if (sim->p2) { sim->p2->set_forces(f); return VF_OK; }
     std::copy(f, f + 3 * sim->nn(), sim->hforce.begin());
     sim->hasForce = std::any_of(sim->hforce.begin(), sim->hforce.end(), [](double v) { return v != 0.0; });
     sim->force.resize(3 * sim->nn());
@@ -1986,6 +2059,7 @@ int vf_sim_set_forces(vf_sim *sim, const double *f) {
 int vf_sim_set_gravity(vf_sim *sim, const double g[3]) {
     VF_API_BEGIN
     need(sim && g, "null argument");
+    if (sim->p2) { sim->p2->grav[0] = g[0]; sim->p2->grav[1] = g[1]; return VF_OK; }
     for (int c = 0; c < 3; ++c) sim->grav[c] = g[c];
     VF_API_END
 }
@@ -1993,6 +2067,7 @@ int vf_sim_set_gravity(vf_sim *sim, const double g[3]) {
 int vf_sim_set_mask_layer(vf_sim *sim, int64_t layer) {
     VF_API_BEGIN
     need(sim, "null sim");
+    if (sim->p2) { sim->p2->set_mask_height(sim->p2->h[1] * (double)layer, true); return VF_OK; }
     sim->set_mask_height(sim->h[1] * (double)layer, true);
     VF_API_END
 }
@@ -2000,6 +2075,13 @@ int vf_sim_set_mask_layer(vf_sim *sim, int64_t layer) {
 int vf_sim_get_mask_info(vf_sim *sim, int64_t *firstMasked, int64_t *firstDetached, double *height) {
     VF_API_BEGIN
     need(sim, "null sim");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        if (firstMasked) *firstMasked = p2.firstMasked;
+        if (firstDetached) *firstDetached = p2.firstDetached;
+        if (height) *height = p2.maskHeight;
+        return VF_OK;
+    }
     if (firstMasked) *firstMasked = sim->firstMasked;
     if (firstDetached) *firstDetached = sim->firstDetached;
     if (height) *height = sim->maskHeight;
@@ -2009,6 +2091,16 @@ int vf_sim_get_mask_info(vf_sim *sim, int64_t *firstMasked, int64_t *firstDetach
 int vf_sim_apply_K(vf_sim *sim, const double *u, double *y, int where) {
     VF_API_BEGIN
     need(sim && u && y, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        const size_t n = p2.ndof();
+        P2Stage st{p2, where};
+        const double *du = st.in(u, n, 0);
+        double *dy = st.out(y, n, 1);
+        p2.apply_K(du, dy);
+        st.finish(y, dy, n);
+        return VF_OK;
+    }
     sim->refresh_grid();
     const size_t n = 3 * sim->nn();
     Staged st{sim, where};
@@ -2022,6 +2114,15 @@ int vf_sim_apply_K(vf_sim *sim, const double *u, double *y, int where) {
 int vf_sim_build_load(vf_sim *sim, double *f, int where) {
     VF_API_BEGIN
     need(sim && f, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        const size_t n = p2.ndof();
+        P2Stage st{p2, where};
+        double *df = st.out(f, n, 1);
+        p2.build_load(df);
+        st.finish(f, df, n);
+        return VF_OK;
+    }
     sim->refresh_grid();
     const size_t n = 3 * sim->nn();
     Staged st{sim, where};
@@ -2037,6 +2138,17 @@ int vf_sim_build_load(vf_sim *sim, double *f, int where) {
 static int grad_impl(vf_sim *sim, const double *u, double *g, int where, int accumulate) {
     VF_API_BEGIN
     need(sim && u && g, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        const size_t n = p2.ndof(), ne = p2.nel();
+        P2Stage st{p2, where};
+        const double *du = st.in(u, n, 0);
+        double *dg = where == VF_DEVICE ? g : p2.stage(1, ne);
+        if (where == VF_HOST && accumulate) h2d(dg, g, ne, p2.stream);
+        p2.compliance_gradient(du, dg, accumulate != 0);
+        st.finish(g, dg, ne);
+        return VF_OK;
+    }
     sim->refresh_grid();
     const size_t n = 3 * sim->nn(), ne = sim->nel();
     Staged st{sim, where};
@@ -2058,6 +2170,16 @@ int vf_sim_accumulate_compliance_gradient(vf_sim *sim, const double *u, double *
 int vf_sim_energy_density(vf_sim *sim, const double *u, double *out, int where) {
     VF_API_BEGIN
     need(sim && u && out, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        const size_t n = p2.ndof(), ne = p2.nel();
+        P2Stage st{p2, where};
+        const double *du = st.in(u, n, 0);
+        double *dout = st.out(out, ne, 1);
+        p2.energy_density(du, dout);
+        st.finish(out, dout, ne);
+        return VF_OK;
+    }
     const size_t n = 3 * sim->nn(), ne = sim->nel();
     Staged st{sim, where};
     const double *du = st.in(u, n, sim->stage_in(n));
@@ -2070,6 +2192,15 @@ int vf_sim_energy_density(vf_sim *sim, const double *u, double *out, int where) 
 int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int where, double *out) {
     VF_API_BEGIN
     need(sim && a && b && out, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        const size_t n = p2.ndof();
+        P2Stage st{p2, where};
+        const double *da = st.in(a, n, 0);
+        const double *db = st.in(b, n, 1);
+        *out = p2.dot(da, db, masked != 0);
+        return VF_OK;
+    }
     sim->refresh_grid();
     const size_t n = 3 * sim->nn();
     Staged st{sim, where};
@@ -2089,6 +2220,17 @@ int vf_sim_dot(vf_sim *sim, const double *a, const double *b, int masked, int wh
 int vf_sim_constant_strain_load(vf_sim *sim, const double eps[9], double *F, int where) {
     VF_API_BEGIN
     need(sim && eps && F, "null argument");
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        need(std::fabs(eps[1] - eps[2]) <= 1e-12 * (std::fabs(eps[1]) + std::fabs(eps[2]) + 1e-300),
+             "constantStrainLoad: the strain must be a symmetric 2x2 matrix");
+        const size_t n = p2.ndof();
+        P2Stage st{p2, where};
+        double *dF = st.out(F, n, 1);
+        p2.constant_strain_load(eps, dF);
+        st.finish(F, dF, n);
+        return VF_OK;
+    }
     for (int a = 0; a < 3; ++a)
         for (int b = a + 1; b < 3; ++b)
             need(std::fabs(eps[3 * a + b] - eps[3 * b + a]) <= 1e-12 * (std::fabs(eps[3 * a + b]) + std::fabs(eps[3 * b + a]) + 1e-300),
@@ -2118,6 +2260,29 @@ int vf_sim_sample_nodal_field(vf_sim *sim, const double *u, int64_t ncomp, const
     need(sim && u && p && out, "null argument");
     need(ncomp >= 1 && np >= 0 && np < INT_MAX / 3, "invalid sizes");
     if (np == 0) return VF_OK;
+    if (sim->p2) {
+        auto &p2 = *sim->p2;
+        hipStream_t s = p2.stream;
+        const size_t nu = (size_t)ncomp * p2.nn(), npo = (size_t)ncomp * np;
+        DBuf<double> hu, hp, ho;
+        const double *du = u, *dp = p;
+        double *dout = out;
+        if (where == VF_HOST) {
+            hu.resize(nu); hp.resize(2 * (size_t)np); ho.resize(npo);
+            h2d(hu.data(), u, nu, s);
+            h2d(hp.data(), p, 2 * (size_t)np, s);
+            du = hu.data(); dp = hp.data(); dout = ho.data();
+        }
+        DBuf<int> bad(1);
+        launch_fill<int>(1, bad.data(), 0, s);
+        p2.sample(du, (int)ncomp, dp, (int)np, dout, bad.data());
+        int flag = 0;
+        d2h(&flag, bad.data(), 1, s);
+        if (where == VF_HOST) d2h(out, dout, npo, s);
+        sync(s);
+        if (flag) throw std::runtime_error("Point out of bounds");
+        return VF_OK;
+    }
     hipStream_t s = sim->stream;
     const size_t nu = (size_t)ncomp * sim->nn(), npo = (size_t)ncomp * np;
     DBuf<double> hu, hp, ho;
@@ -2143,6 +2308,7 @@ int vf_sim_sample_nodal_field(vf_sim *sim, const double *u, int64_t ncomp, const
 int vf_sim_set_stretchings(vf_sim *sim, const double h[3]) {
     VF_API_BEGIN
     need(sim && h, "null argument");
+    if (sim->p2) { sim->p2->set_stretchings(h); return VF_OK; }
     for (int d = 0; d < 3; ++d) need(h[d] > 0, "stretchings must be positive");
     std::copy(h, h + 3, sim->h);
     sim->update_K0();
@@ -2154,6 +2320,7 @@ int vf_sim_set_stretchings(vf_sim *sim, const double h[3]) {
 int vf_sim_get_stretchings(vf_sim *sim, double h[3]) {
     VF_API_BEGIN
     need(sim && h, "null argument");
+    if (sim->p2) { h[0] = sim->p2->h[0]; h[1] = sim->p2->h[1]; h[2] = 0.0; return VF_OK; }
     std::copy(sim->h, sim->h + 3, h);
     VF_API_END
 }
@@ -2163,6 +2330,10 @@ int vf_mg_create(vf_mg **out, vf_sim *fine, int64_t levels) {
     VF_API_BEGIN
     need(out && fine, "null argument");
     *out = nullptr;
+    if (fine->p2) {
+        *out = new vf_mg(fine, std::make_unique<vf::planar::PMG>(fine->p2.get(), (int)levels));
+        return VF_OK;
+    }
     *out = new vf_mg(fine, (int)levels);
     VF_API_END
 }
@@ -2184,6 +2355,12 @@ int vf_mg_level_nodes(vf_mg *mg, int64_t l, int64_t nn[3]) {
     VF_API_BEGIN
     need(mg && nn, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
+    if (mg->p2) {
+        nn[0] = mg->p2->lv[l]->g.nn[0];
+        nn[1] = mg->p2->lv[l]->g.nn[1];
+        nn[2] = 1;
+        return VF_OK;
+    }
     for (int d = 0; d < 3; ++d) nn[d] = mg->lv[l]->g.nn[d];
     VF_API_END
 }
@@ -2192,6 +2369,11 @@ int vf_mg_level_dirichlet(vf_mg *mg, int64_t l, uint8_t *mask) {
     VF_API_BEGIN
     need(mg && mask, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
+    if (mg->p2) {
+        const auto &m = l == 0 ? mg->p2->sim->hmask : mg->p2->lv[l]->hmask;
+        std::copy(m.begin(), m.end(), mask);
+        return VF_OK;
+    }
     const auto &m = l == 0 ? mg->sim->hmask : mg->lv[l]->hmask;
     std::copy(m.begin(), m.end(), mask);
     VF_API_END
@@ -2201,12 +2383,14 @@ int vf_mg_set_symmetric_gs(vf_mg *mg, int symmetric) {
     VF_API_BEGIN
     need(mg, "null mg");
     mg->symmetricGS = symmetric != 0;
+    if (mg->p2) mg->p2->symmetricGS = symmetric != 0;
     VF_API_END
 }
 
 int vf_mg_update_stiffness(vf_mg *mg) {
     VF_API_BEGIN
     need(mg, "null mg");
+    if (mg->p2) { mg->p2->update_stiffness(); return VF_OK; }
     // asynchronous on the solver's stream: later calls are ordered after it, and a failed factorisation
     // is reported by the first coarse solve
     mg->update_stiffness();
@@ -2216,6 +2400,7 @@ int vf_mg_update_stiffness(vf_mg *mg) {
 int vf_mg_finish_update(vf_mg *mg) {
     VF_API_BEGIN
     need(mg, "null mg");
+    if (mg->p2) { sync(mg->p2->st()); return VF_OK; }
     mg->finish_factor();
     VF_API_END
 }
@@ -2223,6 +2408,13 @@ int vf_mg_finish_update(vf_mg *mg) {
 int vf_mg_set_mask_layer(vf_mg *mg, int64_t layer) {
     VF_API_BEGIN
     need(mg, "null mg");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.sim->set_mask_height(m2.sim->h[1] * (double)layer, true);
+        m2.sync_masks();
+        m2.stiffReady = false;
+        return VF_OK;
+    }
     mg->sim->set_mask_height(mg->sim->h[1] * (double)layer, true);
     mg->sync_masks();
     mg->stiffnessReady = false;
@@ -2235,6 +2427,18 @@ int vf_mg_apply_K(vf_mg *mg, int64_t l, const double *u, double *y, int where) {
     VF_API_BEGIN
     need(mg && u && y, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.sync_masks();
+        if (l >= 1) m2.ensure_stiffness();
+        const size_t n = m2.level_n((int)l);
+        P2Stage st{*m2.sim, where};
+        const double *du = st.in(u, n, 0);
+        double *dy = st.out(y, n, 1);
+        m2.apply_K((int)l, du, dy, true, false);
+        st.finish(y, dy, n);
+        return VF_OK;
+    }
     mg->sync_masks();
     if (l >= 1) mg->ensure_blocks<double>();
     const size_t n = level_n(mg, l);
@@ -2251,6 +2455,20 @@ int vf_mg_residual(vf_mg *mg, int64_t l, const double *u, const double *b, doubl
     VF_API_BEGIN
     need(mg && u && b && r, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.sync_masks();
+        if (l >= 1) m2.ensure_stiffness();
+        const size_t n = m2.level_n((int)l);
+        P2Stage st{*m2.sim, where};
+        const double *du = st.in(u, n, 0);
+        const double *db = st.in(b, n, 1);
+        double *dr = st.out(r, n, 2);
+        launch_fill<double>(n, dr, 0.0, m2.st());
+        m2.residual((int)l, du, db, dr);
+        st.finish(r, dr, n);
+        return VF_OK;
+    }
     mg->sync_masks();
     if (l >= 1) mg->ensure_blocks<double>();
     const size_t n = level_n(mg, l);
@@ -2268,6 +2486,18 @@ int vf_mg_smooth(vf_mg *mg, int64_t l, double *u, const double *b, int forward, 
     VF_API_BEGIN
     need(mg && u && b, "null argument");
     need(l >= 0 && l < mg->L, "smoothing needs a non-coarsest level");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.sync_masks();
+        if (l >= 1) m2.ensure_stiffness();
+        const size_t n = m2.level_n((int)l);
+        P2Stage st{*m2.sim, where};
+        double *du = st.inout(u, n, 0);
+        const double *db = st.in(b, n, 1);
+        m2.smooth((int)l, du, db, forward != 0);
+        st.finish(u, du, n);
+        return VF_OK;
+    }
     mg->sync_masks();
     if (l >= 1) mg->ensure_blocks<double>();
     const size_t n = level_n(mg, l);
@@ -2284,6 +2514,17 @@ int vf_mg_restrict(vf_mg *mg, int64_t l, const double *fine, double *coarse, int
     VF_API_BEGIN
     need(mg && fine && coarse, "null argument");
     need(l >= 0 && l < mg->L, "level out of range");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.sync_masks();
+        const size_t nf = m2.level_n((int)l), nc = m2.level_n((int)l + 1);
+        P2Stage st{*m2.sim, where};
+        const double *df = st.in(fine, nf, 0);
+        double *dc = st.out(coarse, nc, 1);
+        m2.restrict_to((int)l, df, dc);
+        st.finish(coarse, dc, nc);
+        return VF_OK;
+    }
     mg->sync_masks();
     const size_t nf = level_n(mg, l), nc = level_n(mg, l + 1);
     Staged st{mg->sim, where};
@@ -2298,6 +2539,17 @@ int vf_mg_interpolate(vf_mg *mg, int64_t l, const double *coarse, double *fine, 
     VF_API_BEGIN
     need(mg && fine && coarse, "null argument");
     need(l >= 0 && l < mg->L, "level out of range");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.sync_masks();
+        const size_t nf = m2.level_n((int)l), nc = m2.level_n((int)l + 1);
+        P2Stage st{*m2.sim, where};
+        const double *dc = st.in(coarse, nc, 0);
+        double *df = st.inout(fine, nf, 1);
+        m2.interpolate((int)l, dc, df, accumulate != 0);
+        st.finish(fine, df, nf);
+        return VF_OK;
+    }
     mg->sync_masks();
     const size_t nf = level_n(mg, l), nc = level_n(mg, l + 1);
     Staged st{mg->sim, where};
@@ -2312,6 +2564,17 @@ int vf_mg_interpolate(vf_mg *mg, int64_t l, const double *coarse, double *fine, 
 int vf_mg_coarse_solve(vf_mg *mg, const double *b, double *x, int where) {
     VF_API_BEGIN
     need(mg && b && x, "null argument");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        m2.ensure_stiffness();
+        const size_t n = m2.level_n(m2.L);
+        P2Stage st{*m2.sim, where};
+        const double *db = st.in(b, n, 0);
+        double *dx = st.out(x, n, 1);
+        m2.coarse_solve(db, dx);
+        st.finish(x, dx, n);
+        return VF_OK;
+    }
     mg->sync_masks();
     mg->ensure_stiffness();
     const size_t n = level_n(mg, mg->L);
@@ -2327,6 +2590,15 @@ int vf_mg_zero_dirichlet(vf_mg *mg, int64_t l, double *u, int where) {
     VF_API_BEGIN
     need(mg && u, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        const size_t n = m2.level_n((int)l);
+        P2Stage st{*m2.sim, where};
+        double *du = st.inout(u, n, 0);
+        m2.zero_dirichlet((int)l, du);
+        st.finish(u, du, n);
+        return VF_OK;
+    }
     const size_t n = level_n(mg, l);
     double *du = where == VF_DEVICE ? u : mg->sim->stage_out(n);
     if (where == VF_HOST) h2d(du, u, n, mg->st());
@@ -2361,6 +2633,15 @@ int vf_mg_debug_get(vf_mg *mg, int64_t l, int which, double *out, int where) {
     need(mg && out, "null argument");
     need(l >= 0 && l <= mg->L, "level out of range");
     need(which == 0 || which == 1, "which must be 0 (x) or 1 (b)");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        const size_t n = m2.level_n((int)l);
+        const double *src = which ? m2.lv[l]->b.data() : m2.lv[l]->x.data();
+        if (where == VF_DEVICE) d2d(out, src, n, m2.st());
+        else d2h(out, src, n, m2.st());
+        sync(m2.st());
+        return VF_OK;
+    }
     if (mg->dbgPrec < 0) throw std::runtime_error("no multigrid cycle has run yet");
     const size_t n = level_n(mg, l);
     hipStream_t s = mg->st();
@@ -2387,6 +2668,17 @@ int vf_mg_solve(vf_mg *mg, double *x, const double *f, int64_t numSteps, int64_t
                 int zeroDirichlet, int fullMultigrid, vf_mg_callback cb, void *user, int where) {
     VF_API_BEGIN
     need(mg && x && f, "null argument");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        const size_t n = m2.level_n(0);
+        P2Stage st{*m2.sim, where};
+        double *dx = st.inout(x, n, 0);
+        const double *df = st.in(f, n, 1);
+        m2.solve(dx, df, (int)numSteps, (int)smoothing, stiffnessUpdated != 0, zeroDirichlet != 0, fullMultigrid != 0,
+                 cb, user);
+        st.finish(x, dx, n);
+        return VF_OK;
+    }
     const size_t n = level_n(mg, 0);
     hipStream_t s = mg->st();
     double *dx = x;
@@ -2411,6 +2703,17 @@ int vf_mg_precondition(vf_mg *mg, const double *r, double *s, int64_t mgIteratio
                        int fullMultigrid, int precision, int where) {
     VF_API_BEGIN
     need(mg && r && s, "null argument");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        need(precision == VF_PREC_FP64, "planar grids run the FP64 V-cycle only");
+        const size_t n = m2.level_n(0);
+        P2Stage st{*m2.sim, where};
+        const double *dr = st.in(r, n, 0);
+        double *ds = st.out(s, n, 1);
+        m2.precondition(dr, ds, (int)mgIterations, (int)smoothing, fullMultigrid != 0);
+        st.finish(s, ds, n);
+        return VF_OK;
+    }
     const size_t n = level_n(mg, 0);
     Staged st{mg->sim, where};
     // the preconditioner reads r while writing s: keep them in distinct staging buffers
@@ -2428,6 +2731,26 @@ int vf_mg_pcg(vf_mg *mg, double *x, const double *b, int64_t maxIter, double tol
     VF_API_BEGIN
     need(mg && x && b, "null argument");
     need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "unknown precision");
+    if (mg->p2) {
+        auto &m2 = *mg->p2;
+        need(precision == VF_PREC_FP64, "planar grids run the FP64 V-cycle only");
+        const size_t n = m2.level_n(0);
+        P2Stage st{*m2.sim, where};
+        double *dx = st.inout(x, n, 0);
+        const double *db = st.in(b, n, 1);
+        const auto t0 = std::chrono::steady_clock::now();
+        double rn = 0, bn = 0;
+        const int64_t its = m2.pcg(dx, db, maxIter, tol, (int)mgIterations, (int)smoothing, fullMultigrid != 0,
+                                   dirichletAlreadySatisfied != 0, cb, user, &rn, &bn);
+        st.finish(x, dx, n);
+        if (stats) {
+            stats->iterations = its;
+            stats->residual_norm = rn;
+            stats->rhs_norm = bn;
+            stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        }
+        return VF_OK;
+    }
     const size_t n = level_n(mg, 0);
     hipStream_t s = mg->st();
     double *dx = x;
@@ -2498,6 +2821,99 @@ static std::vector<double> sym_pinv_solve(std::vector<double> A, const std::vect
     return c;
 }
 
+// The layer-by-layer evaluation on a planar grid: the same steps as lbl_run below (LayerByLayer.hh:223-300 with
+// the guesses of LayerByLayer.hh:24-217) on the planar simulator and hierarchy.
+static void lbl_run_2d(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double tol, int64_t mgIterations,
+                       int64_t smoothing, int fullMultigrid, int precision, int method, int64_t history, bool zeroInit,
+                       double *uFull, bool uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
+                       double *perLayerCompliance, int64_t *perLayerIterations, vf_pcg_callback itcb, void *ituser,
+                       vf_lbl_callback lblcb, void *lbluser) {
+    need(precision == VF_PREC_FP64, "planar grids run the FP64 V-cycle only");
+    vf::planar::PMG &m = *mg->p2;
+    vf::planar::PSim &sim = *m.sim;
+    hipStream_t s = sim.stream;
+    const int64_t numLayers = sim.ne[1];
+    const size_t n = sim.ndof(), ne = sim.nel();
+    const int H = method == VF_LBL_INIT_ZERO ? 0 : (int)history;
+    DBuf<double> f(n), u(n), g(ne);
+    std::vector<DBuf<double>> hist(H), Ku(method == VF_LBL_INIT_SUBSPACE ? H : 0);
+    for (auto &h : hist) h.resize(n);
+    for (auto &k : Ku) k.resize(n);
+    std::deque<int> order;  // history slots, newest first
+    launch_fill<double>(ne, g.data(), 0.0, s);
+    sim.set_mask_height(sim.h[1] * (double)numLayers, true);
+    double total = 0;
+    int64_t count = 0;
+    for (int64_t l = numLayers; l > 0; l -= std::min(layerIncrement, l)) {
+        if (l < numLayers) sim.remove_top_layers(layerIncrement);
+        m.sync_masks();
+        sim.build_load(f.data());
+        const int hs = (int)order.size();
+        if (l == numLayers && !zeroInit && uFull && uFullValid) {
+            if (uFullWhere == VF_DEVICE) d2d(u.data(), uFull, n, s);
+            else h2d(u.data(), uFull, n, s);
+        } else if (hs == 0) {
+            launch_fill<double>(n, u.data(), 0.0, s);
+        } else if (method == VF_LBL_INIT_FD) {
+            d2d(u.data(), hist[order[0]].data(), n, s);
+            if (hs >= 2) {  // 2 u_0 - u_1
+                launch_axpy_plain(n, 1.0, hist[order[0]].data(), u.data(), s);
+                launch_axpy_plain(n, -1.0, hist[order[1]].data(), u.data(), s);
+            }
+        } else {  // subspace
+            for (int j = 0; j < hs; ++j) sim.apply_K(hist[order[j]].data(), Ku[j].data());
+            std::vector<double> A((size_t)hs * hs), bb(hs);
+            for (int i = 0; i < hs; ++i) {
+                bb[i] = sim.dot(hist[order[i]].data(), f.data(), true);
+                for (int j = 0; j <= i; ++j)
+                    A[(size_t)i * hs + j] = A[(size_t)j * hs + i] = sim.dot(hist[order[i]].data(), Ku[j].data(), true);
+            }
+            const std::vector<double> c = sym_pinv_solve(A, bb, hs);
+            launch_fill<double>(n, u.data(), 0.0, s);
+            for (int i = 0; i < hs; ++i) launch_axpy_plain(n, c[i], hist[order[i]].data(), u.data(), s);
+        }
+        sim.zero_detached(u.data());
+        const int64_t its = m.pcg(u.data(), f.data(), maxIter, tol, (int)mgIterations, (int)smoothing,
+                                  fullMultigrid != 0, true, itcb, ituser, nullptr, nullptr);
+        const double comp = sim.dot(f.data(), u.data(), true);
+        if (lblcb) {
+            DBuf<double> gl(ne);
+            sim.compliance_gradient(u.data(), gl.data(), false);
+            std::vector<double> hgl(ne), hu(n);
+            d2h(hgl.data(), gl.data(), ne, s);
+            d2h(hu.data(), u.data(), n, s);
+            if (lblcb(lbluser, l, comp, hgl.data(), hu.data())) throw std::runtime_error("layer-by-layer run aborted by callback");
+        }
+        if (perLayerCompliance) perLayerCompliance[count] = comp;
+        if (perLayerIterations) perLayerIterations[count] = its;
+        total += comp;
+        sim.compliance_gradient(u.data(), g.data(), true);
+        ++count;
+        if (l == numLayers && uFull) {
+            if (uFullWhere == VF_DEVICE) d2d(uFull, u.data(), n, s);
+            else d2h(uFull, u.data(), n, s);
+        }
+        if (H > 0 && l >= layerIncrement) {
+            int slot;
+            if ((int)order.size() < H) {
+                slot = (int)order.size();
+            } else {
+                slot = order.back();
+                order.pop_back();
+            }
+            d2d(hist[slot].data(), u.data(), n, s);
+            order.push_front(slot);
+        }
+    }
+    *objective = 0.5 * total / (double)count;
+    std::vector<double> hg(ne);
+    d2h(hg.data(), g.data(), ne, s);
+    for (auto &v : hg) v /= (double)count;
+    if (gradWhere == VF_DEVICE) h2d(grad, hg.data(), ne, s);
+    else std::copy(hg.begin(), hg.end(), grad);
+    sync(s);
+}
+
 // LayerByLayerEvaluator::run (LayerByLayer.hh:223-300) with its initial guesses (LayerByLayer.hh:24-217):
 // zero; the previous layers' solutions extrapolated (history 1: copied, history 2: 2 u_0 - u_1); or the
 // Galerkin projection of the next system onto the last N solutions, g = sum_i c_i u_i with
@@ -2509,6 +2925,12 @@ static void lbl_run(vf_mg *mg, int64_t layerIncrement, int64_t maxIter, double t
                     double *uFull, bool uFullValid, int uFullWhere, double *objective, double *grad, int gradWhere,
                     double *perLayerCompliance, int64_t *perLayerIterations, vf_pcg_callback itcb = nullptr,
                     void *ituser = nullptr, vf_lbl_callback lblcb = nullptr, void *lbluser = nullptr) {
+    if (mg->p2) {
+        lbl_run_2d(mg, layerIncrement, maxIter, tol, mgIterations, smoothing, fullMultigrid, precision, method, history,
+                   zeroInit, uFull, uFullValid, uFullWhere, objective, grad, gradWhere, perLayerCompliance,
+                   perLayerIterations, itcb, ituser, lblcb, lbluser);
+        return;
+    }
     vf_sim *sim = mg->sim;
     hipStream_t s = mg->st();
     const int64_t numLayers = sim->ne[1];
@@ -3120,6 +3542,21 @@ int vf_smoothing_filter(const int64_t dims[3], double rmin, int type, const doub
     VF_API_END
 }
 
+int vf_smoothing_filter_2d(const int64_t dims[2], double rmin, int type, const double *in, double *out, int where,
+                           void *stream) {
+    VF_API_BEGIN
+    need(dims && in && out, "null argument");
+    need(rmin > 0, "filter radius must be positive");
+    need(in != out, "smoothing filter cannot run in place");
+    need(dims[0] >= 1 && dims[1] >= 1 && dims[0] * dims[1] < INT_MAX, "invalid grid dimensions");
+    const int d[2] = {(int)dims[0], (int)dims[1]};
+    const size_t n = (size_t)d[0] * d[1];
+    with_buffers(n, n, in, out, where, as_stream(stream), [&](const double *i, const double *, double *o) {
+        vf::planar::launch_smoothing_filter_2d(d, rmin, type, i, o, as_stream(stream));
+    });
+    VF_API_END
+}
+
 int vf_projection(double beta, int64_t n, const double *in, double *out, int where, void *stream) {
     VF_API_BEGIN
     need(in && out, "null argument");
@@ -3198,6 +3635,7 @@ int vf_mean_noalloc(int64_t n, const double *x, void *stream, double *out, int o
 int vf_bench_apply_K(vf_sim *sim, int64_t reps, int precision, double *avg_ms) {
     VF_API_BEGIN
     need(sim && avg_ms && reps > 0, "invalid argument");
+    need(!sim->p2, "the kernel benchmarks cover the 3D kernels");
     sim->refresh_grid();
     const size_t n = 3 * sim->nn();
     hipStream_t s = sim->stream;
@@ -3262,6 +3700,7 @@ int vf_mg_kernel_timing(vf_mg *mg, int64_t *count, double *total_ms) {
 int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]) {
     VF_API_BEGIN
     need(mg && choice && ms, "null argument");
+    need(!mg->p2, "planar grids have no autotuned kernel families");
     need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
     mg->sync_masks();
     mg->ensure_stiffness();
@@ -3282,6 +3721,7 @@ int vf_mg_tuning(vf_mg *mg, int precision, int choice[2], double ms[6]) {
 int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[6]) {
     VF_API_BEGIN
     need(mg && choice && ms, "null argument");
+    need(!mg->p2, "planar grids have no autotuned kernel families");
     need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");
     mg->sync_masks();
     mg->ensure_stiffness();
@@ -3297,6 +3737,7 @@ int vf_mg_tuning_level1(vf_mg *mg, int precision, int choice[2], double ms[6]) {
 int vf_bench_mg_kernel(vf_mg *mg, int which, int64_t level, int64_t reps, int precision, double *avg_ms) {
     VF_API_BEGIN
     need(mg && avg_ms && reps > 0, "invalid argument");
+    need(!mg->p2, "the kernel benchmarks cover the 3D kernels");
     need(level >= 0 && level < mg->L, "level must be a smoothed level (0 <= l < numLevels)");
     need(which >= 0 && which <= 2, "which must be 0 (K u), 1 (residual) or 2 (one smoothing sweep)");
     need(precision == VF_PREC_FP64 || precision == VF_PREC_FP32, "precision must be VF_PREC_FP64 or VF_PREC_FP32");

@@ -275,10 +275,13 @@ void launch_assemble_l1_blocks(const Grid &gc, const Grid &gf, const double *Cha
 size_t band_storage(int n, int kb);
 void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int n, int kb, double *B, double *Linv,
                            double *X, double *XT, int *flag, hipStream_t s);
-// x = X^T X b on the reduced unknowns (zero elsewhere)
+// the same on a lower band already assembled in B (band_storage(n, kb) doubles, zero outside the band)
+void band_factor_inverse(int n, int kb, double *B, double *Linv, double *X, double *XT, int *flag, hipStream_t s);
+// x = X^T X b on the reduced unknowns (zero elsewhere); reduced DoF i is component f % 3 of node f / 3,
+// f = redToFull[i]; x holds ncomp planes of nnodes values
 template <typename T>
 void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes, const T *b,
-                              T *x, double *tmp_in, double *tmp_out, hipStream_t s);
+                              T *x, double *tmp_in, double *tmp_out, hipStream_t s, int ncomp = 3);
 // the same for nsys independent systems in one launch sequence: system z has n[z] (device) unknowns in
 // nbz[z] = ceil(n[z] / 64) of the nb blocks (only those are factored and inverted), its element matrices at
 // Ke + z keStride on the sub-grid g, its reduced map at red + z redStride; band storage

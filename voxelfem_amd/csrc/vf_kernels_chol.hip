@@ -523,13 +523,17 @@ size_t band_storage(int n, int kb) {
 
 void band_cholesky_inverse(const Grid &g, const double *Ke, const int *red, int n, int kb, double *B, double *Linv,
                            double *X, double *XT, int *flag, hipStream_t s) {
-    const int nb = (n + NB - 1) / NB;
-    const size_t ld = (size_t)nb * NB;
     // fill kernels rather than memsets: the sequence may be captured into a HIP graph
     launch_fill<double>(band_storage(n, kb), B, 0.0, s);
+    hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B, 0, 0, 0);
+    band_factor_inverse(n, kb, B, Linv, X, XT, flag, s);
+}
+
+void band_factor_inverse(int n, int kb, double *B, double *Linv, double *X, double *XT, int *flag, hipStream_t s) {
+    const int nb = (n + NB - 1) / NB;
+    const size_t ld = (size_t)nb * NB;
     launch_fill<double>(ld * ld, X, 0.0, s);
     launch_fill<int>(1, flag, 0, s);
-    hipLaunchKernelGGL(k_band_assemble, dim3(nblk((long)g.nnodes * 81)), dim3(TPB), 0, s, g, Ke, red, kb, B, 0, 0, 0);
     if ((int)ld > n) hipLaunchKernelGGL(k_band_pad, dim3(1), dim3(NB), 0, s, n, nb, kb, B);
     for (int k = 0; k < nb; ++k) {
         hipLaunchKernelGGL(k_potrf_diag, dim3(1), dim3(PTPB), 0, s, k, kb, B, Linv, flag, 0, 0, nullptr);
@@ -600,9 +604,9 @@ template void launch_coarse_chol_apply_batched<float>(int, const int *, int, con
 
 template <typename T>
 void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const int *redToFull, int nnodes,
-                              const T *b, T *x, double *tmp_in, double *tmp_out, hipStream_t s) {
+                              const T *b, T *x, double *tmp_in, double *tmp_out, hipStream_t s, int ncomp) {
     const int ld = (nred + NB - 1) / NB * NB;
-    hipLaunchKernelGGL(k_cfill<T>, dim3(nblk(3L * nnodes)), dim3(TPB), 0, s, (size_t)3 * nnodes, x, T(0));
+    hipLaunchKernelGGL(k_cfill<T>, dim3(nblk((long)ncomp * nnodes)), dim3(TPB), 0, s, (size_t)ncomp * nnodes, x, T(0));
     if (nred == 0) return;
     hipLaunchKernelGGL(k_cgather<T>, dim3(nblk(nred)), dim3(TPB), 0, s, nred, redToFull, nnodes, b, tmp_in);
     hipLaunchKernelGGL(k_trmv_lower, dim3((nred + 3) / 4), dim3(TPB), 0, s, nred, ld, X, tmp_in, tmp_out);
@@ -612,8 +616,8 @@ void launch_coarse_chol_apply(int nred, const double *X, const double *XT, const
 }
 
 template void launch_coarse_chol_apply<double>(int, const double *, const double *, const int *, int, const double *,
-                                               double *, double *, double *, hipStream_t);
+                                               double *, double *, double *, hipStream_t, int);
 template void launch_coarse_chol_apply<float>(int, const double *, const double *, const int *, int, const float *,
-                                              float *, double *, double *, hipStream_t);
+                                              float *, double *, double *, hipStream_t, int);
 
 }  // namespace vf

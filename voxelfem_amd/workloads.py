@@ -37,6 +37,30 @@ MBB_BC = {
 
 MATERIAL = {"type": "isotropic_material", "dim": 3, "density": 1.0, "young": 1.0, "poisson": 0.3}
 
+# planar (2D) workloads: the region language is the same; a planar grid reads the first two entries of
+# every corner and value
+CANTILEVER2D_BC = {
+    # x = 0 clamped; unit downward load at the lower right corner (x = L, y = 0)
+    "regions": [
+        {"type": "dirichlet", "value": [0, 0, 0],
+         "box%": {"minCorner": [-1e-4, -1e-4, 0], "maxCorner": [1e-4, 1.0001, 0]}},
+        {"type": "force", "value": [0, -1, 0],
+         "box%": {"minCorner": [0.9999, -1e-4, 0], "maxCorner": [1.0001, 1e-4, 0]}},
+    ]
+}
+
+MBB2D_BC = {
+    # pinned lower left corner, roller (u_y) at the lower right corner, unit downward load at the top middle
+    "regions": [
+        {"type": "dirichlet", "value": [0, 0, 0],
+         "box%": {"minCorner": [-1e-4, -1e-4, 0], "maxCorner": [1e-4, 1e-4, 0]}},
+        {"type": "dirichlety", "value": [0, 0, 0],
+         "box%": {"minCorner": [0.9999, -1e-4, 0], "maxCorner": [1.0001, 1e-4, 0]}},
+        {"type": "force", "value": [0, -1, 0],
+         "box%": {"minCorner": [0.4999, 0.9999, 0], "maxCorner": [0.5001, 1.0001, 0]}},
+    ]
+}
+
 GRIDS = {
     "cantilever_64x32x32": (64, 32, 32),
     "cantilever_256x128x128": (256, 128, 128),
@@ -91,6 +115,51 @@ def configure(pv, ne, bc, rho, directory=None, Emin=1e-9, symmetry_x=False):
     if rho is not None:
         tps.setDensities(np.asarray(rho, dtype=np.float64))
     return tps
+
+
+def configure2d(pv, ne, bc, rho, directory=None, Emin=1e-9, law=None):
+    """A planar ([1, 1]) pyVoxelFEM simulator on [0, nx] x [0, ny] with the standard material and law."""
+    tps = pv.TensorProductSimulator([1, 1], [[0.0, 0.0], [float(n) for n in ne]], list(ne))
+    d = directory or tempfile.mkdtemp(prefix="vf_wl_")
+    tps.readMaterial(write_json(MATERIAL, d, "material.json"))
+    tps.E_0 = 1.0
+    tps.E_min = Emin
+    tps.gamma = 3.0
+    if law is not None:
+        tps.interpolationLaw = law
+    if bc is not None:
+        tps.applyDisplacementsAndLoadsFromFile(write_json(bc, d))
+    if rho is not None:
+        tps.setDensities(np.asarray(rho, dtype=np.float64))
+    return tps
+
+
+def cantilever2d_mask(ne):
+    X = _node_positions(ne)
+    return np.where(X[:, 0] == 0, 3, 0).astype(np.uint8)
+
+
+def cantilever2d_load(ne):
+    X = _node_positions(ne)
+    f = np.zeros((len(X), 2))
+    f[(X[:, 0] == ne[0]) & (X[:, 1] == 0), 1] = -1.0
+    return f
+
+
+def mbb2d_mask(ne):
+    X = _node_positions(ne)
+    m = np.zeros(len(X), np.uint8)
+    m[(X[:, 0] == 0) & (X[:, 1] == 0)] = 3
+    m[(X[:, 0] == ne[0]) & (X[:, 1] == 0)] = 2
+    return m
+
+
+def mbb2d_load(ne):
+    X = _node_positions(ne)
+    f = np.zeros((len(X), 2))
+    top = (X[:, 0] == ne[0] / 2) & (X[:, 1] == ne[1])
+    f[top, 1] = -1.0 / top.sum()
+    return f
 
 
 def _node_positions(ne):
