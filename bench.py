@@ -32,6 +32,8 @@ def parse():
     ap.add_argument("--ku-reps", type=int, default=20)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-lbl", action="store_true", help="skip the layer-by-layer batched-vs-sequential timing")
+    ap.add_argument("--cpu-configs", action="store_true",
+                    help="also time the CPU restatement on the workloads C0-C3 (about a minute of host time)")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS") or len(os.sched_getaffinity(0))))
     return ap.parse_args()
@@ -118,6 +120,77 @@ def cpu_baseline(ne, rho_value, threads, gpu_solutions):
         "gpu_vs_oracle": errs,
         "ku_gdofs": 3 * nn / t_ku / 1e9,
     }
+
+
+def cpu_config_baselines(threads):
+    """The oracle on the workloads C0-C3 as their GPU reports (voxelfem_amd.configs) run them: C0 one FP64 solve to
+    1e-8 plus the sensitivity; C1 the FP64 PCG to 1e-8; C2 the filtered MBB solve to 1e-12 plus the gradient
+    through the filter chain; C3 the 40 zero-guess layer solves to 1e-8 with their gradients. Seconds, PCG
+    iterations and final relative residuals."""
+    import numpy as np
+
+    from oracle import oracle as orc
+    from voxelfem_amd import workloads as wl
+    from voxelfem_amd.mg_levels import optMGLevel
+
+    orc.set_threads(threads)
+
+    def sim(ne, rho, mask, f):
+        s = orc.Sim(ne)
+        s.set_etensor(1.0, 0.3)
+        s.set_law(0, 1.0, 1e-9, 3.0, 3.0)
+        s.set_densities(rho)
+        s.set_dirichlet(mask)
+        if f is not None:
+            s.set_forces(f)
+        return s
+
+    def solve(s, ne, f, tol):
+        mg = orc.MG(s, optMGLevel(ne, 3))
+        u, its, hist = mg.pcg(np.zeros_like(f), f, 3000, tol, 1, 1, False, hist_len=3000)
+        return u, its, float(hist[-1] / np.linalg.norm(f)) if its else 0.0
+
+    out = {}
+    ne = wl.GRIDS["cantilever_64x32x32"]
+    f = wl.cantilever_load(ne)
+    s = sim(ne, wl.noise_density(ne, 0), wl.cantilever_mask(ne), f)
+    t0 = time.perf_counter()
+    u, its, rr = solve(s, ne, f, 1e-8)
+    s.compliance_gradient(u)
+    out["c0"] = {"s": time.perf_counter() - t0, "pcg_iterations": its, "final_rel_residual": rr}
+    print(f"cpu baseline c0: {out['c0']}", file=sys.stderr, flush=True)
+    ne = wl.GRIDS["cantilever_256x128x128"]
+    f = wl.cantilever_load(ne)
+    s = sim(ne, wl.smoothed_density(ne, 1), wl.cantilever_mask(ne), f)
+    t0 = time.perf_counter()
+    u, its, rr = solve(s, ne, f, 1e-8)
+    out["c1"] = {"s": time.perf_counter() - t0, "pcg_iterations": its, "final_rel_residual": rr}
+    print(f"cpu baseline c1: {out['c1']}", file=sys.stderr, flush=True)
+    ne = wl.GRIDS["mbb_128x64x64"]
+    n = int(np.prod(ne))
+    x0 = np.random.default_rng(2).uniform(0.2, 0.4, n)
+    f = wl.mbb_load(ne)
+    chain = orc.Chain(ne, 2.5, True, 8.0)
+    t0 = time.perf_counter()
+    _, p = chain.forward(x0)
+    s = sim(ne, p, wl.mbb_mask(ne), f)
+    u, its, rr = solve(s, ne, f, 1e-12)
+    chain.backprop(s.compliance_gradient(u), x0)
+    out["c2"] = {"s": time.perf_counter() - t0, "pcg_iterations": its, "final_rel_residual": rr}
+    print(f"cpu baseline c2: {out['c2']}", file=sys.stderr, flush=True)
+    ne = wl.GRIDS["lbl_160x80x80"]
+    nny, nnz = ne[1] + 1, ne[2] + 1
+    iy = (np.arange(int(np.prod([k + 1 for k in ne]))) // nnz) % nny
+    s = sim(ne, wl.smoothed_density(ne, 3), np.where(iy == 0, 7, 0).astype(np.uint8), None)
+    s.set_gravity([0.0, -1.0, 0.0])
+    mg = orc.MG(s, optMGLevel(ne, 3))
+    t0 = time.perf_counter()
+    _, _, _, its = mg.lbl_run_zero(2, 1000, 1e-8)
+    out["c3"] = {"s": time.perf_counter() - t0, "pcg_iterations_total": int(np.sum(its)), "layer_solves": len(its)}
+    print(f"cpu baseline c3: {out['c3']}", file=sys.stderr, flush=True)
+    model, ncpu, avail = _host_cpu()
+    out["host"] = f"{orc.get_threads()} OpenMP threads on '{model}' ({avail} of {ncpu} logical CPUs available)"
+    return out
 
 
 def lbl_batched_vs_sequential(pv, precision):
@@ -312,6 +385,14 @@ def main():
                                  "mixed": lbl_batched_vs_sequential(pv, "fp32")}
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(ne, rho_value, args.cpu_threads, solutions)
+    if args.cpu_configs:
+        if args.no_cpu_baseline:
+            from oracle import oracle as orc
+
+            path, _ = _oracle_for_this_host()
+            if path:
+                orc.use_library(path)
+        line["cpu_baseline_configs"] = cpu_config_baselines(args.cpu_threads)
     print(json.dumps(line), flush=True)
 
 

@@ -116,7 +116,10 @@ __device__ __forceinline__ void elem_rows(const PGrid &g, const double *__restri
     for (int c = 0; c < 2; ++c) {
         double t = 0;
 #pragma unroll
-        for (int a = 0; a < 8; ++a) t += K[a * 8 + 2 * ln + c] * ue[a];  // column 2 ln + c (K is symmetric)
+        // K is symmetric: the fine level reads column 2 ln + c of K0 (cache-resident, the oracle's order of the
+        // products); coarse levels read row 2 ln + c, whose 8 doubles are contiguous (rows 2 ln and 2 ln + 1 fill
+        // two 64-byte lines), instead of one line per entry of a column
+        for (int a = 0; a < 8; ++a) t += (FINE ? K[a * 8 + 2 * ln + c] : K[(2 * ln + c) * 8 + a]) * ue[a];
         out[c] += s * t;
         if (M) {
             M[2 * c] += s * K[(2 * ln + c) * 8 + 2 * ln];
