@@ -270,6 +270,26 @@ int vf_smoothing_filter(const int64_t dims[3], double rmin, int type, const doub
  * Linear weight rmin - |(a, b)|) */
 int vf_smoothing_filter_2d(const int64_t dims[2], double rmin, int type, const double *in, double *out, int where,
                            void *stream);
+/* UpsampleFilter::apply / backprop (TopologyOptimizationFilter.hh:419-525), degree 1, on an nd-dimensional
+ * (nd = 2 or 3) grid of inDims values (each >= 2): out has (inDims[k] - 1) * factor + 1 values along axis k,
+ * the multilinear interpolant of the coarse cells; the backprop maps a gradient on that fine grid back to the
+ * inDims grid (the transpose). C order, last index fastest. */
+int vf_upsample_filter(int64_t nd, const int64_t *inDims, int64_t factor, const double *in, double *out, int where,
+                       void *stream);
+int vf_upsample_filter_backprop(int64_t nd, const int64_t *inDims, int64_t factor, const double *gOut, double *gIn,
+                                int where, void *stream);
+/* VertexToCellFilter::apply / backprop (TopologyOptimizationFilter.hh:527-597): the cell values (inDims[k] - 1
+ * along each axis) are the means of their 2^nd corner vertices; the backprop is the transpose. */
+int vf_vertex_to_cell(int64_t nd, const int64_t *inDims, const double *in, double *out, int where, void *stream);
+int vf_vertex_to_cell_backprop(int64_t nd, const int64_t *inDims, const double *gOut, double *gIn, int where,
+                               void *stream);
+/* LangelaarFilter::apply / backprop (TopologyOptimizationFilter.hh:600-712) on a planar grid dims[0] x dims[1]
+ * (y the build direction), device pointers: out = the filtered densities, smax (nx ny) the smooth maxima of the
+ * supporting voxels, which backprop reads with the filtered densities of the same apply; scratch holds nx ny
+ * multipliers. */
+int vf_langelaar(const int64_t dims[2], const double *in, double *out, double *smax, void *stream);
+int vf_langelaar_backprop(const int64_t dims[2], const double *g, const double *vars, const double *filtered,
+                          const double *smax, double *scratch, double *out, void *stream);
 int vf_projection(double beta, int64_t n, const double *in, double *out, int where, void *stream);
 int vf_projection_backprop(double beta, int64_t n, const double *g, const double *vars, double *out, int where,
                            void *stream);

@@ -5,6 +5,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -36,7 +37,7 @@ def test_pybind_surface():
     for name in ["TensorProductSimulator", "TopologyOptimizationProblem", "ComplianceObjective",
                  "MultigridComplianceObjective", "LayerByLayerEvaluator", "OCOptimizer", "getClassName",
                  "FilterChain", "SmoothingFilter", "ProjectionFilter", "PythonFilter", "TotalVolumeConstraint",
-                 "InterpolationLaw", "NumberType"]:
+                 "InterpolationLaw", "NumberType", "UpsampleFilter", "VertexToCellFilter", "LangelaarFilter"]:
         assert hasattr(pv, name), name
     tps_cls = pv.detail.TensorProductSimulator1_1_1
     for meth in ["numNodes", "numElements", "getDensities", "setDensities", "setUniformDensities", "applyK",
@@ -61,3 +62,31 @@ def test_level_heuristic_matches_survey_counts():
                                         (384, 192, 192)]] == [2, 4, 3, 3, 5]
     assert possibleMaxMGLevel((160, 80, 80)) == 4
     assert optMGLevel((8, 4, 4), 3) == 1 and optMGLevel((1, 4, 4), 3) == 0
+
+
+def test_grid_changing_filter_dimensions():
+    """UpsampleFilter / VertexToCellFilter / LangelaarFilter dimension rules and their errors
+    (TopologyOptimizationFilter.hh:441-456, 587-595), checked without touching a device."""
+    from voxelfem_amd import pyVoxelFEM as pv
+
+    u = pv.UpsampleFilter(3)
+    u.setInputDimensions([4, 5])
+    assert (u.inputDimensions, u.outputDimensions) == ([4, 5], [10, 13])
+    u.setOutputDimensions([7, 4, 13])
+    assert (u.inputDimensions, u.outputDimensions) == ([3, 2, 5], [7, 4, 13])
+    with pytest.raises(RuntimeError, match="not divisible by factor"):
+        u.setOutputDimensions([11, 13])
+    with pytest.raises(RuntimeError, match="2\\^d grid or larger"):
+        u.setInputDimensions([1, 5])
+    v = pv.VertexToCellFilter()
+    v.setOutputDimensions([6, 7])
+    assert (v.inputDimensions, v.outputDimensions) == ([7, 8], [6, 7])
+    with pytest.raises(RuntimeError, match="2\\^d or larger"):
+        v.setInputDimensions([1, 3, 3])
+    lg = pv.LangelaarFilter()
+    lg.setOutputDimensions([5, 4])
+    assert lg.inputDimensions == [5, 4]
+    with pytest.raises(RuntimeError, match="planar grids"):
+        lg.setOutputDimensions([4, 4, 4])
+    with pytest.raises(RuntimeError, match="not set"):
+        pv.VertexToCellFilter().apply(np.zeros(4))

@@ -22,7 +22,7 @@ BUILD = os.path.join(HERE, "_build")
 INCLUDE = os.path.join(ROOT, "include")
 
 HIP_SOURCES = ["vf_kernels_ops.hip", "vf_kernels_tiled.hip", "vf_kernels_walsh.hip", "vf_kernels_setup.hip",
-               "vf_kernels_chol.hip", "vf_kernels_vec.hip", "vf_kernels_l1mfma.hip", "vf_planar.hip", "vf_host.hip"]
+               "vf_kernels_chol.hip", "vf_kernels_vec.hip", "vf_kernels_l1mfma.hip", "vf_planar.hip", "vf_kernels_filters.hip", "vf_host.hip"]
 ARCH = os.environ.get("VF_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled for every HIP translation unit

@@ -37,6 +37,12 @@ def lib():
         L.vf_sim_apply_K.argtypes = [vp, vp, vp, i32]
         L.vf_smoothing_filter.argtypes = [ctypes.POINTER(i64), dbl, i32, vp, vp, i32, vp]
         L.vf_projection.argtypes = [dbl, i64, vp, vp, i32, vp]
+        L.vf_upsample_filter.argtypes = [i64, ctypes.POINTER(i64), i64, vp, vp, i32, vp]
+        L.vf_upsample_filter_backprop.argtypes = [i64, ctypes.POINTER(i64), i64, vp, vp, i32, vp]
+        L.vf_vertex_to_cell.argtypes = [i64, ctypes.POINTER(i64), vp, vp, i32, vp]
+        L.vf_vertex_to_cell_backprop.argtypes = [i64, ctypes.POINTER(i64), vp, vp, i32, vp]
+        L.vf_langelaar.argtypes = [ctypes.POINTER(i64), vp, vp, vp, vp]
+        L.vf_langelaar_backprop.argtypes = [ctypes.POINTER(i64), vp, vp, vp, vp, vp, vp, vp]
         L.vf_projection_backprop.argtypes = [dbl, i64, vp, vp, vp, i32, vp]
         L.vf_mg_update_stiffness.argtypes = [vp]
         L.vf_mg_finish_update.argtypes = [vp]

@@ -268,20 +268,37 @@ struct LevelView {
 // ====================================================================================================
 // Filters (TopologyOptimizationFilter.hh:90-400) -- device-resident
 // ====================================================================================================
+// A filter maps a grid of inDims values to one of outDims values (TopologyOptimizationFilter.hh:16-88); most keep
+// the grid. dims: 2 or 3 entries (planar or 3D grid), C order.
 struct Filter {
-    std::array<int64_t, 3> dims{0, 0, 0};  // a planar grid: {nx, ny, 1} with nd = 2
-    int nd = 3;
+    std::vector<int64_t> inDims, outDims;
     bool dimsSet = false;
     virtual ~Filter() = default;
     virtual std::string name() const = 0;
-    size_t size() const { return (size_t)(dims[0] * dims[1] * dims[2]); }
-    void setDims(const std::vector<int64_t> &d) {
+    static size_t prod(const std::vector<int64_t> &d) {
+        size_t p = 1;
+        for (auto v : d) p *= (size_t)v;
+        return p;
+    }
+    static void checkCount(const std::vector<int64_t> &d) {
         if (d.size() != 3 && d.size() != 2) throw std::runtime_error("Dimension count mismatch");
-        nd = (int)d.size();
-        dims = {d[0], d[1], nd == 3 ? d[2] : 1};
+    }
+    size_t inSize() const { return prod(inDims); }
+    size_t outSize() const { return prod(outDims); }
+    size_t size() const { return outSize(); }
+    int nd() const { return (int)outDims.size(); }
+    // the grid as three extents (a planar grid: {nx, ny, 1})
+    static std::array<int64_t, 3> pad3(const std::vector<int64_t> &d) { return {d[0], d[1], d.size() == 3 ? d[2] : 1}; }
+    void setInputDims(const std::vector<int64_t> &d) {
+        checkCount(d);
+        setInput(d);
         dimsSet = true;
     }
-    std::vector<int64_t> gridDims() const { return std::vector<int64_t>(dims.begin(), dims.begin() + nd); }
+    void setOutputDims(const std::vector<int64_t> &d) {
+        checkCount(d);
+        setOutput(d);
+        dimsSet = true;
+    }
     void checkDims() const {
         if (!dimsSet)
             throw std::runtime_error(
@@ -290,6 +307,10 @@ struct Filter {
     }
     virtual void apply(const DevVec &in, DevVec &out) = 0;
     virtual void backprop(const DevVec &g, const DevVec &vars, DevVec &out) = 0;
+
+protected:
+    virtual void setInput(const std::vector<int64_t> &d) { inDims = outDims = d; }
+    virtual void setOutput(const std::vector<int64_t> &d) { inDims = outDims = d; }
 };
 using FilterPtr = std::shared_ptr<Filter>;
 
@@ -304,8 +325,9 @@ struct SmoothingFilter : Filter {
     void apply(const DevVec &in, DevVec &out) override {
         checkDims();
         out.resize(size());
-        if (nd == 2) check(vf_smoothing_filter_2d(dims.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
-        else check(vf_smoothing_filter(dims.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
+        const auto d = pad3(outDims);
+        if (nd() == 2) check(vf_smoothing_filter_2d(d.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
+        else check(vf_smoothing_filter(d.data(), kernelRadius(), (int)type, in.p, out.p, VF_DEVICE, nullptr));
     }
     void backprop(const DevVec &g, const DevVec &, DevVec &out) override { apply(g, out); }  // S^T = S
 };
@@ -338,32 +360,154 @@ struct PythonFilter : Filter {
     void apply(const DevVec &in, DevVec &out) override {
         if (!apply_cb) throw std::runtime_error("Apply callback must be configured");
         auto hin = from_vec(in.host());
-        py::array_t<double> res((py::ssize_t)size());
+        py::array_t<double> res((py::ssize_t)outSize());
         apply_cb(hin, res);
-        out.upload(res.data(), size());
+        out.upload(res.data(), outSize());
     }
     void backprop(const DevVec &g, const DevVec &vars, DevVec &out) override {
         if (!backprop_cb) throw std::runtime_error("Backprop callback must be configured");
         auto hg = from_vec(g.host()), hv = from_vec(vars.host());
-        py::array_t<double> res((py::ssize_t)size());
+        py::array_t<double> res((py::ssize_t)inSize());
         backprop_cb(hg, hv, res);
-        out.upload(res.data(), size());
+        out.upload(res.data(), inSize());
     }
 };
 
+// UpsampleFilter (TopologyOptimizationFilter.hh:419-525): degree-1 interpolation onto a grid `factor` times finer
+struct UpsampleFilter : Filter {
+    int64_t factor;
+    explicit UpsampleFilter(int64_t f) : factor(f) {
+        if (f < 1) throw std::runtime_error("UpsampleFilter factor must be positive");
+    }
+    std::string name() const override { return "UpsampleFilter"; }
+    void apply(const DevVec &in, DevVec &out) override {
+        checkDims();
+        out.resize(outSize());
+        check(vf_upsample_filter((int64_t)nd(), inDims.data(), factor, in.p, out.p, VF_DEVICE, nullptr));
+    }
+    void backprop(const DevVec &g, const DevVec &, DevVec &out) override {
+        checkDims();
+        out.resize(inSize());
+        check(vf_upsample_filter_backprop((int64_t)nd(), inDims.data(), factor, g.p, out.p, VF_DEVICE, nullptr));
+    }
+
+protected:
+    void setInput(const std::vector<int64_t> &d) override {
+        for (auto v : d)
+            if (v < 2) throw std::runtime_error("Interpolation can only be applied to a 2^d grid or larger");
+        inDims = d;
+        outDims.resize(d.size());
+        for (size_t k = 0; k < d.size(); ++k) outDims[k] = (d[k] - 1) * factor + 1;
+    }
+    void setOutput(const std::vector<int64_t> &d) override {
+        for (auto v : d)
+            if (v < 2) throw std::runtime_error("Interpolation can only be applied to a 2^d grid or larger");
+        outDims = d;
+        inDims.resize(d.size());
+        for (size_t k = 0; k < d.size(); ++k) {
+            inDims[k] = (d[k] - 1) / factor + 1;
+            if ((inDims[k] - 1) * factor + 1 != d[k]) throw std::runtime_error("Output size is not divisible by factor");
+        }
+    }
+};
+
+// VertexToCellFilter (TopologyOptimizationFilter.hh:527-597): vertex values to cell means
+struct VertexToCellFilter : Filter {
+    std::string name() const override { return "VertexToCellFilter"; }
+    void apply(const DevVec &in, DevVec &out) override {
+        checkDims();
+        out.resize(outSize());
+        check(vf_vertex_to_cell((int64_t)nd(), inDims.data(), in.p, out.p, VF_DEVICE, nullptr));
+    }
+    void backprop(const DevVec &g, const DevVec &, DevVec &out) override {
+        checkDims();
+        out.resize(inSize());
+        check(vf_vertex_to_cell_backprop((int64_t)nd(), inDims.data(), g.p, out.p, VF_DEVICE, nullptr));
+    }
+
+protected:
+    void setInput(const std::vector<int64_t> &d) override {
+        for (auto v : d)
+            if (v < 2) throw std::runtime_error("Input grid must be 2^d or larger.");
+        inDims = d;
+        outDims.resize(d.size());
+        for (size_t k = 0; k < d.size(); ++k) outDims[k] = d[k] - 1;
+    }
+    void setOutput(const std::vector<int64_t> &d) override {
+        outDims = d;
+        inDims.resize(d.size());
+        for (size_t k = 0; k < d.size(); ++k) inDims[k] = d[k] + 1;
+    }
+};
+
+// LangelaarFilter (TopologyOptimizationFilter.hh:600-712), build direction y. The filtered densities and the
+// smooth maxima of the last apply are kept for backprop, as in the reference. Planar grids only: on 3D grids the
+// reference's supporting region (NDVector.hh:212-229) holds the voxel itself, whose output it reads before
+// computing it, so its result depends on the buffer's earlier contents.
+struct LangelaarFilter : Filter {
+    DevVec filtered, smax, scratch;
+    std::string name() const override { return "LangelaarFilter"; }
+    void apply(const DevVec &in, DevVec &out) override {
+        checkDims();
+        out.resize(outSize());
+        smax.resize(outSize());
+        check(vf_langelaar(outDims.data(), in.p, out.p, smax.p, nullptr));
+        filtered.copy_from(out);
+    }
+    void backprop(const DevVec &g, const DevVec &vars, DevVec &out) override {
+        checkDims();
+        if (filtered.n != outSize()) throw std::runtime_error("LangelaarFilter: apply must run before backprop");
+        out.resize(inSize());
+        scratch.resize(inSize());
+        check(vf_langelaar_backprop(outDims.data(), g.p, vars.p, filtered.p, smax.p, scratch.p, out.p, nullptr));
+    }
+
+protected:
+    void setInput(const std::vector<int64_t> &d) override { setOutput(d); }
+    void setOutput(const std::vector<int64_t> &d) override {
+        if (d.size() != 2)
+            throw std::runtime_error("LangelaarFilter is provided for planar grids: on 3D grids the reference's supporting "
+                                     "region reads the voxel's own output before computing it");
+        inDims = outDims = d;
+    }
+};
+
+// FilterChain (TopologyOptimizationFilter.hh:90-187): the chain is sized from its output grid backwards, so
+// grid-changing filters give a design grid different from the physical one
 struct FilterChain {
     std::vector<FilterPtr> filters;
     std::vector<std::unique_ptr<DevVec>> vars;  // vars[0] design, vars.back() physical
-    std::array<int64_t, 3> dims;
-    int nd = 3;
-    FilterChain(std::vector<FilterPtr> f, const std::vector<int64_t> &outDims) : filters(std::move(f)) {
-        if (outDims.size() != 3 && outDims.size() != 2) throw std::runtime_error("Dimension count mismatch");
-        nd = (int)outDims.size();
-        dims = {outDims[0], outDims[1], nd == 3 ? outDims[2] : 1};
-        for (auto &fl : filters) fl->setDims(outDims);
-        for (size_t i = 0; i <= filters.size(); ++i) vars.emplace_back(std::make_unique<DevVec>(size()));
+    std::vector<int64_t> inDims, outDims;
+    FilterChain(std::vector<FilterPtr> f, const std::vector<int64_t> &outGridDims) : filters(std::move(f)) {
+        setOutputDimensions(outGridDims);
     }
-    size_t size() const { return (size_t)(dims[0] * dims[1] * dims[2]); }
+    void allocate() {
+        vars.clear();
+        vars.emplace_back(std::make_unique<DevVec>(Filter::prod(inDims)));
+        for (auto &fl : filters) vars.emplace_back(std::make_unique<DevVec>(fl->outSize()));
+    }
+    void setOutputDimensions(std::vector<int64_t> dims) {
+        Filter::checkCount(dims);
+        outDims = dims;
+        for (size_t i = filters.size(); i-- > 0;) {
+            filters[i]->setOutputDims(dims);
+            dims = filters[i]->inDims;
+        }
+        inDims = dims;
+        allocate();
+    }
+    void setInputDimensions(std::vector<int64_t> dims) {
+        Filter::checkCount(dims);
+        inDims = dims;
+        for (auto &fl : filters) {
+            fl->setInputDims(dims);
+            dims = fl->outDims;
+        }
+        outDims = dims;
+        allocate();
+    }
+    size_t size() const { return Filter::prod(inDims); }
+    size_t numPhysicalVars() const { return Filter::prod(outDims); }
     DevVec &design() { return *vars.front(); }
     DevVec &physical() { return *vars.back(); }
     // setDesignVars (TopologyOptimizationFilter.hh:142-152)
@@ -1506,14 +1650,14 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
 
     // ------------------------------------------------------------------ filters and constraints
     py::class_<Filter, FilterPtr>(detail, "Filter")
-        .def("setInputDimensions", [](Filter &f, const std::vector<int64_t> &d) { f.setDims(d); }, py::arg("gridDims"))
-        .def("setOutputDimensions", [](Filter &f, const std::vector<int64_t> &d) { f.setDims(d); }, py::arg("gridDims"))
-        .def_property_readonly("inputDimensions", [](const Filter &f) { return f.gridDims(); })
-        .def_property_readonly("outputDimensions", [](const Filter &f) { return f.gridDims(); })
+        .def("setInputDimensions", [](Filter &f, const std::vector<int64_t> &d) { f.setInputDims(d); }, py::arg("gridDims"))
+        .def("setOutputDimensions", [](Filter &f, const std::vector<int64_t> &d) { f.setOutputDims(d); }, py::arg("gridDims"))
+        .def_property_readonly("inputDimensions", [](const Filter &f) { return f.inDims; })
+        .def_property_readonly("outputDimensions", [](const Filter &f) { return f.outDims; })
         .def("apply", [](Filter &f, const Arr &x) {
                 f.checkDims();
                 DevVec in, out;
-                auto v = to_vec(x, f.size(), "input");
+                auto v = to_vec(x, f.inSize(), "input");
                 in.upload(v.data(), v.size());
                 f.apply(in, out);
                 return from_vec(out.host());
@@ -1522,17 +1666,19 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
     py::class_<FilterChain, FilterChainPtr>(m, "FilterChain")
         .def(py::init<std::vector<FilterPtr>, const std::vector<int64_t> &>(), py::arg("filters"), py::arg("outGridDimensions"))
         .def("numVars", &FilterChain::size)
-        .def("numPhysicalVars", &FilterChain::size)
-        .def("gridDims", [](const FilterChain &c) { return std::vector<int64_t>(c.dims.begin(), c.dims.begin() + c.nd); })
-        .def("physicalGridDims", [](const FilterChain &c) { return std::vector<int64_t>(c.dims.begin(), c.dims.begin() + c.nd); })
+        .def("numPhysicalVars", &FilterChain::size)  // the reference binds numVars here as well (VoxelFEM.cc:331)
+        .def("gridDims", [](const FilterChain &c) { return c.inDims; }, "Input grid dimensions")
+        .def("physicalGridDims", [](const FilterChain &c) { return c.outDims; }, "Output grid dimensions")
+        .def("setInputDimensions", &FilterChain::setInputDimensions, py::arg("dims"))
+        .def("setOutputDimensions", &FilterChain::setOutputDimensions, py::arg("dims"))
         .def("setDesignVars", [](FilterChain &c, const Arr &x) {
                 auto v = to_vec(x, c.size(), "design variable");
                 DevVec d; d.upload(v.data(), v.size());
                 c.setDesignVarsDevice(d);
             }, py::arg("xDesign"))
         .def("backprop", [](FilterChain &c, const Arr &g) {
-                if ((size_t)g.size() != c.size()) throw std::runtime_error("Size mismatch");
-                DevVec d; d.upload(g.data(), c.size());
+                if ((size_t)g.size() != c.numPhysicalVars()) throw std::runtime_error("Size mismatch");
+                DevVec d; d.upload(g.data(), c.numPhysicalVars());
                 c.backprop(d);
                 return from_vec(d.host());
             }, py::arg("g"))
@@ -1563,6 +1709,14 @@ PYBIND11_MODULE(pyVoxelFEM, m) {
         .def_readwrite("type", &SmoothingFilter::type)
         // extension: real-valued hat radius (w = rmin - |d| > 0); 0 restores the integer convention
         .def_readwrite("rmin", &SmoothingFilter::rmin);
+
+    py::class_<UpsampleFilter, Filter, std::shared_ptr<UpsampleFilter>>(m, "UpsampleFilter")
+        .def(py::init<int64_t>(), py::arg("factor"))
+        .def_readonly("factor", &UpsampleFilter::factor);
+    py::class_<VertexToCellFilter, Filter, std::shared_ptr<VertexToCellFilter>>(m, "VertexToCellFilter")
+        .def(py::init<>());
+    py::class_<LangelaarFilter, Filter, std::shared_ptr<LangelaarFilter>>(m, "LangelaarFilter")
+        .def(py::init<>());
 
     py::class_<Constraint, ConstraintPtr>(detail, "Constraint");
     py::class_<TotalVolumeConstraint, Constraint, std::shared_ptr<TotalVolumeConstraint>>(m, "TotalVolumeConstraint")

@@ -3557,6 +3557,86 @@ int vf_smoothing_filter_2d(const int64_t dims[2], double rmin, int type, const d
     VF_API_END
 }
 
+static void check_grid_dims(int64_t nd, const int64_t *dims, int64_t minimum) {
+    need(dims != nullptr, "null argument");
+    need(nd == 2 || nd == 3, "grid dimension count must be 2 or 3");
+    int64_t n = 1;
+    for (int k = 0; k < nd; ++k) {
+        need(dims[k] >= minimum, "grid too small for this filter");
+        n *= dims[k];
+    }
+    need(n < INT_MAX, "grid too large");
+}
+
+static int upsample_impl(int64_t nd, const int64_t *inDims, int64_t factor, const double *a, double *b, bool adjoint,
+                         int where, void *stream) {
+    VF_API_BEGIN
+    check_grid_dims(nd, inDims, 2);
+    need(a && b && a != b, "null or aliased argument");
+    need(factor >= 1 && factor < 4096, "upsampling factor must be positive");
+    int64_t od[3];
+    upsample_dims((int)nd, inDims, (int)factor, od);
+    size_t nin = 1, nout = 1;
+    for (int k = 0; k < nd; ++k) {
+        nin *= (size_t)inDims[k];
+        nout *= (size_t)od[k];
+    }
+    need(nout < (size_t)INT_MAX, "grid too large");
+    with_buffers(adjoint ? nout : nin, adjoint ? nin : nout, a, b, where, as_stream(stream),
+                 [&](const double *i, const double *, double *o) {
+                     launch_upsample((int)nd, inDims, (int)factor, i, o, adjoint, as_stream(stream));
+                 });
+    VF_API_END
+}
+int vf_upsample_filter(int64_t nd, const int64_t *inDims, int64_t factor, const double *in, double *out, int where,
+                       void *stream) {
+    return upsample_impl(nd, inDims, factor, in, out, false, where, stream);
+}
+int vf_upsample_filter_backprop(int64_t nd, const int64_t *inDims, int64_t factor, const double *gOut, double *gIn,
+                                int where, void *stream) {
+    return upsample_impl(nd, inDims, factor, gOut, gIn, true, where, stream);
+}
+
+static int v2c_impl(int64_t nd, const int64_t *inDims, const double *a, double *b, bool adjoint, int where,
+                    void *stream) {
+    VF_API_BEGIN
+    check_grid_dims(nd, inDims, 2);
+    need(a && b && a != b, "null or aliased argument");
+    size_t nin = 1, nout = 1;
+    for (int k = 0; k < nd; ++k) {
+        nin *= (size_t)inDims[k];
+        nout *= (size_t)(inDims[k] - 1);
+    }
+    with_buffers(adjoint ? nout : nin, adjoint ? nin : nout, a, b, where, as_stream(stream),
+                 [&](const double *i, const double *, double *o) {
+                     launch_vertex_to_cell((int)nd, inDims, i, o, adjoint, as_stream(stream));
+                 });
+    VF_API_END
+}
+int vf_vertex_to_cell(int64_t nd, const int64_t *inDims, const double *in, double *out, int where, void *stream) {
+    return v2c_impl(nd, inDims, in, out, false, where, stream);
+}
+int vf_vertex_to_cell_backprop(int64_t nd, const int64_t *inDims, const double *gOut, double *gIn, int where,
+                               void *stream) {
+    return v2c_impl(nd, inDims, gOut, gIn, true, where, stream);
+}
+
+int vf_langelaar(const int64_t dims[2], const double *in, double *out, double *smax, void *stream) {
+    VF_API_BEGIN
+    check_grid_dims(2, dims, 1);
+    need(in && out && smax && in != out, "null or aliased argument");
+    launch_langelaar((int)dims[0], (int)dims[1], in, out, smax, as_stream(stream));
+    VF_API_END
+}
+int vf_langelaar_backprop(const int64_t dims[2], const double *g, const double *vars, const double *filtered,
+                          const double *smax, double *scratch, double *out, void *stream) {
+    VF_API_BEGIN
+    check_grid_dims(2, dims, 1);
+    need(g && vars && filtered && smax && scratch && out, "null argument");
+    launch_langelaar_backprop((int)dims[0], (int)dims[1], g, vars, filtered, smax, scratch, out, as_stream(stream));
+    VF_API_END
+}
+
 int vf_projection(double beta, int64_t n, const double *in, double *out, int where, void *stream) {
     VF_API_BEGIN
     need(in && out, "null argument");

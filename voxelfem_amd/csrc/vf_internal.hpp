@@ -348,6 +348,15 @@ void launch_self_weight(const Grid &g, const double *rho, const double *grav, do
 // y += a x
 void launch_axpy_plain(size_t n, double a, const double *x, double *y, hipStream_t s);
 void launch_smoothing_filter(const int dims[3], double rmin, int type, const double *in, double *out, hipStream_t s);
+// UpsampleFilter / VertexToCellFilter on an nd-dimensional grid of inDims values (adjoint: in holds the output
+// grid's gradient, out the input grid's); LangelaarFilter on a planar nx x ny grid (lam: nx ny scratch)
+void upsample_dims(int nd, const int64_t *in, int factor, int64_t *out);
+void launch_upsample(int nd, const int64_t *inDims, int factor, const double *in, double *out, bool adjoint,
+                     hipStream_t s);
+void launch_vertex_to_cell(int nd, const int64_t *inDims, const double *in, double *out, bool adjoint, hipStream_t s);
+void launch_langelaar(int nx, int ny, const double *in, double *out, double *smax, hipStream_t s);
+void launch_langelaar_backprop(int nx, int ny, const double *g, const double *vars, const double *filtered,
+                               const double *smax, double *lam, double *out, hipStream_t s);
 void launch_projection(double beta, size_t n, const double *in, double *out, hipStream_t s);
 void launch_projection_backprop(double beta, size_t n, const double *g, const double *vars, double *out,
                                 hipStream_t s);
