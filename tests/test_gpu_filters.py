@@ -3,7 +3,8 @@ TopologyOptimizationFilter.hh:419-712) on the device with their numpy restatemen
 tests/test_filters_oracle.py).
 
 Tolerances: upsample and vertex-to-cell to 1e-13 relative (the kernels may contract a multiply-add); Langelaar
-filtered densities to 1e-12 and its backprop to 1e-10 relative (device pow against the host's, carried through the
+filtered densities to 1e-12 and its backprop to 1e-10 relative (powers by repeated squaring and device pow against
+the host's pow, carried through the
 layer recursion).
 """
 import ctypes
@@ -34,7 +35,8 @@ def chain_backprop(pv, filters, out_dims, x, g):
     return fc, fc.backprop(g.ravel())
 
 
-@pytest.mark.parametrize("shape,factor", [((5, 4), 2), ((3, 7), 3), ((4, 3, 5), 2), ((2, 3, 2), 4), ((6, 5), 1)])
+@pytest.mark.parametrize("shape,factor", [((5, 4), 2), ((3, 7), 3), ((4, 3, 5), 2), ((2, 3, 2), 4), ((6, 5), 1),
+                                          ((30, 70), 3), ((6, 11, 40), 2)])
 def test_upsample_matches_oracle(pv, shape, factor):
     rng = np.random.default_rng(shape[0] * 31 + factor)
     x = rng.random(shape)
@@ -52,7 +54,7 @@ def test_upsample_matches_oracle(pv, shape, factor):
     assert rel(gin, F.upsample_backprop(g, shape, factor).ravel()) < 1e-13
 
 
-@pytest.mark.parametrize("shape", [(5, 4), (2, 9), (4, 3, 5), (2, 2, 2)])
+@pytest.mark.parametrize("shape", [(5, 4), (2, 9), (4, 3, 5), (2, 2, 2), (41, 130), (7, 9, 70)])
 def test_vertex_to_cell_matches_oracle(pv, shape):
     rng = np.random.default_rng(shape[0] * 7 + len(shape))
     x = rng.random(shape)
@@ -91,8 +93,9 @@ def test_c_abi_host_buffers(pv):
     assert b"too small" in L.vf_last_error()
 
 
-@pytest.mark.parametrize("shape", [(9, 7), (1, 5), (40, 24)])
+@pytest.mark.parametrize("shape", [(9, 7), (1, 5), (40, 24), (2048, 3), (2100, 4)])
 def test_langelaar_matches_oracle(pv, shape):
+    """Grids up to 2048 columns run the single-workgroup recurrence, wider ones one launch per layer."""
     rng = np.random.default_rng(shape[0] + 100 * shape[1])
     x = 0.05 + 0.9 * rng.random(shape)
     filtered, smax = F.langelaar(x)
