@@ -285,8 +285,8 @@ int vf_vertex_to_cell_backprop(int64_t nd, const int64_t *inDims, const double *
                                void *stream);
 /* LangelaarFilter::apply / backprop (TopologyOptimizationFilter.hh:600-712) on a planar grid dims[0] x dims[1]
  * (y the build direction), device pointers: out = the filtered densities, smax (nx ny) the smooth maxima of the
- * supporting voxels, which backprop reads with the filtered densities of the same apply; scratch holds nx ny
- * multipliers. */
+ * supporting voxels, which backprop reads with the filtered densities of the same apply; scratch is a workspace
+ * of nx ny doubles. out and scratch must not alias the inputs or each other (out also serves as workspace). */
 int vf_langelaar(const int64_t dims[2], const double *in, double *out, double *smax, void *stream);
 int vf_langelaar_backprop(const int64_t dims[2], const double *g, const double *vars, const double *filtered,
                           const double *smax, double *scratch, double *out, void *stream);

@@ -3624,7 +3624,7 @@ int vf_vertex_to_cell_backprop(int64_t nd, const int64_t *inDims, const double *
 int vf_langelaar(const int64_t dims[2], const double *in, double *out, double *smax, void *stream) {
     VF_API_BEGIN
     check_grid_dims(2, dims, 1);
-    need(in && out && smax && in != out, "null or aliased argument");
+    need(in && out && smax && in != out && smax != in && smax != out, "null or aliased argument");
     launch_langelaar((int)dims[0], (int)dims[1], in, out, smax, as_stream(stream));
     VF_API_END
 }
@@ -3633,6 +3633,9 @@ int vf_langelaar_backprop(const int64_t dims[2], const double *g, const double *
     VF_API_BEGIN
     check_grid_dims(2, dims, 1);
     need(g && vars && filtered && smax && scratch && out, "null argument");
+    need(out != g && out != vars && out != filtered && out != smax && out != scratch && scratch != g &&
+             scratch != vars && scratch != filtered && scratch != smax,
+         "out and scratch must not alias the inputs or each other");
     launch_langelaar_backprop((int)dims[0], (int)dims[1], g, vars, filtered, smax, scratch, out, as_stream(stream));
     VF_API_END
 }
