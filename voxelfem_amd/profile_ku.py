@@ -1,10 +1,12 @@
 """Driver for counter-based profiling of the fine-level K u (run it under rocprofv3 --pmc):
 
     python -m voxelfem_amd.profile_ku [--grid 384,192,192] [--variant 3] [--reps 3] [--which 0] [--prec fp64]
+                                      [--level 0]
 
 It builds the cantilever workload's hierarchy, launches the chosen fine-level kernel family `reps` times
 through vf_bench_mg_kernel (variant: 1 register-gather, 2 LDS-tiled, 3 Walsh-basis K u with the tiled
-smoother, 4 Walsh-basis K u with the structured gather smoother; which: 0 K u, 2 one smoothing sweep), then
+smoother, 4 Walsh-basis K u with the structured gather smoother; which: 0 K u, 2 one smoothing sweep; level:
+the hierarchy level, whose kernel family the autotuner picks on levels >= 1), then
 runs a 2 GiB device fill and a 2 GiB device copy whose byte counts are known, to calibrate FETCH_SIZE /
 WRITE_SIZE.
 """
@@ -20,6 +22,7 @@ def main(argv=None):
     ap.add_argument("--reps", type=int, default=3)
     ap.add_argument("--which", type=int, default=0, choices=[0, 2])
     ap.add_argument("--prec", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--level", type=int, default=0)
     args = ap.parse_args(argv)
     import torch
 
@@ -34,10 +37,10 @@ def main(argv=None):
     capi.check(capi.lib().vf_mg_update_stiffness(h))
     capi.set_kernel_variant(args.variant)
     prec = capi.VF_PREC_FP32 if args.prec == "fp32" else capi.VF_PREC_FP64
-    ms = capi.bench_mg_kernel(h, args.which, 0, args.reps, prec)
+    ms = capi.bench_mg_kernel(h, args.which, args.level, args.reps, prec)
     capi.set_kernel_variant(0)
     what = "K u" if args.which == 0 else "sweep"
-    print(f"{what} {args.prec} variant {args.variant}: {ms * 1e3:.1f} us per call", flush=True)
+    print(f"{what} {args.prec} level {args.level} variant {args.variant}: {ms * 1e3:.1f} us per call", flush=True)
     a = torch.ones(2**28, dtype=torch.float64, device="cuda:0")
     b = torch.zeros_like(a)
     b.copy_(a)
