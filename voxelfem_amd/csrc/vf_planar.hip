@@ -185,23 +185,50 @@ __global__ void __launch_bounds__(TPB) k_residual2(PGrid g, const double *__rest
     }
 }
 
+// computeResidual on a coarse level with four work-items per node (one per incident element, summed in
+// node_Ku's order by the first of the four, as in k_smooth2_x4): the same values as k_residual2<false>
+__global__ void __launch_bounds__(TPB) k_residual2_x4(PGrid g, const double *__restrict__ Ke,
+                                                     const uint8_t *__restrict__ dmask, const double *__restrict__ u,
+                                                     const double *__restrict__ b, double *__restrict__ r) {
+    const int tt = blockIdx.x * TPB + threadIdx.x, n = tt >> 2, p = tt & 3;
+    if (n >= g.nnodes) return;  // all four work-items of a node
+    const int ix = n / g.nn[1], iy = n % g.nn[1];
+    double v[2] = {0, 0};
+    if (iy < g.ydet) {
+        const int i = 1 - (p >> 1), j = 1 - (p & 1);
+        const int ex = ix - i, ey = iy - j;
+        if (ex >= 0 && ey >= 0 && ex < g.ne[0] && ey < g.ne[1])
+            elem_rows<false>(g, nullptr, nullptr, Ke, u, ex, ey, 2 * i + j, v, nullptr);
+    }
+    const int q0 = (int)(threadIdx.x & 63) & ~3;
+    double f[2];
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        double a = __shfl(v[k], q0, 64);
+        a += __shfl(v[k], q0 + 1, 64);
+        a += __shfl(v[k], q0 + 2, 64);
+        a += __shfl(v[k], q0 + 3, 64);
+        f[k] = a;
+    }
+    if (p) return;
+    const uint8_t dm = dmask[n];
+    if (iy < g.ydet) {
+        r[n] = (dm & 1) ? 0.0 : b[n] - f[0];
+        r[g.nnodes + n] = (dm & 2) ? 0.0 : b[g.nnodes + n] - f[1];
+    } else if (iy < g.ydet + 4) {
+        r[n] = (dm & 1) ? 0.0 : b[n];
+        r[g.nnodes + n] = (dm & 2) ? 0.0 : b[g.nnodes + n];
+    } else {
+        if (dm & 1) r[n] = 0.0;
+        if (dm & 2) r[g.nnodes + n] = 0.0;
+    }
+}
+
 // one colour of the multicolour block Gauss-Seidel sweep (MultigridSolver.hh:347-379, 408-458): nodes
 // (ox + 2 i, oy + 2 j) of the non-detached rows; the 2 x 2 block solve for free nodes, the point sweep in
 // component order (reversed backwards) for partially constrained ones, fully constrained nodes kept
-template <bool FINE>
-__global__ void __launch_bounds__(TPB) k_smooth2(PGrid g, const double *__restrict__ K0, const double *__restrict__ E,
-                                                 const double *__restrict__ Ke, const uint8_t *__restrict__ dmask,
-                                                 double *__restrict__ u, const double *__restrict__ b, int ox, int oy,
-                                                 int forward) {
-    const int mx = (g.nn[0] - ox + 1) / 2, my = (g.ydet - oy + 1) / 2;
-    const int t = blockIdx.x * TPB + threadIdx.x;
-    if (my <= 0 || t >= mx * my) return;
-    const int ix = ox + 2 * (t / my), iy = oy + 2 * (t % my);
-    const int n = ix * g.nn[1] + iy;
-    const uint8_t dm = dmask[n];
-    if (dm == 3) return;
-    double f[2] = {0, 0}, M[4] = {0, 0, 0, 0};
-    node_Ku<FINE>(g, K0, E, Ke, u, ix, iy, f, M);
+__device__ __forceinline__ void gs_update2(const PGrid &g, uint8_t dm, const double f[2], const double M[4],
+                                           double *__restrict__ u, const double *__restrict__ b, int n, int forward) {
     const double r0 = b[n] - f[0], r1 = b[g.nnodes + n] - f[1];
     double d0, d1;
     if (dm) {
@@ -220,6 +247,54 @@ __global__ void __launch_bounds__(TPB) k_smooth2(PGrid g, const double *__restri
     }
     u[n] += d0;
     u[g.nnodes + n] += d1;
+}
+template <bool FINE>
+__global__ void __launch_bounds__(TPB) k_smooth2(PGrid g, const double *__restrict__ K0, const double *__restrict__ E,
+                                                 const double *__restrict__ Ke, const uint8_t *__restrict__ dmask,
+                                                 double *__restrict__ u, const double *__restrict__ b, int ox, int oy,
+                                                 int forward) {
+    const int mx = (g.nn[0] - ox + 1) / 2, my = (g.ydet - oy + 1) / 2;
+    const int t = blockIdx.x * TPB + threadIdx.x;
+    if (my <= 0 || t >= mx * my) return;
+    const int ix = ox + 2 * (t / my), iy = oy + 2 * (t % my);
+    const int n = ix * g.nn[1] + iy;
+    const uint8_t dm = dmask[n];
+    if (dm == 3) return;
+    double f[2] = {0, 0}, M[4] = {0, 0, 0, 0};
+    node_Ku<FINE>(g, K0, E, Ke, u, ix, iy, f, M);
+    gs_update2(g, dm, f, M, u, b, n, forward);
+}
+// The same colour step on a coarse level with four work-items per node, one per incident element in node_Ku's
+// order ((1,1), (1,0), (0,1), (0,0)): four times the work-items in flight and a quarter of the dependent gathers
+// each. The first of the four sums the contributions in that order (the coarse products carry no modulus, so
+// the sums equal node_Ku's bit for bit) and updates the node.
+__global__ void __launch_bounds__(TPB) k_smooth2_x4(PGrid g, const double *__restrict__ Ke,
+                                                    const uint8_t *__restrict__ dmask, double *__restrict__ u,
+                                                    const double *__restrict__ b, int ox, int oy, int forward) {
+    const int mx = (g.nn[0] - ox + 1) / 2, my = (g.ydet - oy + 1) / 2;
+    const int tt = blockIdx.x * TPB + threadIdx.x, t = tt >> 2, p = tt & 3;
+    if (my <= 0 || t >= mx * my) return;  // all four work-items of a node
+    const int ix = ox + 2 * (t / my), iy = oy + 2 * (t % my);
+    const int n = ix * g.nn[1] + iy;
+    const uint8_t dm = dmask[n];
+    if (dm == 3) return;  // all four work-items of a node
+    const int i = 1 - (p >> 1), j = 1 - (p & 1);
+    const int ex = ix - i, ey = iy - j;
+    double v[6] = {0, 0, 0, 0, 0, 0};  // f[2], M[4] of this element
+    if (ex >= 0 && ey >= 0 && ex < g.ne[0] && ey < g.ne[1])
+        elem_rows<false>(g, nullptr, nullptr, Ke, u, ex, ey, 2 * i + j, v, v + 2);
+    const int q0 = (int)(threadIdx.x & 63) & ~3;
+    double acc[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        double a = __shfl(v[k], q0, 64);
+        a += __shfl(v[k], q0 + 1, 64);
+        a += __shfl(v[k], q0 + 2, 64);
+        a += __shfl(v[k], q0 + 3, 64);
+        acc[k] = a;
+    }
+    if (p) return;
+    gs_update2(g, dm, acc, acc + 2, u, b, n, forward);
 }
 
 // restriction (MultigridSolver.hh:216-275): the transpose of the bilinear interpolation over the fine
@@ -1004,7 +1079,7 @@ void PMG::residual(int l, const double *u, const double *b, double *r) {
         hipLaunchKernelGGL(k_residual2<true>, dim3(nblk(g.nnodes)), dim3(TPB), 0, st(), g, sim->K0.data(),
                            sim->E.data(), nullptr, dm, u, b, r);
     else
-        hipLaunchKernelGGL(k_residual2<false>, dim3(nblk(g.nnodes)), dim3(TPB), 0, st(), g, nullptr, nullptr,
+        hipLaunchKernelGGL(k_residual2_x4, dim3(nblk(4 * (size_t)g.nnodes)), dim3(TPB), 0, st(), g,
                            lv[l]->Ke.data(), dm, u, b, r);
     check_launch();
 }
@@ -1020,8 +1095,8 @@ void PMG::smooth(int l, double *u, const double *b, bool forward) {
             hipLaunchKernelGGL(k_smooth2<true>, dim3(nblk(cnt)), dim3(TPB), 0, st(), g, sim->K0.data(), sim->E.data(),
                                nullptr, dm, u, b, ox, oy, forward ? 1 : 0);
         else
-            hipLaunchKernelGGL(k_smooth2<false>, dim3(nblk(cnt)), dim3(TPB), 0, st(), g, nullptr, nullptr,
-                               lv[l]->Ke.data(), dm, u, b, ox, oy, forward ? 1 : 0);
+            hipLaunchKernelGGL(k_smooth2_x4, dim3(nblk(4 * (size_t)cnt)), dim3(TPB), 0, st(), g, lv[l]->Ke.data(), dm,
+                               u, b, ox, oy, forward ? 1 : 0);
         check_launch();
     }
 }
